@@ -1,0 +1,394 @@
+"""CPU oracle pinned against the reference's own tests and against mathematics.
+
+Every check below fixes the oracle independently of its own formulas:
+known-answer vectors from the reference's tests (test/cones/*, test/kkt/*,
+test/solver/*), closed forms (projections onto cones), NT-scaling identities,
+dense FP64 linear algebra (numpy), brute-force active-set enumeration on tiny
+QPs, and the KKT optimality certificate.
+"""
+import itertools
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from conicxx_amd import problems as PR
+
+pytestmark = []
+
+
+# ----------------------------------------------------------------------------
+# cones: test/cones/test_second_order_cone.cpp, test_nonnegative_cone.cpp,
+#        test_cone_set.cpp
+# ----------------------------------------------------------------------------
+def test_soc_product_and_inverse_roundtrip(oracle):
+    u = np.array([3.0, 0.5, -0.2, 0.1])
+    v = np.array([2.0, 0.3, 0.4, -0.1])
+    w = oracle.soc_product(u, v)
+    exp = np.concatenate([[u @ v], u[0] * v[1:] + v[0] * u[1:]])
+    np.testing.assert_allclose(w, exp, atol=1e-13)
+    np.testing.assert_allclose(oracle.soc_inverse_product(u, w), v, atol=1e-10)
+
+
+def test_soc_identity_is_product_neutral(oracle):
+    u = np.array([2.0, 0.3, -0.1, 0.4])
+    np.testing.assert_allclose(oracle.soc_product(u, np.array([1.0, 0, 0, 0])), u, atol=1e-13)
+
+
+def test_soc_nt_identities_reference_point(oracle):
+    s = np.array([3.0, 1.0, 1.0])
+    z = np.array([2.0, 0.5, 0.5])
+    W, H = oracle.soc_scaling(s, z)
+    np.testing.assert_allclose(H @ z, s, atol=1e-10)
+    np.testing.assert_allclose(np.linalg.solve(W, s), W @ z, atol=1e-10)
+    assert np.linalg.norm(H - H.T) < 1e-12
+    x = np.array([0.4, -0.2, 0.7])
+    np.testing.assert_allclose(W @ (W @ x), H @ x, atol=1e-10)
+
+
+@pytest.mark.parametrize("d", [2, 3, 4, 7, 50])
+def test_soc_nt_identities_random_interior(oracle, d):
+    rng = np.random.default_rng(d)
+    J = np.diag([1.0] + [-1.0] * (d - 1))
+    for _ in range(20):
+        u = rng.standard_normal(d - 1) * rng.uniform(0.1, 10)
+        w_ = rng.standard_normal(d - 1) * rng.uniform(0.1, 10)
+        s = np.concatenate([[np.linalg.norm(u) * (1 + rng.uniform(1e-3, 1))], u])
+        z = np.concatenate([[np.linalg.norm(w_) * (1 + rng.uniform(1e-3, 1))], w_])
+        W, H = oracle.soc_scaling(s, z)
+        # defining properties of the NT scaling point: W symmetric, W z = W^-1 s,
+        # W J W = (det-scaled) J (W is a scaled hyperbolic reflection)
+        np.testing.assert_allclose(W, W.T, rtol=1e-12, atol=1e-12)
+        scale = max(1.0, np.abs(s).max())
+        np.testing.assert_allclose(H @ z, s, rtol=1e-8, atol=1e-9 * scale)
+        np.testing.assert_allclose(W @ z, np.linalg.solve(W, s), rtol=1e-8, atol=1e-9 * scale)
+        beta2 = np.sqrt((s @ J @ s) / (z @ J @ z))
+        np.testing.assert_allclose(W @ J @ W, beta2 * J, rtol=1e-7, atol=1e-7 * beta2)
+
+
+def test_soc_margin_and_max_step_kats(oracle):
+    mn, _ = oracle.cone_margins(0, 0, [3], np.array([3.0, 1.0, 1.0]))
+    assert abs(mn - (3.0 - np.sqrt(2.0))) < 1e-14
+    x = np.array([1.0, 0.0, 0.0])
+    assert abs(oracle.soc_max_step(x, np.array([-1.0, 0, 0]), 10.0) - 1.0) < 1e-12
+    assert abs(oracle.soc_max_step(x, np.array([1.0, 1.0, 0]), 10.0) - 10.0) < 1e-12
+    assert abs(oracle.soc_max_step(x, np.array([0.0, 1.0, 0]), 10.0) - 1.0) < 1e-12
+
+
+@pytest.mark.parametrize("d", [3, 5])
+def test_soc_max_step_brute_force(oracle, d):
+    """Largest alpha keeping x + alpha dx in the closed cone, by bisection."""
+    rng = np.random.default_rng(10 + d)
+
+    def inside(y):
+        return y[0] - np.linalg.norm(y[1:]) >= -1e-15 * max(1.0, abs(y[0]))
+
+    for _ in range(200):
+        u = rng.standard_normal(d - 1)
+        x = np.concatenate([[np.linalg.norm(u) + rng.uniform(0.01, 1)], u])
+        dx = rng.standard_normal(d) * rng.uniform(0.1, 10)
+        a = oracle.soc_max_step(x, dx, 5.0)
+        if a < 5.0:
+            assert inside(x + a * (1 - 1e-9) * dx)
+            assert not inside(x + a * (1 + 1e-6) * dx + 0.0)
+        else:
+            for t in np.linspace(0, 5.0, 51):
+                assert inside(x + t * dx)
+
+
+def test_nonneg_max_step_and_margins(oracle):
+    x = np.array([1.0, 2.0, 3.0])
+    dx = np.array([-2.0, -1.0, 5.0])
+    assert abs(oracle.cone_max_step(0, 3, [], x, dx, 10.0) - 0.5) < 1e-14
+    mn, ps = oracle.cone_margins(2, 2, [], np.array([100.0, -100.0, -1.0, 3.0]))
+    assert mn == -1.0 and ps == 0.0
+    x = np.array([1.0, 2.0, 3.0, 1.0, 1.0, 0.5, 0.4, 0.4])
+    mn, ps = oracle.cone_margins(0, 2, [3, 3], x)
+    m = [1.0, 3.0 - np.sqrt(2.0), 0.5 - np.sqrt(0.32)]
+    assert abs(mn - min(m)) < 1e-12
+    assert abs(ps - sum(max(v, 0) for v in m)) < 1e-12
+
+
+def test_cone_set_scaling_apply(oracle):
+    """test_cone_set.cpp:42-60: Hs z == s on cone blocks, 0 on the zero block."""
+    s = np.array([0.0, 1.0, 4.0, 3.0, 1.0, 1.0])
+    z = np.array([0.0, 2.0, 1.0, 2.0, 0.5, 0.5])
+    lam, Hz, Wz, Winv = oracle.cone_scaling_apply(1, 2, [3], s, z, z)
+    assert abs(Hz[0]) < 1e-14
+    np.testing.assert_allclose(Hz[1:], s[1:], atol=1e-10)
+    lam_s, _, _, Winv_s = oracle.cone_scaling_apply(1, 2, [3], s, z, s)
+    np.testing.assert_allclose(Winv_s[1:], lam[1:], atol=1e-10)  # W^-1 s == W z
+
+
+# ----------------------------------------------------------------------------
+# equilibration: test/kkt/test_equilibration.cpp
+# ----------------------------------------------------------------------------
+def _sp(t, shape):
+    r, c, v = zip(*t)
+    M = sp.csc_matrix((v, (r, c)), shape=shape)
+    M.sort_indices()
+    return M
+
+
+def test_equilibration_identity_when_disabled(oracle):
+    P = _sp([(0, 0, 1.0), (1, 1, 1.0)], (2, 2))
+    A = _sp([(0, 0, 1.0), (1, 1, 1.0)], (2, 2))
+    d, e, c = oracle.equilibrate(P, A, 0, 2, [], equilibrate=0)
+    assert np.all(d == 1) and np.all(e == 1) and c == 1.0
+
+
+def test_equilibration_roundtrip_preserves_kkt(oracle):
+    P = _sp([(0, 0, 1e6), (1, 1, 1e-4), (2, 2, 50.0)], (3, 3))
+    A = _sp([(0, 0, 1e5), (1, 1, 1e-3), (2, 2, 20.0), (3, 0, 1e5), (3, 1, 1e-3), (3, 2, 20.0)], (4, 3))
+    d, e, c = oracle.equilibrate(P, A, 1, 3, [])
+    assert np.all(d > 0) and np.all(e > 0) and c > 0
+    x = np.array([0.5, -1.5, 2.0])
+    z = np.array([0.2, -0.3, 0.1, 0.4])
+    Pd = P.toarray()
+    q = -(Pd @ x) - A.T @ z
+    b = np.array([1.0, 2, 3, 4])
+    s = b - A @ x
+    Ps = c * np.diag(d) @ Pd @ np.diag(d)
+    As = np.diag(e) @ A.toarray() @ np.diag(d)
+    xh, sh, zh = x / d, s * e, z * c / e
+    np.testing.assert_allclose(Ps @ xh + As.T @ zh + c * d * q, 0, atol=1e-8)
+    np.testing.assert_allclose(As @ xh + sh - e * b, 0, atol=1e-8)
+
+
+def test_equilibration_soc_block_uniform_and_ruiz_converges(oracle):
+    P = _sp([(0, 0, 1.0), (1, 1, 1.0), (2, 2, 1.0)], (3, 3))
+    A = _sp([(0, 0, 100.0), (1, 1, 1.0), (2, 2, 0.01)], (3, 3))
+    d, e, c = oracle.equilibrate(P, A, 0, 0, [3])
+    assert abs(e[0] - e[1]) < 1e-12 and abs(e[1] - e[2]) < 1e-12
+    # Ruiz iterations drive the column/row infinity norms of [P A'; A 0] towards 1
+    prob = PR.config0(seed=3)
+    d, e, c = oracle.equilibrate(prob.P, prob.A, prob.zero_dim, prob.nonneg_dim, prob.soc_dims,
+                                 equilibrate_max_iter=25)
+    Pf = prob.P_full().toarray()
+    Ad = prob.A.toarray()
+    Ps = np.diag(d) @ Pf @ np.diag(d)
+    As = np.diag(e) @ Ad @ np.diag(d)
+    col = np.maximum(np.abs(Ps).max(0), np.abs(As).max(0))
+    row = np.abs(As).max(1)
+    assert 0.5 < col.min() and col.max() < 2.0
+    k = prob.zero_dim + prob.nonneg_dim
+    assert 0.5 < row[:k].min() and row[:k].max() < 2.0
+    # rows of one SOC block share a single scale: the block's largest row goes to 1
+    blk = row[k:].reshape(-1, 3).max(1)
+    assert 0.5 < blk.min() and blk.max() < 2.0
+
+
+# ----------------------------------------------------------------------------
+# KKT: test/kkt/test_kkt_system.cpp (dense FP64 reference of the same matrix)
+# ----------------------------------------------------------------------------
+def _dense_K(P, A, Hs, rp=1e-8, ra=1e-8):
+    Pd = P.toarray()
+    Pd = np.triu(Pd) + np.triu(Pd, 1).T
+    n, m = Pd.shape[0], A.shape[0]
+    K = np.zeros((n + m, n + m))
+    K[:n, :n] = Pd + rp * np.eye(n)
+    K[:n, n:] = A.toarray().T
+    K[n:, :n] = A.toarray()
+    K[n:, n:] = -Hs - ra * np.eye(m)
+    return K
+
+
+def test_kkt_matches_dense_reference(oracle):
+    P = _sp([(0, 0, 2.0), (1, 1, 3.0)], (2, 2))
+    A = _sp([(0, 0, 1.0), (1, 1, 1.0)], (2, 2))
+    rhs = np.array([1.0, 2, 3, 4])
+    x, rel, ok = oracle.kkt_solve(P, A, 0, 2, [], rhs)
+    assert ok and rel < 1e-10
+    np.testing.assert_allclose(x, np.linalg.solve(_dense_K(P, A, np.eye(2)), rhs), atol=1e-8)
+    s = np.array([4.0, 9.0]); z = np.array([1.0, 1.0])
+    rhs = np.array([1.0, -1.0, 0.5, 2.0])
+    x, rel, ok = oracle.kkt_solve(P, A, 0, 2, [], rhs, s=s, z=z)
+    np.testing.assert_allclose(x, np.linalg.solve(_dense_K(P, A, np.diag(s / z)), rhs), atol=1e-8)
+
+
+def test_kkt_rank_deficient_equality_block(oracle):
+    P = _sp([(0, 0, 1.0), (1, 1, 1.0)], (2, 2))
+    A = _sp([(0, 0, 1.0), (0, 1, 1.0), (1, 0, 1.0), (1, 1, 1.0)], (2, 2))
+    x, rel, ok = oracle.kkt_solve(P, A, 2, 0, [], np.array([1.0, 1.0, 0.0, 0.0]))
+    assert ok and np.all(np.isfinite(x)) and rel < 1e-6
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_kkt_random_soc_vs_dense(oracle, seed):
+    rng = np.random.default_rng(seed)
+    prob = PR.config0(seed=seed, n=40, zero=4, nonneg=10, n_soc=4)
+    m = prob.m
+    s = np.concatenate([np.zeros(4), rng.uniform(0.1, 2, 10)] +
+                       [np.concatenate([[2.0], rng.uniform(-0.5, 0.5, 2)]) for _ in range(4)])
+    z = np.concatenate([rng.standard_normal(4), rng.uniform(0.1, 2, 10)] +
+                       [np.concatenate([[1.5], rng.uniform(-0.5, 0.5, 2)]) for _ in range(4)])
+    # independent Hs: nonneg s/z, SOC W^2 from the W that satisfies W z = W^-1 s
+    Hs = np.zeros((m, m))
+    Hs[4:14, 4:14] = np.diag(s[4:14] / z[4:14])
+    for k in range(4):
+        o = 14 + 3 * k
+        W, H = oracle.soc_scaling(s[o:o + 3], z[o:o + 3])
+        np.testing.assert_allclose(H @ z[o:o + 3], s[o:o + 3], atol=1e-10)
+        Hs[o:o + 3, o:o + 3] = H
+    rhs = rng.standard_normal(prob.n + m)
+    x, rel, ok = oracle.kkt_solve(prob.P, prob.A, 4, 10, [3] * 4, rhs, s=s, z=z)
+    assert ok and rel < 1e-10
+    xr = np.linalg.solve(_dense_K(prob.P, prob.A, Hs), rhs)
+    np.testing.assert_allclose(x, xr, rtol=1e-8, atol=1e-8 * np.abs(xr).max())
+
+
+# ----------------------------------------------------------------------------
+# end to end: test/solver/*.cpp known answers
+# ----------------------------------------------------------------------------
+@pytest.mark.parametrize("case", PR.kat_problems(), ids=lambda c: c[0])
+def test_solver_known_answers(oracle, case):
+    name, p, status, x, obj, tol = case
+    r = oracle.solve(p.P, p.q, p.A, p.b, p.zero_dim, p.nonneg_dim, p.soc_dims)
+    assert r["status"] == status
+    if x is not None:
+        np.testing.assert_allclose(r["x"], x, atol=tol)
+        cert = PR.kkt_certificate(p, r["x"], r["s"], r["z"])
+        assert cert["stationarity"] < tol and cert["feasibility"] < tol
+        assert cert["complementarity"] < tol
+        assert cert["margin_s"] > -tol and cert["margin_z"] > -tol
+    if obj is not None:
+        assert abs(r["objective"] - obj) < tol
+
+
+def _friction_projection(f, mu):
+    a = np.linalg.norm(f[1:])
+    if a <= mu * f[0]:
+        return f.copy()
+    if f[0] + mu * a <= 0:
+        return np.zeros(3)
+    f0 = (f[0] + mu * a) / (1 + mu * mu)
+    return np.concatenate([[f0], (mu * f0 / a) * f[1:]])
+
+
+def test_friction_demo_timesteps_closed_form(oracle):
+    """examples/friction_contact_demo.cpp: 4 warm-started timesteps of updateData(q).
+
+    A warm start reuses the previous homogeneous iterate with tau reset to 1
+    (solver_impl.cpp:374-383, 458-463); the solve then stops at the gap
+    tolerance on a less centred iterate, so x is pinned to 2e-4 there and to
+    1e-6 on a cold start of the same data.
+    """
+    s = oracle.OracleSolver()
+    p0 = PR.friction_demo_problem([5.0, 1.0, 1.0])
+    assert s.setup(p0.P, p0.q, p0.A, p0.b, 0, 0, [3])
+    for t in range(4):
+        f = np.array([5.0 - t, 1.0 + 2.0 * t, 1.0 + 0.5 * t])
+        assert s.update_data(q=-f)
+        r = s.solve()
+        assert r["status"] == 1
+        np.testing.assert_allclose(r["x"], _friction_projection(f, 0.5), atol=2e-4)
+        p = PR.friction_demo_problem(f)
+        rc = oracle.solve(p.P, p.q, p.A, p.b, 0, 0, [3])
+        assert rc["status"] == 1
+        np.testing.assert_allclose(rc["x"], _friction_projection(f, 0.5), atol=1e-6)
+
+
+def test_update_reuse_matches_fresh_and_closed_form(oracle):
+    """test/solver/test_update_reuse.cpp:32-73"""
+    targets = [np.array([1.0, 2.0]), np.array([-1.0, 3.0]), np.array([0.5, -0.5])]
+    p0 = PR.box_qp(targets[0])
+    reused = oracle.OracleSolver()
+    assert reused.setup(p0.P, p0.q, p0.A, p0.b, 0, 2, [])
+    for t, c in enumerate(targets):
+        p = PR.box_qp(c)
+        if t:
+            assert reused.update_data(q=p.q)
+        r1 = reused.solve()
+        r2 = oracle.solve(p.P, p.q, p.A, p.b, 0, 2, [])
+        assert r1["status"] == 1 and r2["status"] == 1
+        np.testing.assert_allclose(r1["x"], r2["x"], atol=1e-6)
+        np.testing.assert_allclose(r1["x"], np.maximum(c, 0), atol=1e-6)
+
+
+def test_update_pattern_mismatch_rejected(oracle):
+    """test/solver/test_update_reuse.cpp:75-89"""
+    p0 = PR.box_qp([1.0, 1.0])
+    s = oracle.OracleSolver()
+    assert s.setup(p0.P, p0.q, p0.A, p0.b, 0, 2, [])
+    A_bad = _sp([(0, 0, -1.0), (0, 1, 0.3), (1, 1, -1.0)], (2, 2))
+    assert not s.update_data(A=A_bad)
+    assert s.setup(p0.P, p0.q, A_bad, p0.b, 0, 2, [])
+    assert s.solve()["status"] == 1
+
+
+def test_failure_statuses(oracle):
+    p = PR.box_qp([1.0, 2.0])
+    r = oracle.solve(p.P, p.q, p.A, p.b, 0, 2, [], max_iter=1)
+    assert r["status"] == 4  # MaxIterations
+    q = p.q.copy(); q[0] = np.nan
+    r = oracle.solve(p.P, q, p.A, p.b, 0, 2, [])
+    assert r["status"] == 5  # NumericalError
+    s = oracle.OracleSolver()
+    assert s.solve()["status"] == 5  # solve() before setup()
+
+
+def _brute_force_qp(P, q, A, b):
+    """min 1/2 x'Px + q'x s.t. Ax <= b by enumerating active sets (P > 0)."""
+    n, m = len(q), len(b)
+    best = None
+    for k in range(m + 1):
+        for S in itertools.combinations(range(m), k):
+            S = list(S)
+            K = np.zeros((n + k, n + k))
+            K[:n, :n] = P
+            K[:n, n:] = A[S].T
+            K[n:, :n] = A[S]
+            try:
+                sol = np.linalg.solve(K, np.concatenate([-q, b[S]]))
+            except np.linalg.LinAlgError:
+                continue
+            x, zS = sol[:n], sol[n:]
+            if np.all(A @ x <= b + 1e-9) and np.all(zS >= -1e-9):
+                val = 0.5 * x @ P @ x + q @ x
+                if best is None or val < best[1]:
+                    best = (x, val)
+    return best
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_random_small_qp_brute_force(oracle, seed):
+    rng = np.random.default_rng(100 + seed)
+    n, m = 4, 7
+    M = rng.standard_normal((n, n))
+    Pd = M @ M.T + 0.1 * np.eye(n)
+    q = rng.standard_normal(n)
+    A = rng.standard_normal((m, n))
+    b = rng.uniform(0.1, 1.0, m)  # x = 0 strictly feasible
+    x_bf, val = _brute_force_qp(Pd, q, A, b)
+    P = sp.csc_matrix(np.triu(Pd)); P.sort_indices()
+    As = sp.csc_matrix(A); As.sort_indices()
+    r = oracle.solve(P, q, As, b, 0, m, [])
+    assert r["status"] == 1
+    np.testing.assert_allclose(r["x"], x_bf, atol=1e-6)
+    assert abs(r["objective"] - val) < 1e-6 * max(1, abs(val))
+
+
+@pytest.mark.parametrize("maker", [lambda: PR.config0(seed=0), lambda: PR.config0(seed=1),
+                                   lambda: PR.config2_problem(0), lambda: PR.config2_problem(7),
+                                   lambda: PR.config3_problem(0)],
+                         ids=["c0s0", "c0s1", "c2p0", "c2p7", "c3p0"])
+def test_random_instances_certificate(oracle, maker):
+    p = maker()
+    r = oracle.solve(p.P, p.q, p.A, p.b, p.zero_dim, p.nonneg_dim, p.soc_dims)
+    assert r["status"] == 1
+    c = PR.kkt_certificate(p, r["x"], r["s"], r["z"])
+    scale = 1 + np.abs(p.q).max() + np.abs(p.b).max()
+    assert c["stationarity"] < 1e-6 * scale and c["feasibility"] < 1e-6 * scale
+    assert c["margin_s"] > -1e-7 and c["margin_z"] > -1e-7
+
+
+def test_regularized_backend_matches_default(oracle):
+    p = PR.config0(seed=2)
+    r1 = oracle.solve(p.P, p.q, p.A, p.b, p.zero_dim, p.nonneg_dim, p.soc_dims)
+    r2 = oracle.solve(p.P, p.q, p.A, p.b, p.zero_dim, p.nonneg_dim, p.soc_dims, linear_solver=2)
+    assert r1["status"] == r2["status"] == 1
+    # different pivot handling -> different iteration path; both optimal
+    for r in (r1, r2):
+        c = PR.kkt_certificate(p, r["x"], r["s"], r["z"])
+        assert c["stationarity"] < 1e-6 and c["feasibility"] < 1e-6
+    np.testing.assert_allclose(r1["x"], r2["x"], rtol=1e-4, atol=1e-4)
