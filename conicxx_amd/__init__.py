@@ -1,0 +1,7 @@
+"""MI355X-native conic interior-point solver (gfx950 HIP kernels behind a C-ABI).
+
+The public Python surface mirrors the reference's Solver / Settings /
+ConeSpec / Solution interface; see conicxx_amd.solver.
+"""
+from .solver import (ConeSpec, FactorPrecision, Info, LinearSolverBackend, Settings, Solution,  # noqa: F401
+                     Solver, Status, analyze_pattern, toString)

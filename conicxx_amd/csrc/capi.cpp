@@ -1,0 +1,192 @@
+// C-ABI of the solver (declared in include/conicxx_c.h).  No exception
+// crosses this boundary: HIP or allocation failures become return codes or
+// a NumericalError status, as the reference's Solver never throws
+// (include/conicxx/solver.h:43-47, 86-89).
+#include <cstdio>
+#include <exception>
+#include <new>
+
+#include "../../include/conicxx_c.h"
+#include "solver.hpp"
+#include "symbolic.hpp"
+
+extern "C" {
+
+struct cx_solver {
+  cx::Solver* impl;
+};
+
+void cx_default_settings(cx_settings* s) {
+  if (!s) return;
+  s->tol_feas = 1e-8;
+  s->tol_gap_abs = 1e-8;
+  s->tol_gap_rel = 1e-8;
+  s->tol_infeas = 1e-7;
+  s->max_iter = 200;
+  s->static_reg_P = 1e-8;
+  s->static_reg_A = 1e-8;
+  s->dynamic_reg_eps = 1e-14;
+  s->dynamic_reg_delta = 1e-7;
+  s->refine_max_iter = 3;
+  s->refine_tol = 1e-12;
+  s->max_step_fraction = 0.99;
+  s->equilibrate = 1;
+  s->equilibrate_max_iter = 10;
+  s->equilibrate_min_scale = 1e-4;
+  s->equilibrate_max_scale = 1e4;
+  s->warm_start = 1;
+  s->verbose = 0;
+  s->record_timings = 0;
+  s->validate_inputs = 1;
+  s->linear_solver = CX_LS_QDLDL;
+  s->factor_precision = CX_FACTOR_FP64;
+}
+
+cx_solver* cx_create(void) {
+  try {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return nullptr;
+    cx_solver* h = new cx_solver;
+    h->impl = new cx::Solver();
+    return h;
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "[conicxx_amd] cx_create failed: %s\n", e.what());
+    return nullptr;
+  }
+}
+
+void cx_destroy(cx_solver* h) {
+  if (!h) return;
+  try {
+    delete h->impl;
+  } catch (...) {
+  }
+  delete h;
+}
+
+int cx_setup(cx_solver* h, int n, int m, const int* P_colptr, const int* P_rowind,
+             const double* P_val, const double* q, const int* A_colptr, const int* A_rowind,
+             const double* A_val, const double* b, int zero_dim, int nonneg_dim, int n_soc,
+             const int* soc_dims, const cx_settings* settings) {
+  if (!h) return 0;
+  try {
+    cx_settings st;
+    if (settings)
+      st = *settings;
+    else
+      cx_default_settings(&st);
+    return h->impl->setup(n, m, P_colptr, P_rowind, P_val, q, A_colptr, A_rowind, A_val, b, zero_dim,
+                          nonneg_dim, n_soc, soc_dims, st)
+               ? 1
+               : 0;
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "[conicxx_amd] setup failed: %s\n", e.what());
+    return 0;
+  }
+}
+
+int cx_update_data(cx_solver* h, const int* P_colptr, const int* P_rowind, const double* P_val,
+                   const double* q, const int* A_colptr, const int* A_rowind, const double* A_val,
+                   const double* b) {
+  if (!h) return 0;
+  try {
+    return h->impl->update_data(P_colptr, P_rowind, P_val, q, A_colptr, A_rowind, A_val, b) ? 1 : 0;
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "[conicxx_amd] updateData failed: %s\n", e.what());
+    return 0;
+  }
+}
+
+void cx_set_warm_start(cx_solver* h, const double* x, const double* s, const double* z) {
+  if (!h) return;
+  try {
+    h->impl->set_warm_start(x, s, z);
+  } catch (...) {
+  }
+}
+
+void cx_set_settings(cx_solver* h, const cx_settings* settings) {
+  if (h && settings) h->impl->set_settings(*settings);
+}
+
+void cx_get_settings(const cx_solver* h, cx_settings* out) {
+  if (h && out) *out = h->impl->settings();
+}
+
+int cx_solve(cx_solver* h) {
+  if (!h) return CX_NUMERICAL_ERROR;
+  try {
+    return h->impl->solve();
+  } catch (...) {
+    return CX_NUMERICAL_ERROR;
+  }
+}
+
+int cx_get_solution(const cx_solver* h, double* x, double* s, double* z, double* objective,
+                    cx_info* info) {
+  if (!h) return CX_UNSOLVED;
+  return h->impl->get_solution(x, s, z, objective, info);
+}
+
+const char* cx_status_string(int status) {
+  switch (status) {
+    case CX_UNSOLVED: return "Unsolved";
+    case CX_SOLVED: return "Solved";
+    case CX_PRIMAL_INFEASIBLE: return "PrimalInfeasible";
+    case CX_DUAL_INFEASIBLE: return "DualInfeasible";
+    case CX_MAX_ITERATIONS: return "MaxIterations";
+    case CX_NUMERICAL_ERROR: return "NumericalError";
+  }
+  return "Unknown";
+}
+
+int cx_get_dims(const cx_solver* h, int* n, int* m) {
+  if (!h || !n || !m) return 0;
+  h->impl->dims(n, m);
+  return 1;
+}
+
+void cx_get_kkt_stats(const cx_solver* h, double* stats8) {
+  if (h && stats8) h->impl->kkt_stats(stats8);
+}
+
+void cx_get_profile(const cx_solver* h, double* out8) {
+  if (h && out8) h->impl->profile(out8);
+}
+
+int cx_analyze_pattern(int N, const int* colptr, const int* rowind, int* perm, int* colcount,
+                       double* stats8) {
+  try {
+    cx::LowerPattern K;
+    K.N = N;
+    K.colptr.assign(colptr, colptr + N + 1);
+    K.rowind.assign(rowind, rowind + colptr[N]);
+    cx::SupernodalPlan P = cx::plan_supernodes(K);
+    if (perm)
+      for (int k = 0; k < N; ++k) perm[k] = P.perm[k];
+    if (colcount) {
+      // exact counts of the final (postordered) order
+      std::vector<int> par = cx::etree_of(K, P.perm);
+      std::vector<int> ident(N);
+      for (int k = 0; k < N; ++k) ident[k] = k;
+      std::vector<int> cc = cx::column_counts(K, P.perm, par, ident);
+      for (int k = 0; k < N; ++k) colcount[k] = cc[k];
+    }
+    if (stats8) {
+      stats8[0] = P.N;
+      stats8[1] = K.colptr[N];
+      stats8[2] = P.ns;
+      stats8[3] = P.nlevels;
+      stats8[4] = P.nnzL;
+      stats8[5] = P.flops;
+      stats8[6] = P.max_front;
+      stats8[7] = P.nnzL_exact;
+    }
+    return 1;
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "[conicxx_amd] analyze failed: %s\n", e.what());
+    return 0;
+  }
+}
+
+}  // extern "C"

@@ -1,0 +1,735 @@
+// Device kernels of the interior-point iteration.
+//
+// Cone algebra follows src/conicxx/cones/*.cpp.  The SOC Nesterov-Todd
+// scaling is kept in its rank-one form W = beta (2 v v' - J) (the matrix the
+// reference builds densely at second_order_cone.cpp:133-136), so
+//   W x      = beta (2 v (v'x) - J x)
+//   W^-1 x   = (1/beta) (2 Jv ((Jv)'x) - J x)       (v'Jv = 1)
+//   Hs x     = beta^2 (4 (v'v) v (v'x) - 2 v ((Jv)'x) - 2 Jv (v'x) + x)
+// with the reference's identity fallback when W is non-finite or
+// max|W_ij| > 1e8 (second_order_cone.cpp:152-156).  Small SOC blocks are
+// handled one per thread, large ones one per 64-lane wave with shuffle
+// reductions.  Reductions are two-stage and deterministic.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+
+#include "common.hpp"
+#include "ipm_kernels.hpp"
+
+namespace cx {
+
+namespace {
+
+constexpr double kTiny = 1e-12;
+constexpr double kAbsFloor = DBL_MIN * 1e4;
+constexpr double kStepEps = DBL_MIN * 1e4;
+
+template <int T>
+__device__ inline double tsum(double v) {
+  if constexpr (T > 1) {
+#pragma unroll
+    for (int o = T / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  }
+  return v;
+}
+template <int T>
+__device__ inline double tmin(double v) {
+  if constexpr (T > 1) {
+#pragma unroll
+    for (int o = T / 2; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+  }
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// block-level reduction helpers (256 threads)
+// ---------------------------------------------------------------------------
+__device__ inline double block_sum(double v, double* sh) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[wv] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0) r = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+  return r;
+}
+__device__ inline double block_min(double v, double* sh) {
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[wv] = v;
+  __syncthreads();
+  double r = INFINITY;
+  if (threadIdx.x == 0) r = fmin(fmin(sh[0], sh[1]), fmin(sh[2], sh[3]));
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// multi-term reductions
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_reduce_partial(RedBatch B, double* partial) {
+  __shared__ double sh[4];
+  for (int o = 0; o < B.nops; ++o) {
+    const RedOp op = B.op[o];
+    double acc = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < op.len; i += gridDim.x * blockDim.x) {
+      if (op.kind == RED_DOT) {
+        acc += op.a[i] * op.b[i];
+      } else if (op.kind == RED_DOT_AXPY) {
+        acc += (op.a[i] + op.alpha * op.c[i]) * (op.b[i] + op.alpha * op.d[i]);
+      } else {
+        acc += isfinite(op.a[i]) ? 0.0 : 1.0;
+      }
+    }
+    double r = block_sum(acc, sh);
+    if (threadIdx.x == 0) partial[o * RED_BLOCKS + blockIdx.x] = r;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_reduce_final(const double* partial, int nops, double* out) {
+  __shared__ double sh[4];
+  for (int o = 0; o < nops; ++o) {
+    double v = (threadIdx.x < RED_BLOCKS) ? partial[o * RED_BLOCKS + threadIdx.x] : 0.0;
+    double r = block_sum(v, sh);
+    if (threadIdx.x == 0) out[o] = r;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// SOC helpers with team size T (1: one thread per block, 64: one wave per block)
+// ---------------------------------------------------------------------------
+struct SocState {
+  double beta, vv;
+  int ident;
+};
+
+__device__ inline double jsign(int i) { return i == 0 ? 1.0 : -1.0; }
+
+// (W x)_i given v'x
+__device__ inline double w_apply(const SocState& S, const double* v, int i, double xi, double vx) {
+  if (S.ident) return xi;
+  return S.beta * (2.0 * v[i] * vx - jsign(i) * xi);
+}
+// (W^-1 x)_i given (Jv)'x
+__device__ inline double winv_apply(const SocState& S, const double* v, int i, double xi, double jvx) {
+  if (S.ident) return xi;
+  return (2.0 * jsign(i) * v[i] * jvx - jsign(i) * xi) / S.beta;
+}
+
+template <int T>
+__device__ void soc_scaling(int lane, int off, int d, const double* s, const double* z, double* v,
+                            double* beta_o, double* vv_o, int* ident_o) {
+  const double s0 = s[off], z0 = z[off];
+  double ps1 = 0.0, pz1 = 0.0, psz = 0.0;
+  for (int i = 1 + lane; i < d; i += T) {
+    const double si = s[off + i], zi = z[off + i];
+    ps1 += si * si;
+    pz1 += zi * zi;
+    psz += si * zi;
+  }
+  ps1 = tsum<T>(ps1);
+  pz1 = tsum<T>(pz1);
+  psz = tsum<T>(psz);
+  const double a = sqrt(fmax(fmax(s0 * s0 - ps1, 1e-12 * s0 * s0), kAbsFloor));
+  const double b = sqrt(fmax(fmax(z0 * z0 - pz1, 1e-12 * z0 * z0), kAbsFloor));
+  const double beta = fmin(fmax(sqrt(a / b), 1e-8), 1e8);
+  const double sz = s0 * z0 + psz;
+  const double c = sqrt(fmax((sz / (a * b) + 1.0) / 2.0, kTiny));
+  double v0 = (z0 / b + s0 / a) / (2.0 * c);
+  v0 += 1.0;
+  v0 = fmax(v0, kTiny);
+  const double sc = sqrt(2.0 * v0);
+  const double v0n = v0 / sc;
+  // v tail, v'v, finiteness, two largest |v_i| (i >= 1)
+  double pvv = 0.0, m1 = 0.0, m2 = 0.0, dmax = 0.0;
+  bool fin = true;
+  for (int i = 1 + lane; i < d; i += T) {
+    double vi = ((-z[off + i] / b) + s[off + i] / a) / (2.0 * c);
+    vi = vi / sc;
+    v[off + i] = vi;
+    pvv += vi * vi;
+    const double av = fabs(vi);
+    if (!isfinite(vi)) fin = false;
+    if (av > m1) m2 = m1, m1 = av;
+    else if (av > m2) m2 = av;
+    dmax = fmax(dmax, fabs(2.0 * vi * vi + 1.0));
+  }
+  pvv = tsum<T>(pvv);
+  dmax = fmax(-tmin<T>(-dmax), 0.0);
+  // merge top-two across the team
+  if constexpr (T > 1) {
+#pragma unroll
+    for (int o = T / 2; o > 0; o >>= 1) {
+      double b1 = __shfl_xor(m1, o, 64), b2 = __shfl_xor(m2, o, 64);
+      double hi = fmax(m1, b1);
+      double lo = fmax(fmin(m1, b1), fmax(m2, b2));
+      m1 = hi;
+      m2 = lo;
+    }
+  }
+  int nonfin = fin ? 0 : 1;
+  if constexpr (T > 1) {
+#pragma unroll
+    for (int o = T / 2; o > 0; o >>= 1) nonfin |= __shfl_xor(nonfin, o, 64);
+  }
+  const double vv = v0n * v0n + pvv;
+  // max |W_ij| / beta over diagonal and off-diagonal entries
+  double wdiag = fmax(fabs(2.0 * v0n * v0n - 1.0), dmax);
+  double off01 = 2.0 * fabs(v0n) * m1;  // (0, i) entries
+  double offii = 2.0 * m1 * m2;         // (i, j) entries, i != j >= 1
+  double wmax = beta * fmax(wdiag, fmax(off01, offii));
+  bool bad = nonfin || !isfinite(v0n) || !isfinite(beta) || !isfinite(wmax) || wmax > 1e8;
+  if (lane == 0) {
+    v[off] = v0n;
+    *beta_o = beta;
+    *vv_o = vv;
+    *ident_o = bad ? 1 : 0;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_soc_scaling_small(ConeDev C, const double* s, const double* z) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= C.nsmall) return;
+  int k = C.small_list[t];
+  soc_scaling<1>(0, C.soc_off[k], C.soc_dim[k], s, z, C.v, &C.beta[k], &C.vv[k], &C.ident[k]);
+}
+__global__ __launch_bounds__(256) void k_soc_scaling_large(ConeDev C, const double* s, const double* z) {
+  int t = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  int lane = threadIdx.x & 63;
+  if (t >= C.nlarge) return;
+  int k = C.large_list[t];
+  soc_scaling<64>(lane, C.soc_off[k], C.soc_dim[k], s, z, C.v, &C.beta[k], &C.vv[k], &C.ident[k]);
+}
+__global__ void k_nn_scaling(ConeDev C, const double* s, const double* z) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= C.nn_dim) return;
+  int r = C.nn_off + i;
+  double w = sqrt(s[r] / z[r]);
+  C.w[r] = w;
+  C.hsd[r] = w * w;
+}
+
+// out = -(Hs x + y)
+template <int T>
+__device__ void soc_neg_hs_plus(int lane, const ConeDev& C, int k, const double* x, const double* y,
+                                double* out) {
+  const int off = C.soc_off[k], d = C.soc_dim[k];
+  SocState S{C.beta[k], C.vv[k], C.ident[k]};
+  const double* v = C.v + off;
+  double vx = 0.0, jvx = 0.0;
+  for (int i = lane; i < d; i += T) {
+    vx += v[i] * x[off + i];
+    jvx += jsign(i) * v[i] * x[off + i];
+  }
+  vx = tsum<T>(vx);
+  jvx = tsum<T>(jvx);
+  const double b2 = S.beta * S.beta;
+  for (int i = lane; i < d; i += T) {
+    double h;
+    if (S.ident)
+      h = x[off + i];
+    else
+      h = b2 * (4.0 * S.vv * v[i] * vx - 2.0 * v[i] * jvx - 2.0 * jsign(i) * v[i] * vx + x[off + i]);
+    out[off + i] = -(h + y[off + i]);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_neg_hs_plus(ConeDev C, int m, const double* x,
+                                                     const double* y, double* out) {
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  // zero and nonnegative rows elementwise
+  for (int i = tid; i < C.zero_dim + C.nn_dim; i += gridDim.x * blockDim.x) {
+    if (i < C.zero_dim) {
+      int r = C.zero_off + i;
+      out[r] = -(0.0 + y[r]);
+    } else {
+      int r = C.nn_off + (i - C.zero_dim);
+      out[r] = -(C.hsd[r] * x[r] + y[r]);
+    }
+  }
+  for (int t = tid; t < C.nsmall; t += gridDim.x * blockDim.x) soc_neg_hs_plus<1>(0, C, C.small_list[t], x, y, out);
+  const int wave = tid >> 6, lane = threadIdx.x & 63, nw = (gridDim.x * blockDim.x) >> 6;
+  for (int t = wave; t < C.nlarge; t += nw) soc_neg_hs_plus<64>(lane, C, C.large_list[t], x, y, out);
+}
+
+// combined-step cone rhs (solver_impl.cpp:233-251) for one SOC block
+template <int T>
+__device__ void soc_combined(int lane, const ConeDev& C, int k, const double* z, const double* dsa,
+                             const double* dza, const double* rz, double smu, double oms,
+                             double* dsc_out, double* tail) {
+  const int off = C.soc_off[k], d = C.soc_dim[k];
+  SocState S{C.beta[k], C.vv[k], C.ident[k]};
+  const double* v = C.v + off;
+  // pass 1: v'z, (Jv)'ds_aff, v'dz_aff
+  double vz = 0.0, jvds = 0.0, vdz = 0.0;
+  for (int i = lane; i < d; i += T) {
+    vz += v[i] * z[off + i];
+    jvds += jsign(i) * v[i] * dsa[off + i];
+    vdz += v[i] * dza[off + i];
+  }
+  vz = tsum<T>(vz);
+  jvds = tsum<T>(jvds);
+  vdz = tsum<T>(vdz);
+  // element values: lambda = W z, a1 = W^-1 ds_aff, a2 = W dz_aff
+  const double l0 = w_apply(S, v, 0, z[off], vz);
+  const double a10 = winv_apply(S, v, 0, dsa[off], jvds);
+  const double a20 = w_apply(S, v, 0, dza[off], vdz);
+  // pass 2: lambda'lambda, a1'a2, ||lambda_1||^2
+  double lam2 = 0.0, a12 = 0.0, r1 = 0.0;
+  for (int i = lane; i < d; i += T) {
+    double li = w_apply(S, v, i, z[off + i], vz);
+    double a1 = winv_apply(S, v, i, dsa[off + i], jvds);
+    double a2 = w_apply(S, v, i, dza[off + i], vdz);
+    lam2 += li * li;
+    a12 += a1 * a2;
+    if (i > 0) r1 += li * li;
+  }
+  lam2 = tsum<T>(lam2);
+  a12 = tsum<T>(a12);
+  r1 = tsum<T>(r1);
+  const double dsc0 = lam2 + a12 - smu;
+  // pass 3: nu = lambda_1' dsc_1
+  double nu = 0.0;
+  for (int i = 1 + lane; i < d; i += T) {
+    double li = w_apply(S, v, i, z[off + i], vz);
+    double a1 = winv_apply(S, v, i, dsa[off + i], jvds);
+    double a2 = w_apply(S, v, i, dza[off + i], vdz);
+    double dsci = (l0 * li + li * l0) + (a10 * a2 + a20 * a1);
+    nu += li * dsci;
+  }
+  nu = tsum<T>(nu);
+  // inverse product lambda \ dsc (second_order_cone.cpp:50-68)
+  double rho = l0 * l0 - r1;
+  if (fabs(rho) < kTiny) rho = (rho >= 0 ? kTiny : -kTiny);
+  double u0 = l0;
+  if (fabs(u0) < kTiny) u0 = kTiny;
+  const double lid0 = (u0 * dsc0 - nu) / rho;
+  // pass 4: v' lid  (W lid needs it)
+  double vlid = 0.0;
+  for (int i = lane; i < d; i += T) {
+    double lidi;
+    if (i == 0) {
+      lidi = lid0;
+    } else {
+      double li = w_apply(S, v, i, z[off + i], vz);
+      double a1 = winv_apply(S, v, i, dsa[off + i], jvds);
+      double a2 = w_apply(S, v, i, dza[off + i], vdz);
+      double dsci = (l0 * li + li * l0) + (a10 * a2 + a20 * a1);
+      lidi = ((nu / u0 - dsc0) * li + (rho / u0) * dsci) / rho;
+    }
+    vlid += v[i] * lidi;
+  }
+  vlid = tsum<T>(vlid);
+  // pass 5: ds_const = W lid
+  for (int i = lane; i < d; i += T) {
+    double lidi;
+    if (i == 0) {
+      lidi = lid0;
+    } else {
+      double li = w_apply(S, v, i, z[off + i], vz);
+      double a1 = winv_apply(S, v, i, dsa[off + i], jvds);
+      double a2 = w_apply(S, v, i, dza[off + i], vdz);
+      double dsci = (l0 * li + li * l0) + (a10 * a2 + a20 * a1);
+      lidi = ((nu / u0 - dsc0) * li + (rho / u0) * dsci) / rho;
+    }
+    double dsi = w_apply(S, v, i, lidi, vlid);
+    dsc_out[off + i] = dsi;
+    tail[off + i] = dsi - oms * rz[off + i];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_combined_rhs(ConeDev C, const double* z, const double* dsa,
+                                                      const double* dza, const double* rz,
+                                                      double smu, double oms, double* dsc,
+                                                      double* tail) {
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  for (int i = tid; i < C.zero_dim + C.nn_dim; i += gridDim.x * blockDim.x) {
+    if (i < C.zero_dim) {
+      int r = C.zero_off + i;
+      dsc[r] = 0.0;
+      tail[r] = 0.0 - oms * rz[r];
+    } else {
+      int r = C.nn_off + (i - C.zero_dim);
+      double w = C.w[r];
+      double lam = w * z[r];
+      double a1 = dsa[r] / w;
+      double a2 = w * dza[r];
+      double dc = lam * lam + a1 * a2 - smu;
+      double lid = dc / lam;
+      double ds = w * lid;
+      dsc[r] = ds;
+      tail[r] = ds - oms * rz[r];
+    }
+  }
+  for (int t = tid; t < C.nsmall; t += gridDim.x * blockDim.x)
+    soc_combined<1>(0, C, C.small_list[t], z, dsa, dza, rz, smu, oms, dsc, tail);
+  const int wave = tid >> 6, lane = threadIdx.x & 63, nw = (gridDim.x * blockDim.x) >> 6;
+  for (int t = wave; t < C.nlarge; t += nw)
+    soc_combined<64>(lane, C, C.large_list[t], z, dsa, dza, rz, smu, oms, dsc, tail);
+}
+
+// SecondOrderCone::maxStep (second_order_cone.cpp:174-215), alpha_max = +inf
+template <int T>
+__device__ double soc_max_step(int lane, int off, int d, const double* x, const double* dx) {
+  double pd = 0.0, dd = 0.0, pp = 0.0;
+  for (int i = 1 + lane; i < d; i += T) {
+    pd += x[off + i] * dx[off + i];
+    dd += dx[off + i] * dx[off + i];
+    pp += x[off + i] * x[off + i];
+  }
+  pd = tsum<T>(pd);
+  dd = tsum<T>(dd);
+  pp = tsum<T>(pp);
+  const double p0 = x[off], d0 = dx[off];
+  const double a = d0 * d0 - dd;
+  const double b = 2.0 * (p0 * d0 - pd);
+  const double c = p0 * p0 - pp;
+  double alpha = INFINITY;
+  auto feas = [&](double r) { return r > 0 && (p0 + r * d0) >= 0; };
+  if (fabs(a) < kStepEps) {
+    if (fabs(b) > kStepEps) {
+      double r = -c / b;
+      if (feas(r)) alpha = fmin(alpha, r);
+    }
+    return alpha;
+  }
+  double disc = b * b - 4.0 * a * c;
+  if (disc < 0) return alpha;
+  double sq = sqrt(disc);
+  double q = (b >= 0) ? -0.5 * (b + sq) : -0.5 * (b - sq);
+  double r1, r2;
+  if (fabs(q) > kStepEps) {
+    r1 = q / a;
+    r2 = c / q;
+  } else {
+    r1 = r2 = -b / (2.0 * a);
+  }
+  if (feas(r1)) alpha = fmin(alpha, r1);
+  if (feas(r2)) alpha = fmin(alpha, r2);
+  return alpha;
+}
+
+__global__ __launch_bounds__(256) void k_max_step(ConeDev C, const double* x, const double* dx,
+                                                  double* partial) {
+  __shared__ double sh[4];
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  double a = INFINITY;
+  for (int i = tid; i < C.nn_dim; i += gridDim.x * blockDim.x) {
+    int r = C.nn_off + i;
+    if (dx[r] < 0) a = fmin(a, -x[r] / dx[r]);
+  }
+  for (int t = tid; t < C.nsmall; t += gridDim.x * blockDim.x) {
+    int k = C.small_list[t];
+    a = fmin(a, soc_max_step<1>(0, C.soc_off[k], C.soc_dim[k], x, dx));
+  }
+  const int wave = tid >> 6, lane = threadIdx.x & 63, nw = (gridDim.x * blockDim.x) >> 6;
+  for (int t = wave; t < C.nlarge; t += nw) {
+    int k = C.large_list[t];
+    a = fmin(a, soc_max_step<64>(lane, C.soc_off[k], C.soc_dim[k], x, dx));
+  }
+  double r = block_min(a, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = r;
+}
+
+__global__ __launch_bounds__(256) void k_min_final(const double* partial, int n, double* out) {
+  __shared__ double sh[4];
+  double v = INFINITY;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) v = fmin(v, partial[i]);
+  double r = block_min(v, sh);
+  if (threadIdx.x == 0) out[0] = r;
+}
+
+// margins: partial[0..G) = nonneg min, [G..2G) = SOC min, [2G..3G) = SOC sum of positive parts
+__global__ __launch_bounds__(256) void k_margins(ConeDev C, const double* x, double* partial) {
+  __shared__ double sh[4];
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  double nmin = INFINITY, smin = INFINITY, spos = 0.0;
+  for (int i = tid; i < C.nn_dim; i += gridDim.x * blockDim.x) nmin = fmin(nmin, x[C.nn_off + i]);
+  for (int t = tid; t < C.nsmall; t += gridDim.x * blockDim.x) {
+    int k = C.small_list[t], off = C.soc_off[k], d = C.soc_dim[k];
+    double ss = 0.0;
+    for (int i = 1; i < d; ++i) ss += x[off + i] * x[off + i];
+    double mg = x[off] - sqrt(ss);
+    if (isfinite(mg)) smin = fmin(smin, mg), spos += fmax(mg, 0.0);
+  }
+  const int wave = tid >> 6, lane = threadIdx.x & 63, nw = (gridDim.x * blockDim.x) >> 6;
+  for (int t = wave; t < C.nlarge; t += nw) {
+    int k = C.large_list[t], off = C.soc_off[k], d = C.soc_dim[k];
+    double ss = 0.0;
+    for (int i = 1 + lane; i < d; i += 64) ss += x[off + i] * x[off + i];
+    ss = tsum<64>(ss);
+    double mg = x[off] - sqrt(ss);
+    if (lane == 0 && isfinite(mg)) smin = fmin(smin, mg), spos += fmax(mg, 0.0);
+  }
+  double a = block_min(nmin, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = a;
+  double b = block_min(smin, sh);
+  if (threadIdx.x == 0) partial[gridDim.x + blockIdx.x] = b;
+  double c = block_sum(spos, sh);
+  if (threadIdx.x == 0) partial[2 * gridDim.x + blockIdx.x] = c;
+}
+
+__global__ __launch_bounds__(256) void k_margins_final(const double* partial, int G, int has_nn,
+                                                       double* out) {
+  __shared__ double sh[4];
+  double a = INFINITY, b = INFINITY, c = 0.0;
+  for (int i = threadIdx.x; i < G; i += blockDim.x) {
+    a = fmin(a, partial[i]);
+    b = fmin(b, partial[G + i]);
+    c += partial[2 * G + i];
+  }
+  double nmin = block_min(a, sh);
+  double smin = block_min(b, sh);
+  double spos = block_sum(c, sh);
+  if (threadIdx.x == 0) {
+    double mn = smin, ps = spos;
+    if (has_nn && isfinite(nmin)) {
+      mn = fmin(mn, nmin);
+      ps += fmax(nmin, 0.0);
+    }
+    if (!isfinite(mn)) mn = 0.0;
+    out[0] = mn;
+    out[1] = ps;
+  }
+}
+
+__global__ void k_unit_shift(ConeDev C, double* x, double alpha) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < C.nn_dim) x[C.nn_off + i] += alpha;
+  if (i < C.nsoc) x[C.soc_off[i]] += alpha;
+}
+
+__global__ void k_fill(double* x, int n, double v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = v;
+}
+
+__global__ __launch_bounds__(256) void k_csr_mv(int rows, const int* __restrict__ ptr,
+                                                const int* __restrict__ col, const int* __restrict__ src,
+                                                const double* __restrict__ val,
+                                                const double* __restrict__ x, double* y) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= rows) return;
+  double acc = 0.0;
+  for (int e = ptr[i]; e < ptr[i + 1]; ++e) acc += val[src[e]] * x[col[e]];
+  y[i] = acc;
+}
+
+__global__ void k_axpby(int n, double a, const double* x, double b, double* y) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = a * x[i] + b * y[i];
+}
+__global__ void k_lin2(int n, double a, const double* x, double b, const double* y, double* out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = a * x[i] + b * y[i];
+}
+__global__ void k_lin3(int n, double a, const double* x, double b, const double* y, double c,
+                       const double* z, double* out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = a * x[i] + b * y[i] + c * z[i];
+}
+__global__ void k_scale(int n, double a, double* x) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] *= a;
+}
+__global__ void k_copy(int n, const double* x, double* y) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = x[i];
+}
+__global__ void k_scatter(int n, const int* slot, const int* src, const double* val, double add,
+                          double* Kx) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int s = src[i];
+  double v = (s >= 0) ? val[s] : 0.0;
+  Kx[slot[i]] = v + add;
+}
+
+__global__ void k_hs_scatter(ConeDev C, int nhs, const int* slot, const int* kind, const int* blk,
+                             const int* ia, const int* ib, const int* row, double sreg_a, double* Kx) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nhs) return;
+  const int a = ia[t], b = ib[t];
+  double h = 0.0;
+  if (kind[t] == 1) {
+    h = C.hsd[row[t]];
+  } else if (kind[t] == 2) {
+    const int k = blk[t];
+    if (C.ident[k]) {
+      h = (a == b) ? 1.0 : 0.0;
+    } else {
+      const double* v = C.v + C.soc_off[k];
+      const double be = C.beta[k];
+      const double va = v[a], vb = v[b];
+      h = be * be *
+          (4.0 * C.vv[k] * va * vb - 2.0 * va * (jsign(b) * vb) - 2.0 * (jsign(a) * va) * vb +
+           (a == b ? 1.0 : 0.0));
+    }
+  }
+  double val = -h;
+  if (a == b) val -= sreg_a;
+  Kx[slot[t]] = val;
+}
+
+__global__ void k_scale_csc(int cols, const int* ptr, const int* row, const double* in, double c,
+                            const double* drow, const double* dcol, double* out) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= cols) return;
+  for (int k = ptr[j]; k < ptr[j + 1]; ++k) out[k] = in[k] * (c * drow[row[k]] * dcol[j]);
+}
+__global__ void k_vscale(int n, const double* d, double c, const double* in, double* out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = in[i] * (c * d[i]);
+}
+__global__ void k_vmuldiv(int n, const double* in, const double* mul, const double* div, double c,
+                          double* out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    double v = in[i];
+    if (mul) v = v * mul[i];
+    if (div) v = v / div[i];
+    out[i] = v * c;
+  }
+}
+
+inline int g1(long n) { return ceil_div(n > 0 ? n : 1, 256); }
+inline int gcap(long n, int cap = 1024) {
+  int g = g1(n);
+  return g < cap ? g : cap;
+}
+
+}  // namespace
+
+void launch_reduce(const RedBatch& b, double* partial, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_reduce_partial, dim3(RED_BLOCKS), dim3(256), 0, st, b, partial);
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, st, partial, b.nops, out);
+  CX_LAUNCH_CHECK();
+}
+
+void launch_cone_scaling(const ConeDev& C, const double* s, const double* z, hipStream_t st) {
+  if (C.nn_dim) hipLaunchKernelGGL(k_nn_scaling, dim3(g1(C.nn_dim)), dim3(256), 0, st, C, s, z);
+  if (C.nsmall) hipLaunchKernelGGL(k_soc_scaling_small, dim3(g1(C.nsmall)), dim3(256), 0, st, C, s, z);
+  if (C.nlarge) hipLaunchKernelGGL(k_soc_scaling_large, dim3(ceil_div(C.nlarge, 4)), dim3(256), 0, st, C, s, z);
+  CX_LAUNCH_CHECK();
+}
+
+void launch_neg_hs_plus(const ConeDev& C, int m, const double* x, const double* y, double* out,
+                        hipStream_t st) {
+  long work = (long)C.zero_dim + C.nn_dim + C.nsmall + 64L * C.nlarge;
+  (void)m;
+  hipLaunchKernelGGL(k_neg_hs_plus, dim3(gcap(work)), dim3(256), 0, st, C, m, x, y, out);
+  CX_LAUNCH_CHECK();
+}
+
+void launch_combined_rhs(const ConeDev& C, int m, const double* z, const double* ds_aff,
+                         const double* dz_aff, const double* rz, double sigma_mu, double one_m_sigma,
+                         double* ds_const, double* rhs_tail, hipStream_t st) {
+  (void)m;
+  long work = (long)C.zero_dim + C.nn_dim + C.nsmall + 64L * C.nlarge;
+  hipLaunchKernelGGL(k_combined_rhs, dim3(gcap(work)), dim3(256), 0, st, C, z, ds_aff, dz_aff, rz,
+                     sigma_mu, one_m_sigma, ds_const, rhs_tail);
+  CX_LAUNCH_CHECK();
+}
+
+void launch_max_step(const ConeDev& C, const double* x, const double* dx, double* partial,
+                     double* out, hipStream_t st) {
+  long work = (long)C.nn_dim + C.nsmall + 64L * C.nlarge;
+  int G = gcap(work, 240);
+  hipLaunchKernelGGL(k_max_step, dim3(G), dim3(256), 0, st, C, x, dx, partial);
+  hipLaunchKernelGGL(k_min_final, dim3(1), dim3(256), 0, st, partial, G, out);
+  CX_LAUNCH_CHECK();
+}
+
+void launch_margins(const ConeDev& C, const double* x, double* partial, double* out,
+                    hipStream_t st) {
+  long work = (long)C.nn_dim + C.nsmall + 64L * C.nlarge;
+  int G = gcap(work, 240);
+  hipLaunchKernelGGL(k_margins, dim3(G), dim3(256), 0, st, C, x, partial);
+  hipLaunchKernelGGL(k_margins_final, dim3(1), dim3(256), 0, st, partial, G, C.nn_dim > 0 ? 1 : 0, out);
+  CX_LAUNCH_CHECK();
+}
+
+void launch_unit_shift(const ConeDev& C, double* x, double alpha, hipStream_t st) {
+  int n = C.nn_dim > C.nsoc ? C.nn_dim : C.nsoc;
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_unit_shift, dim3(g1(n)), dim3(256), 0, st, C, x, alpha);
+  CX_LAUNCH_CHECK();
+}
+
+void launch_fill(double* x, int n, double v, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_fill, dim3(g1(n)), dim3(256), 0, st, x, n, v);
+  CX_LAUNCH_CHECK();
+}
+
+void launch_csr_mv(int rows, const int* ptr, const int* col, const int* src, const double* val,
+                   const double* x, double* y, hipStream_t st) {
+  if (rows <= 0) return;
+  hipLaunchKernelGGL(k_csr_mv, dim3(g1(rows)), dim3(256), 0, st, rows, ptr, col, src, val, x, y);
+  CX_LAUNCH_CHECK();
+}
+
+void launch_axpby(int n, double a, const double* x, double b, double* y, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_axpby, dim3(g1(n)), dim3(256), 0, st, n, a, x, b, y);
+  CX_LAUNCH_CHECK();
+}
+void launch_lin2(int n, double a, const double* x, double b, const double* y, double* out,
+                 hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_lin2, dim3(g1(n)), dim3(256), 0, st, n, a, x, b, y, out);
+  CX_LAUNCH_CHECK();
+}
+void launch_lin3(int n, double a, const double* x, double b, const double* y, double c,
+                 const double* z, double* out, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_lin3, dim3(g1(n)), dim3(256), 0, st, n, a, x, b, y, c, z, out);
+  CX_LAUNCH_CHECK();
+}
+void launch_scale(int n, double a, double* x, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_scale, dim3(g1(n)), dim3(256), 0, st, n, a, x);
+  CX_LAUNCH_CHECK();
+}
+void launch_copy(int n, const double* x, double* y, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_copy, dim3(g1(n)), dim3(256), 0, st, n, x, y);
+  CX_LAUNCH_CHECK();
+}
+void launch_scatter(int n, const int* slot, const int* src, const double* val, double add,
+                    double* Kx, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_scatter, dim3(g1(n)), dim3(256), 0, st, n, slot, src, val, add, Kx);
+  CX_LAUNCH_CHECK();
+}
+void launch_hs_scatter(const ConeDev& C, int nhs, const int* slot, const int* kind, const int* blk,
+                       const int* ia, const int* ib, const int* row, double sreg_a, double* Kx,
+                       hipStream_t st) {
+  if (nhs <= 0) return;
+  hipLaunchKernelGGL(k_hs_scatter, dim3(g1(nhs)), dim3(256), 0, st, C, nhs, slot, kind, blk, ia, ib,
+                     row, sreg_a, Kx);
+  CX_LAUNCH_CHECK();
+}
+void launch_scale_csc(int cols, const int* ptr, const int* row, const double* in, double c,
+                      const double* drow, const double* dcol, double* out, hipStream_t st) {
+  if (cols <= 0) return;
+  hipLaunchKernelGGL(k_scale_csc, dim3(g1(cols)), dim3(256), 0, st, cols, ptr, row, in, c, drow, dcol,
+                     out);
+  CX_LAUNCH_CHECK();
+}
+void launch_vscale(int n, const double* d, double c, const double* in, double* out, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_vscale, dim3(g1(n)), dim3(256), 0, st, n, d, c, in, out);
+  CX_LAUNCH_CHECK();
+}
+void launch_vmuldiv(int n, const double* in, const double* mul, const double* div, double c,
+                    double* out, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_vmuldiv, dim3(g1(n)), dim3(256), 0, st, n, in, mul, div, c, out);
+  CX_LAUNCH_CHECK();
+}
+
+}  // namespace cx

@@ -1,0 +1,940 @@
+// Device multifrontal LDL^T for the quasi-definite KKT matrix (gfx950).
+//
+// Fronts: supernode s owns a dense nr x nr column-major front F_s (only the
+// lower triangle is meaningful).  Its first nc columns hold, after the
+// partial factorization, L (strictly below the diagonal) and D (diagonal);
+// the trailing (nr-nc)^2 block holds the update matrix passed to the parent.
+//
+// Factorization schedule, per level of the assembly tree (leaves first):
+//   k_assemble      zero-fill, scatter K entries (+ diagonal bump), extend-add
+//                   of the children's update matrices (pull, deterministic)
+//   k_factor_small  fronts with nr <= 64, factored whole in LDS
+//   k_panel         64-column sub-panel: diagonal block LDL^T in LDS plus the
+//                   triangular solve of 64 rows below it (LDS, 4 lanes per row)
+//   k_trail         lower-triangle trailing update C -= L D L^T on FP64 MFMA
+//                   (v_mfma_f64_16x16x4f64), 128x128 tiles
+// Triangular solves: one ticketed launch per sweep over the whole tree with
+// bounded waits on per-block flags (forward: row blocks, backward: column
+// blocks), then an in-block substitution.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <numeric>
+
+#include "kkt_device.hpp"
+
+namespace cx {
+
+namespace {
+
+constexpr int SMALL_NR = 64;  // fronts up to this size are factored in LDS
+constexpr int NB = 64;        // sub-panel width
+constexpr int PW = 256;       // outer panel width
+constexpr int TM = 128;       // trailing-update tile
+constexpr int KC = 16;        // K chunk staged in LDS per step
+constexpr int SB = 64;        // solve block
+constexpr int LDSP = NB + 1;  // padded LDS row stride for 64x64 blocks
+constexpr long SPIN_LIMIT = 1L << 22;  // ~0.2 s of polling
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+__device__ inline double pivot_fix(double d, double sgn, int mode, double eps, double delta,
+                                   int* stats, bool count) {
+  if (mode == PIVOT_REGULARIZE) {
+    // regularizePivot: src/conicxx/kkt/regularized_ldlt.cpp:132-139
+    if (d * sgn >= eps) return d;
+    d += sgn * delta;
+    if (d * sgn < eps) d = sgn * delta;
+    if (count) atomicAdd(&stats[1], 1);
+    return d;
+  }
+  if (d == 0.0 && count) atomicOr(&stats[0], 1);  // QDLDL_factor reports an exact zero pivot
+  return d;
+}
+
+__device__ inline double load_acq(const double* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Bounded waits: give up (and flag the sweep as failed) after SPIN_LIMIT polls,
+// or at once when another workgroup has already flagged a failure.
+__device__ inline bool wait_flag(int* f, int want, int* err) {
+  long spins = 0;
+  while (__hip_atomic_load(f, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != want) {
+    if (++spins > SPIN_LIMIT || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+      atomicOr(err, 1);
+      return false;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+  return true;
+}
+
+__device__ inline bool wait_count(int* f, int want, int* err) {
+  long spins = 0;
+  while (__hip_atomic_load(f, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < want) {
+    if (++spins > SPIN_LIMIT || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+      atomicOr(err, 1);
+      return false;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+  return true;
+}
+
+// Right-looking LDL^T of an n x n block held row-major in LDS (stride LDSP),
+// nc pivot columns; all 256 threads participate.  Pivots at their diagonal
+// positions afterwards, L strictly below.
+__device__ void ldl_lds(double* A, int n, int nc, const double* sgn, int mode, double eps,
+                        double delta, int* stats, bool count, double* colk) {
+  const int tid = threadIdx.x;
+  for (int k = 0; k < nc; ++k) {
+    if (tid == 0) A[k * LDSP + k] = pivot_fix(A[k * LDSP + k], sgn[k], mode, eps, delta, stats, count);
+    __syncthreads();
+    const double dinv = 1.0 / A[k * LDSP + k];
+    for (int i = k + 1 + tid; i < n; i += blockDim.x) {
+      double c = A[i * LDSP + k];
+      colk[i] = c;
+      A[i * LDSP + k] = c * dinv;
+    }
+    __syncthreads();
+    const int m = n - k - 1;
+    for (int idx = tid; idx < m * m; idx += blockDim.x) {
+      int ii = idx % m, jj = idx / m;
+      if (ii >= jj) {
+        int i = k + 1 + ii, j = k + 1 + jj;
+        A[i * LDSP + j] -= A[i * LDSP + k] * colk[j];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// assembly: zero-fill + K scatter + extend-add of children update matrices
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_assemble(
+    const AsmTask* __restrict__ tasks, const AsmSeg* __restrict__ segs, const long* __restrict__ fofs,
+    const int* __restrict__ nrv, const int* __restrict__ ncv, const int* __restrict__ firstv,
+    const int* __restrict__ kcolptr, const int* __restrict__ ksrc, const long* __restrict__ kdst,
+    const signed char* __restrict__ kdiag, const double* __restrict__ Kx, double extra_p,
+    double extra_a, const int* __restrict__ rmap_ofs, const int* __restrict__ rmap, double* F) {
+  const AsmTask T = tasks[blockIdx.x];
+  const int s = T.s, nr = nrv[s], nc = ncv[s];
+  const long base = fofs[s];
+  const int tid = threadIdx.x;
+  // 1. zero the task's columns
+  const long ncolw = (long)(T.j1 - T.j0) * nr;
+  double* Fc = F + base + (long)T.j0 * nr;
+  for (long idx = tid; idx < ncolw; idx += blockDim.x) Fc[idx] = 0.0;
+  __syncthreads();
+  // 2. original K entries of the pivot columns in range (distinct destinations)
+  const int jend = min(T.j1, nc);
+  if (T.j0 < jend) {
+    const int e0 = kcolptr[firstv[s] + T.j0], e1 = kcolptr[firstv[s] + jend];
+    for (int e = e0 + tid; e < e1; e += blockDim.x) {
+      double v = Kx[ksrc[e]];
+      signed char dg = kdiag[e];
+      if (dg > 0) v += extra_p;
+      if (dg < 0) v -= extra_a;
+      F[kdst[e]] += v;
+    }
+  }
+  __syncthreads();
+  // 3. children update matrices, one child at a time
+  const int lane = tid & 63, wv = tid >> 6;
+  for (int g = T.seg_beg; g < T.seg_end; ++g) {
+    const AsmSeg S = segs[g];
+    const int c = S.child, nrc = nrv[c], ncc = ncv[c];
+    const int nu = nrc - ncc;
+    const int* rm = rmap + rmap_ofs[c];
+    const double* Uc = F + fofs[c] + (long)ncc * nrc + ncc;  // U(i', i) = Uc[i*nrc + i']
+    for (int i = S.i0 + wv; i < S.i1; i += 4) {
+      const int pc = rm[i];
+      double* Fp = F + base + (long)pc * nr;
+      const double* Ucol = Uc + (long)i * nrc;
+      for (int ip = i + lane; ip < nu; ip += 64) Fp[rm[ip]] += Ucol[ip];
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// small fronts: whole front in LDS
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_factor_small(
+    const int* __restrict__ list, const long* __restrict__ fofs, const int* __restrict__ nrv,
+    const int* __restrict__ ncv, const int* __restrict__ firstv, const double* __restrict__ sgn,
+    double* F, double* Dg, int mode, double eps, double delta, int* stats) {
+  __shared__ double A[SMALL_NR * LDSP];
+  __shared__ double colk[SMALL_NR];
+  const int s = list[blockIdx.x];
+  const int nr = nrv[s], nc = ncv[s], first = firstv[s];
+  const long base = fofs[s];
+  const int tid = threadIdx.x;
+  for (int idx = tid; idx < nr * nr; idx += blockDim.x) {
+    int col = idx / nr, row = idx % nr;
+    if (row >= col) A[row * LDSP + col] = F[base + idx];
+  }
+  __syncthreads();
+  ldl_lds(A, nr, nc, sgn + first, mode, eps, delta, stats, true, colk);
+  for (int idx = tid; idx < nr * nr; idx += blockDim.x) {
+    int col = idx / nr, row = idx % nr;
+    if (row >= col) F[base + idx] = A[row * LDSP + col];
+  }
+  for (int k = tid; k < nc; k += blockDim.x) Dg[first + k] = A[k * LDSP + k];
+}
+
+// ---------------------------------------------------------------------------
+// sub-panel: diagonal block LDL^T (redundant per task) + TRSM of 64 rows
+// ---------------------------------------------------------------------------
+constexpr int PANEL_ROWS = 64;
+
+__global__ __launch_bounds__(256) void k_panel(
+    const PanelTask* __restrict__ tasks, const long* __restrict__ fofs, const int* __restrict__ nrv,
+    const int* __restrict__ firstv, const double* __restrict__ sgn, double* F, double* Dg, int mode,
+    double eps, double delta, int* stats) {
+  __shared__ double L11[NB * LDSP];
+  __shared__ double X[PANEL_ROWS * LDSP];
+  __shared__ double colk[NB];
+  __shared__ double dinv[NB];
+  const PanelTask T = tasks[blockIdx.x];
+  const int s = T.s, nr = nrv[s], first = firstv[s], a = T.a, w = T.w;
+  const long base = fofs[s];
+  const int tid = threadIdx.x;
+  for (int idx = tid; idx < w * w; idx += blockDim.x) {
+    int j = idx / w, i = idx % w;
+    if (i >= j) L11[i * LDSP + j] = F[base + (long)(a + j) * nr + (a + i)];
+  }
+  __syncthreads();
+  const bool writer = T.r0 < 0;
+  ldl_lds(L11, w, w, sgn + first + a, mode, eps, delta, stats, writer, colk);
+  if (writer) {
+    for (int idx = tid; idx < w * w; idx += blockDim.x) {
+      int j = idx / w, i = idx % w;
+      if (i >= j) F[base + (long)(a + j) * nr + (a + i)] = L11[i * LDSP + j];
+    }
+    for (int k = tid; k < w; k += blockDim.x) Dg[first + a + k] = L11[k * LDSP + k];
+    return;
+  }
+  const int rows = T.r1 - T.r0;
+  for (int k = tid; k < w; k += blockDim.x) dinv[k] = 1.0 / L11[k * LDSP + k];
+  for (int idx = tid; idx < rows * w; idx += blockDim.x) {
+    int k = idx / rows, rr = idx % rows;
+    X[rr * LDSP + k] = F[base + (long)(a + k) * nr + T.r0 + rr];
+  }
+  __syncthreads();
+  // X L11^T = R (unit lower L11): four lanes per row share the inner sums
+  const int rr = tid >> 2, q = tid & 3;
+  if (rr < rows) {
+    for (int k = 1; k < w; ++k) {
+      double part = 0.0;
+      for (int t = q; t < k; t += 4) part += X[rr * LDSP + t] * L11[k * LDSP + t];
+      part += __shfl_xor(part, 1, 64);
+      part += __shfl_xor(part, 2, 64);
+      if (q == 0) X[rr * LDSP + k] -= part;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    }
+  }
+  __syncthreads();
+  for (int idx = tid; idx < rows * w; idx += blockDim.x) {
+    int k = idx / rows, r2 = idx % rows;
+    F[base + (long)(a + k) * nr + T.r0 + r2] = X[r2 * LDSP + k] * dinv[k];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// trailing update on FP64 MFMA:  C[i][j] -= sum_k L[i][k] D[k] L[j][k]
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_trail(const TrailTask* __restrict__ tasks,
+                                               const long* __restrict__ fofs,
+                                               const int* __restrict__ nrv,
+                                               const int* __restrict__ firstv,
+                                               const double* __restrict__ Dg, double* F) {
+  constexpr int LDA = TM + 4;
+  __shared__ double As[KC * LDA];
+  __shared__ double Bs[KC * LDA];
+  const TrailTask T = tasks[blockIdx.x];
+  const int s = T.s, nr = nrv[s], first = firstv[s];
+  const long base = fofs[s];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int wr = (wv >> 1) * 64, wc = (wv & 1) * 64;  // wave tile origin within the block tile
+  v4d acc[4][4];
+#pragma unroll
+  for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+    for (int ni = 0; ni < 4; ++ni) acc[mi][ni] = (v4d){0.0, 0.0, 0.0, 0.0};
+  const int lrow = tid & (TM - 1);  // 0..127
+  const int lk = tid >> 7;          // 0..1
+  for (int k0 = T.k0; k0 < T.k1; k0 += KC) {
+    const int kw = min(KC, T.k1 - k0);
+    // stage A (rows ti..) and B (rows tj.., scaled by D) for this K chunk
+#pragma unroll
+    for (int q = 0; q < KC / 2; ++q) {
+      const int kk = lk + 2 * q;
+      const int gi = T.ti + lrow, gj = T.tj + lrow;
+      double av = 0.0, bv = 0.0;
+      if (kk < kw) {
+        const long colb = base + (long)(k0 + kk) * nr;
+        if (gi < nr) av = F[colb + gi];
+        if (gj < nr) bv = F[colb + gj] * Dg[first + k0 + kk];
+      }
+      As[kk * LDA + lrow] = av;
+      Bs[kk * LDA + lrow] = bv;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < KC; kk += 4) {
+      double af[4], bf[4];
+      const int kr = kk + (lane >> 4);
+#pragma unroll
+      for (int mi = 0; mi < 4; ++mi) af[mi] = As[kr * LDA + wr + mi * 16 + (lane & 15)];
+#pragma unroll
+      for (int ni = 0; ni < 4; ++ni) bf[ni] = Bs[kr * LDA + wc + ni * 16 + (lane & 15)];
+#pragma unroll
+      for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < 4; ++ni)
+          acc[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[mi], bf[ni], acc[mi][ni], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  // write back: lane holds rows 4*(lane/16)+r, column lane%16 of each 16x16 tile
+#pragma unroll
+  for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+    for (int ni = 0; ni < 4; ++ni) {
+      const int j = T.tj + wc + ni * 16 + (lane & 15);
+      if (j >= T.chi) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = T.ti + wr + mi * 16 + 4 * (lane >> 4) + r;
+        if (i < nr && i >= j && i >= T.rlo) F[base + (long)j * nr + i] -= acc[mi][ni][r];
+      }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// pivot statistics: min |D| (NaN-propagating)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_dstats(const double* __restrict__ D, int N, double* out) {
+  __shared__ double red[256];
+  double mn = INFINITY;
+  bool nan = false;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) {
+    double v = fabs(D[i]);
+    if (v != v) nan = true;
+    mn = fmin(mn, v);
+  }
+  red[threadIdx.x] = nan ? NAN : mn;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      double a = red[threadIdx.x], b = red[threadIdx.x + w];
+      red[threadIdx.x] = (a != a || b != b) ? NAN : fmin(a, b);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = (N > 0) ? red[0] : 1.0;
+}
+
+// ---------------------------------------------------------------------------
+// solves
+// ---------------------------------------------------------------------------
+__global__ void k_zero_int(int* p, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
+}
+
+__global__ void k_perm_in(const int* __restrict__ perm, const double* __restrict__ rhs, double* xp,
+                          int N) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < N) xp[k] = rhs[perm[k]];
+}
+__global__ void k_perm_out(const int* __restrict__ perm, const double* __restrict__ xp, double* x,
+                           int N) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < N) x[perm[k]] = xp[k];
+}
+
+// forward: L y = b on the permuted vector; pivot rows of each front get y,
+// non-pivot rows produce the update vector U_s for the parent.
+__global__ __launch_bounds__(256) void k_fwd(
+    const SolveTask* __restrict__ tasks, int* ticket, const long* __restrict__ fofs,
+    const int* __restrict__ nrv, const int* __restrict__ ncv, const int* __restrict__ firstv,
+    const int* __restrict__ rowptr, const int* __restrict__ gptr, const int* __restrict__ gidx,
+    const long* __restrict__ uofs, const int* __restrict__ child_ptr, const int* __restrict__ child,
+    const int* __restrict__ nblk, const int* __restrict__ flag_ofs, int* flags, int* done, int gen,
+    const double* __restrict__ F, double* xp, double* U, int* err) {
+  __shared__ int tk;
+  __shared__ double part[4][SB];
+  __shared__ double yblk[SB];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  if (tid == 0) tk = atomicAdd(ticket, 1);
+  __syncthreads();
+  const SolveTask T = tasks[tk];
+  const int s = T.s, rb = T.blk, nr = nrv[s], nc = ncv[s], first = firstv[s];
+  const long base = fofs[s];
+  const int r0 = rb * SB;
+  const int ncb = (nc + SB - 1) / SB;
+  if (tid == 0) {
+    for (int t = child_ptr[s]; t < child_ptr[s + 1]; ++t) {
+      int c = child[t];
+      wait_count(&done[c], nblk[c], err);
+    }
+  }
+  __syncthreads();
+  const int r = r0 + lane;
+  double acc = 0.0;
+  const int ncbk = min(rb, ncb);
+  for (int cb = 0; cb < ncbk; ++cb) {
+    if (tid == 0) wait_flag(&flags[flag_ofs[s] + cb], gen, err);
+    __syncthreads();
+    if (tid < SB) {
+      int c = cb * SB + tid;
+      yblk[tid] = (c < nc) ? load_acq(&xp[first + c]) : 0.0;
+    }
+    __syncthreads();
+    if (r < nr) {
+      const int cend = min(SB, nc - cb * SB);
+      for (int q = wv; q < cend; q += 4) {
+        const int c = cb * SB + q;
+        acc += F[base + (long)c * nr + r] * yblk[q];
+      }
+    }
+    __syncthreads();
+  }
+  part[wv][lane] = acc;
+  __syncthreads();
+  if (wv == 0) {
+    double val = 0.0;
+    if (r < nr) {
+      val = (r < nc) ? xp[first + r] : 0.0;
+      const int gr = rowptr[s] + r;
+      for (int g = gptr[gr]; g < gptr[gr + 1]; ++g) val += load_acq(&U[gidx[g]]);
+      val -= part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane];
+    }
+    if (rb < ncb) {
+      const int kc = min(SB, nc - r0);
+      for (int k = 0; k < kc; ++k) {
+        double yk = __shfl(val, k, 64);
+        if (lane > k && r < nr) val -= F[base + (long)(r0 + k) * nr + r] * yk;
+      }
+    }
+    if (r < nr) {
+      if (r < nc)
+        xp[first + r] = val;
+      else
+        U[uofs[s] + (r - nc)] = val;
+    }
+  }
+  __threadfence();
+  __syncthreads();
+  if (tid == 0) {
+    if (rb < ncb)
+      __hip_atomic_store(&flags[flag_ofs[s] + rb], gen, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_add(&done[s], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// backward: D^-1 then L^T x = y, one column block of pivots per task.
+__global__ __launch_bounds__(256) void k_bwd(
+    const SolveTask* __restrict__ tasks, int* ticket, const long* __restrict__ fofs,
+    const int* __restrict__ nrv, const int* __restrict__ ncv, const int* __restrict__ firstv,
+    const int* __restrict__ rowptr, const int* __restrict__ rows, const int* __restrict__ parent,
+    const int* __restrict__ nblkb, const int* __restrict__ flag_ofs, int* flags, int* done, int gen,
+    const double* __restrict__ F, const double* __restrict__ Dg, double* xp, int* err) {
+  __shared__ int tk;
+  __shared__ double xs[4][SB];
+  __shared__ double part[4][SB];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  if (tid == 0) tk = atomicAdd(ticket, 1);
+  __syncthreads();
+  const SolveTask T = tasks[tk];
+  const int s = T.s, cb = T.blk, nr = nrv[s], nc = ncv[s], first = firstv[s];
+  const long base = fofs[s];
+  const int c0 = cb * SB, kc = min(SB, nc - c0);
+  const int ncb = (nc + SB - 1) / SB;
+  if (tid == 0 && parent[s] >= 0) wait_count(&done[parent[s]], nblkb[parent[s]], err);
+  __syncthreads();
+  // acc[c] = sum over rows below this block of L[r][c0+c] x[r]
+  double acc = 0.0;
+  const int col = c0 + lane;
+  const int rstart = c0 + kc;  // first row below the diagonal block
+  const int nchunks = (nr - rstart + SB - 1) / SB;
+  for (int q = wv; q < nchunks; q += 4) {
+    const int R = rstart + q * SB;
+    // pivot rows of later column blocks of this front must be final
+    if (lane == 0 && R < nc) {
+      const int b0 = R / SB, b1 = min(R + SB, nc) - 1;
+      for (int b = b0; b <= b1 / SB; ++b)
+        if (b > cb && b < ncb) wait_flag(&flags[flag_ofs[s] + b], gen, err);
+    }
+    __builtin_amdgcn_wave_barrier();
+    {
+      const int rr = R + lane;
+      double xv = 0.0;
+      if (rr < nr) {
+        int gidxr = (rr < nc) ? first + rr : rows[rowptr[s] + rr];
+        xv = load_acq(&xp[gidxr]);
+      }
+      xs[wv][lane] = xv;
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+    if (lane < kc) {
+      const int rlen = min(SB, nr - R);
+      const double* Lc = F + base + (long)col * nr + R;
+      for (int t = 0; t < rlen; ++t) acc += Lc[t] * xs[wv][t];
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  part[wv][lane] = acc;
+  __syncthreads();
+  if (wv == 0) {
+    double y = 0.0;
+    if (lane < kc) {
+      const double dv = Dg[first + col];
+      const double dinv = 1.0 / dv;
+      y = xp[first + col] * dinv - (part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane]);
+    }
+    for (int k = kc - 1; k >= 0; --k) {
+      double xk = __shfl(y, k, 64);
+      if (lane < k) y -= F[base + (long)(c0 + lane) * nr + (c0 + k)] * xk;
+    }
+    if (lane < kc) xp[first + col] = y;
+  }
+  __threadfence();
+  __syncthreads();
+  if (tid == 0) {
+    __hip_atomic_store(&flags[flag_ofs[s] + cb], gen, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_add(&done[s], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// y = K x with K symmetric, stored as a full CSR whose values point into the
+// lower CSC value array.
+__global__ __launch_bounds__(256) void k_spmv_sym(int N, const int* __restrict__ ptr,
+                                                  const int* __restrict__ col,
+                                                  const int* __restrict__ src,
+                                                  const double* __restrict__ Kx,
+                                                  const double* __restrict__ x, double* y) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  double acc = 0.0;
+  for (int e = ptr[i]; e < ptr[i + 1]; ++e) acc += Kx[src[e]] * x[col[e]];
+  y[i] = acc;
+}
+
+}  // namespace
+
+// ===========================================================================
+// host side
+// ===========================================================================
+void DeviceKKT::setup(const LowerPattern& K, int nx, hipStream_t stream) {
+  st_ = stream;
+  nx_ = nx;
+  P_ = plan_supernodes(K);
+  const SupernodalPlan& P = P_;
+  const int N = P.N, ns = P.ns;
+  // front geometry
+  std::vector<long> fofs(ns + 1, 0);
+  std::vector<int> nr(ns), nc(ns), first(ns);
+  for (int s = 0; s < ns; ++s) {
+    nr[s] = P.nr(s);
+    nc[s] = P.sn_ncol[s];
+    first[s] = P.sn_first[s];
+    fofs[s + 1] = fofs[s] + (long)nr[s] * nr[s];
+  }
+  // K entries grouped by permuted column (= column of the lower permuted entry)
+  std::vector<int> kcount(N + 1, 0);
+  const int nnzK = K.colptr[N];
+  std::vector<int> kpcol(nnzK), kprow(nnzK);
+  for (int c = 0; c < N; ++c)
+    for (int k = K.colptr[c]; k < K.colptr[c + 1]; ++k) {
+      int pr = P.iperm[K.rowind[k]], pc = P.iperm[c];
+      int lo = std::min(pr, pc), hi = std::max(pr, pc);
+      kpcol[k] = lo;
+      kprow[k] = hi;
+      kcount[lo + 1]++;
+    }
+  for (int j = 0; j < N; ++j) kcount[j + 1] += kcount[j];
+  std::vector<int> ksrc(nnzK);
+  std::vector<long> kdst(nnzK);
+  std::vector<signed char> kdiag(nnzK);
+  {
+    std::vector<int> fill(kcount.begin(), kcount.end() - 1);
+    // position of a global row inside a front: binary search in the row list
+    for (int c = 0; c < N; ++c)
+      for (int k = K.colptr[c]; k < K.colptr[c + 1]; ++k) {
+        int lo = kpcol[k], hi = kprow[k];
+        int s = P.col_sn[lo];
+        const int* rb = P.rows.data() + P.row_ptr[s];
+        const int* re = P.rows.data() + P.row_ptr[s + 1];
+        const int* it = std::lower_bound(rb, re, hi);
+        if (it == re || *it != hi) throw std::runtime_error("kkt plan: row missing from front");
+        int rloc = (int)(it - rb), cloc = lo - P.sn_first[s];
+        int dst = fill[lo]++;
+        ksrc[dst] = k;
+        kdst[dst] = fofs[s] + (long)cloc * nr[s] + rloc;
+        kdiag[dst] = (K.rowind[k] == c) ? (c < nx ? 1 : -1) : 0;
+      }
+  }
+  // child -> parent relative row maps
+  std::vector<int> rmap_ofs(ns + 1, 0), rmap;
+  for (int s = 0; s < ns; ++s) rmap_ofs[s + 1] = rmap_ofs[s] + (nr[s] - nc[s]);
+  rmap.resize(rmap_ofs[ns]);
+  for (int s = 0; s < ns; ++s) {
+    int p = P.sn_parent[s];
+    if (p < 0) {
+      if (nr[s] != nc[s]) throw std::runtime_error("kkt plan: root with update rows");
+      continue;
+    }
+    const int* pb = P.rows.data() + P.row_ptr[p];
+    const int* pe = P.rows.data() + P.row_ptr[p + 1];
+    for (int i = 0; i < nr[s] - nc[s]; ++i) {
+      int g = P.rows[P.row_ptr[s] + nc[s] + i];
+      const int* it = std::lower_bound(pb, pe, g);
+      if (it == pe || *it != g) throw std::runtime_error("kkt plan: child row not in parent");
+      rmap[rmap_ofs[s] + i] = (int)(it - pb);
+    }
+  }
+  std::vector<double> sgn(N);
+  for (int k = 0; k < N; ++k) sgn[k] = P.perm[k] < nx ? 1.0 : -1.0;
+
+  fofs_.upload(fofs, st_);
+  nr_.upload(nr, st_);
+  nc_.upload(nc, st_);
+  first_.upload(first, st_);
+  rowptr_.upload(P.row_ptr, st_);
+  rows_.upload(P.rows, st_);
+  child_ptr_.upload(P.child_ptr, st_);
+  child_.upload(P.child, st_);
+  sgn_.upload(sgn, st_);
+  kcolptr_.upload(kcount, st_);
+  ksrc_.upload(ksrc, st_);
+  kdst_.upload(kdst, st_);
+  kdiag_.upload(kdiag, st_);
+  rmap_ofs_.upload(rmap_ofs, st_);
+  rmap_.upload(rmap, st_);
+  F_.alloc(fofs[ns]);
+  D_.alloc(N);
+  fstat_.alloc(2);
+  dstat_.alloc(1);
+
+  build_schedule();
+
+  // --- solve plan ---
+  std::vector<int> flag_ofs(ns + 1, 0), nblk(ns), nblkb(ns);
+  std::vector<long> uofs(ns + 1, 0);
+  for (int s = 0; s < ns; ++s) {
+    nblk[s] = ceil_div(nr[s], SB);
+    nblkb[s] = ceil_div(nc[s], SB);
+    flag_ofs[s + 1] = flag_ofs[s] + nblkb[s];
+    uofs[s + 1] = uofs[s] + (nr[s] - nc[s]);
+  }
+  // gather lists: for each front row, the U entries of children that land on it
+  std::vector<int> gcount(P.row_ptr[ns] + 1, 0);
+  for (int c = 0; c < ns; ++c) {
+    int p = P.sn_parent[c];
+    if (p < 0) continue;
+    for (int i = 0; i < nr[c] - nc[c]; ++i) gcount[P.row_ptr[p] + rmap[rmap_ofs[c] + i] + 1]++;
+  }
+  for (size_t i = 0; i + 1 < gcount.size(); ++i) gcount[i + 1] += gcount[i];
+  std::vector<int> gidx(gcount.back());
+  {
+    std::vector<int> fill(gcount.begin(), gcount.end() - 1);
+    for (int c = 0; c < ns; ++c) {
+      int p = P.sn_parent[c];
+      if (p < 0) continue;
+      for (int i = 0; i < nr[c] - nc[c]; ++i)
+        gidx[fill[P.row_ptr[p] + rmap[rmap_ofs[c] + i]]++] = (int)(uofs[c] + i);
+    }
+  }
+  std::vector<SolveTask> fwd, bwd;
+  for (int l = 0; l < P.nlevels; ++l) {
+    int maxb = 0;
+    for (int t = P.level_ptr[l]; t < P.level_ptr[l + 1]; ++t) maxb = std::max(maxb, nblk[P.level_sn[t]]);
+    for (int b = 0; b < maxb; ++b)
+      for (int t = P.level_ptr[l]; t < P.level_ptr[l + 1]; ++t) {
+        int s = P.level_sn[t];
+        if (b < nblk[s]) fwd.push_back({s, b});
+      }
+  }
+  for (int l = P.nlevels - 1; l >= 0; --l) {
+    int maxb = 0;
+    for (int t = P.level_ptr[l]; t < P.level_ptr[l + 1]; ++t) maxb = std::max(maxb, nblkb[P.level_sn[t]]);
+    for (int b = maxb - 1; b >= 0; --b)
+      for (int t = P.level_ptr[l]; t < P.level_ptr[l + 1]; ++t) {
+        int s = P.level_sn[t];
+        if (b < nblkb[s]) bwd.push_back({s, b});
+      }
+  }
+  nfwd_ = (int)fwd.size();
+  nbwd_ = (int)bwd.size();
+  perm_.upload(P.perm, st_);
+  fwd_.upload(fwd, st_);
+  bwd_.upload(bwd, st_);
+  gptr_.upload(gcount, st_);
+  gidx_.upload(gidx, st_);
+  uofs_.upload(uofs, st_);
+  flag_ofs_.upload(flag_ofs, st_);
+  nblk_.upload(nblk, st_);
+  nblkb_.upload(nblkb, st_);
+  parent_.upload(P.sn_parent, st_);
+  flags_.alloc(std::max(1, flag_ofs[ns]));
+  flags_.zero(st_);
+  done_.alloc(2 * ns + 2);
+  serr_.alloc(1);
+  serr_.zero(st_);
+  xp_.alloc(N);
+  U_.alloc(std::max<long>(1, uofs[ns]));
+  gen_ = 0;
+
+  // --- full symmetric CSR of K for the refinement residual ---
+  std::vector<int> fptr(N + 1, 0);
+  for (int c = 0; c < N; ++c)
+    for (int k = K.colptr[c]; k < K.colptr[c + 1]; ++k) {
+      int r = K.rowind[k];
+      fptr[r + 1]++;
+      if (r != c) fptr[c + 1]++;
+    }
+  for (int i = 0; i < N; ++i) fptr[i + 1] += fptr[i];
+  std::vector<int> fcol(fptr[N]), fsrc(fptr[N]);
+  {
+    std::vector<int> fill(fptr.begin(), fptr.end() - 1);
+    for (int c = 0; c < N; ++c)
+      for (int k = K.colptr[c]; k < K.colptr[c + 1]; ++k) {
+        int r = K.rowind[k];
+        fcol[fill[r]] = c, fsrc[fill[r]++] = k;
+        if (r != c) fcol[fill[c]] = r, fsrc[fill[c]++] = k;
+      }
+  }
+  kfptr_.upload(fptr, st_);
+  kfcol_.upload(fcol, st_);
+  kfsrc_.upload(fsrc, st_);
+  {
+    const char* e = std::getenv("CONICXX_TRAIL_TIMING");
+    time_trail_ = e && e[0] == '1';
+  }
+  if (!ev0_) {
+    CX_HIP(hipEventCreate(&ev0_));
+    CX_HIP(hipEventCreate(&ev1_));
+  }
+  CX_HIP(hipStreamSynchronize(st_));
+}
+
+void DeviceKKT::build_schedule() {
+  const SupernodalPlan& P = P_;
+  const int ns = P.ns;
+  launches_.clear();
+  asm_h_.clear();
+  seg_h_.clear();
+  small_h_.clear();
+  panel_h_.clear();
+  trail_h_.clear();
+  trail_launch_flops_.clear();
+  trail_flops_static_ = 0;
+  std::vector<int> rmap_ofs(ns + 1, 0);
+  for (int s = 0; s < ns; ++s) rmap_ofs[s + 1] = rmap_ofs[s] + (P.nr(s) - P.sn_ncol[s]);
+  // parent-relative maps again (host copy for segment search)
+  std::vector<int> rmap(rmap_ofs[ns]);
+  for (int s = 0; s < ns; ++s) {
+    int p = P.sn_parent[s];
+    if (p < 0) continue;
+    const int* pb = P.rows.data() + P.row_ptr[p];
+    const int* pe = P.rows.data() + P.row_ptr[p + 1];
+    for (int i = 0; i < P.nr(s) - P.sn_ncol[s]; ++i)
+      rmap[rmap_ofs[s] + i] = (int)(std::lower_bound(pb, pe, P.rows[P.row_ptr[s] + P.sn_ncol[s] + i]) - pb);
+  }
+  auto push_trail = [&](int s, int ti, int tj, int k0, int k1, int rlo, int chi, int nr, double& fl) {
+    trail_h_.push_back({s, ti, tj, k0, k1, rlo, chi});
+    // algorithmic flops: 2 * K * (#updated lower elements in the tile)
+    double cnt = 0;
+    int jmax = std::min(tj + TM, chi);
+    for (int j = tj; j < jmax; ++j) {
+      int i0 = std::max({ti, j, rlo});
+      int i1 = std::min(ti + TM, nr);
+      if (i1 > i0) cnt += i1 - i0;
+    }
+    fl += 2.0 * (k1 - k0) * cnt;
+  };
+  for (int l = 0; l < P.nlevels; ++l) {
+    // assembly
+    int beg = (int)asm_h_.size();
+    std::vector<int> bigs;
+    for (int t = P.level_ptr[l]; t < P.level_ptr[l + 1]; ++t) {
+      int s = P.level_sn[t];
+      int nr = P.nr(s);
+      int cb = std::max(1, std::min(nr, 16384 / std::max(1, nr)));
+      for (int j0 = 0; j0 < nr; j0 += cb) {
+        int j1 = std::min(nr, j0 + cb);
+        AsmTask T{s, j0, j1, (int)seg_h_.size(), 0};
+        for (int q = P.child_ptr[s]; q < P.child_ptr[s + 1]; ++q) {
+          int c = P.child[q];
+          const int* rb = rmap.data() + rmap_ofs[c];
+          const int* re = rmap.data() + rmap_ofs[c + 1];
+          int i0 = (int)(std::lower_bound(rb, re, j0) - rb);
+          int i1 = (int)(std::lower_bound(rb, re, j1) - rb);
+          if (i1 > i0) seg_h_.push_back({c, i0, i1});
+        }
+        T.seg_end = (int)seg_h_.size();
+        asm_h_.push_back(T);
+      }
+      if (nr <= SMALL_NR)
+        small_h_.push_back(s);
+      else
+        bigs.push_back(s);
+    }
+    launches_.push_back({L_ASSEMBLE, beg, (int)asm_h_.size()});
+    // small fronts of this level
+    int sb = 0;
+    for (int t = P.level_ptr[l]; t < P.level_ptr[l + 1]; ++t)
+      if (P.nr(P.level_sn[t]) <= SMALL_NR) ++sb;
+    if (sb) launches_.push_back({L_SMALL, (int)small_h_.size() - sb, (int)small_h_.size()});
+    // big fronts: blocked right-looking, outer panels of PW, sub-panels of NB
+    int maxnc = 0;
+    for (int s : bigs) maxnc = std::max(maxnc, P.sn_ncol[s]);
+    for (int P0 = 0; P0 < maxnc; P0 += PW) {
+      for (int a0 = P0; a0 < P0 + PW && a0 < maxnc; a0 += NB) {
+        int pbeg = (int)panel_h_.size();
+        int tbeg = (int)trail_h_.size();
+        double fl = 0;
+        for (int s : bigs) {
+          int nc = P.sn_ncol[s], nr = P.nr(s);
+          if (a0 >= nc) continue;
+          int pend = std::min(P0 + PW, nc);
+          int w = std::min(NB, pend - a0);
+          panel_h_.push_back({s, a0, w, -1, -1});
+          for (int r0 = a0 + w; r0 < nr; r0 += PANEL_ROWS) panel_h_.push_back({s, a0, w, r0, std::min(nr, r0 + PANEL_ROWS)});
+          int b = a0 + w;
+          if (b < pend)
+            for (int ti = b; ti < nr; ti += TM)
+              for (int tj = b; tj < pend; tj += TM)
+                if (ti + TM - 1 >= tj) push_trail(s, ti, tj, a0, b, b, pend, nr, fl);
+        }
+        if ((int)panel_h_.size() > pbeg) launches_.push_back({L_PANEL, pbeg, (int)panel_h_.size()});
+        if ((int)trail_h_.size() > tbeg) {
+          launches_.push_back({L_TRAIL, tbeg, (int)trail_h_.size()});
+          trail_launch_flops_.push_back(fl);
+          trail_flops_static_ += fl;
+        }
+      }
+      int tbeg = (int)trail_h_.size();
+      double fl = 0;
+      for (int s : bigs) {
+        int nc = P.sn_ncol[s], nr = P.nr(s);
+        if (P0 >= nc) continue;
+        int P1 = std::min(P0 + PW, nc);
+        if (P1 >= nr) continue;
+        for (int ti = P1; ti < nr; ti += TM)
+          for (int tj = P1; tj < nr; tj += TM)
+            if (ti + TM - 1 >= tj) push_trail(s, ti, tj, P0, P1, P1, nr, nr, fl);
+      }
+      if ((int)trail_h_.size() > tbeg) {
+        launches_.push_back({L_TRAIL, tbeg, (int)trail_h_.size()});
+        trail_launch_flops_.push_back(fl);
+        trail_flops_static_ += fl;
+      }
+    }
+  }
+  asm_.upload(asm_h_, st_);
+  seg_.upload(seg_h_.empty() ? std::vector<AsmSeg>{{0, 0, 0}} : seg_h_, st_);
+  small_.upload(small_h_.empty() ? std::vector<int>{0} : small_h_, st_);
+  panel_.upload(panel_h_.empty() ? std::vector<PanelTask>{{0, 0, 0, 0, 0}} : panel_h_, st_);
+  trail_.upload(trail_h_.empty() ? std::vector<TrailTask>{{0, 0, 0, 0, 0, 0, 0}} : trail_h_, st_);
+}
+
+void DeviceKKT::factorize_async(const double* Kx, double extra_p, double extra_a, int mode,
+                                double eps, double delta) {
+  CX_HIP(hipMemsetAsync(fstat_.p, 0, 2 * sizeof(int), st_));
+  size_t tl = 0;
+  for (const Launch& L : launches_) {
+    const int n = L.end - L.beg;
+    if (n <= 0) continue;
+    switch (L.kind) {
+      case L_ASSEMBLE:
+        hipLaunchKernelGGL(k_assemble, dim3(n), dim3(256), 0, st_, asm_.p + L.beg, seg_.p, fofs_.p,
+                           nr_.p, nc_.p, first_.p, kcolptr_.p, ksrc_.p, kdst_.p, kdiag_.p, Kx,
+                           extra_p, extra_a, rmap_ofs_.p, rmap_.p, F_.p);
+        break;
+      case L_SMALL:
+        hipLaunchKernelGGL(k_factor_small, dim3(n), dim3(256), 0, st_, small_.p + L.beg, fofs_.p,
+                           nr_.p, nc_.p, first_.p, sgn_.p, F_.p, D_.p, mode, eps, delta, fstat_.p);
+        break;
+      case L_PANEL:
+        hipLaunchKernelGGL(k_panel, dim3(n), dim3(256), 0, st_, panel_.p + L.beg, fofs_.p, nr_.p,
+                           first_.p, sgn_.p, F_.p, D_.p, mode, eps, delta, fstat_.p);
+        break;
+      case L_TRAIL:
+        if (time_trail_) CX_HIP(hipEventRecord(ev0_, st_));
+        hipLaunchKernelGGL(k_trail, dim3(n), dim3(256), 0, st_, trail_.p + L.beg, fofs_.p, nr_.p,
+                           first_.p, D_.p, F_.p);
+        if (time_trail_) {
+          CX_HIP(hipEventRecord(ev1_, st_));
+          CX_HIP(hipEventSynchronize(ev1_));
+          float ms = 0;
+          CX_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+          trail_ms_ += ms;
+          trail_launches_ += 1;
+          trail_flops_ += trail_launch_flops_[tl];
+        }
+        ++tl;
+        break;
+    }
+    CX_LAUNCH_CHECK();
+  }
+  hipLaunchKernelGGL(k_dstats, dim3(1), dim3(256), 0, st_, D_.p, P_.N, dstat_.p);
+  CX_LAUNCH_CHECK();
+}
+
+FactorStats DeviceKKT::factor_stats() {
+  FactorStats fs;
+  int st[2] = {0, 0};
+  double mn = 0;
+  fstat_.download(st, 2, st_);
+  dstat_.download(&mn, 1, st_);
+  CX_HIP(hipStreamSynchronize(st_));
+  fs.zero_pivot = st[0];
+  fs.nreg = st[1];
+  fs.min_abs_d = mn;
+  return fs;
+}
+
+void DeviceKKT::solve_async(const double* rhs, double* x) {
+  const int N = P_.N, ns = P_.ns;
+  if (N == 0) return;
+  hipLaunchKernelGGL(k_zero_int, dim3(ceil_div(2 * ns + 2, 256)), dim3(256), 0, st_, done_.p, 2 * ns + 2);
+  hipLaunchKernelGGL(k_perm_in, dim3(ceil_div(N, 256)), dim3(256), 0, st_, perm_.p, rhs, xp_.p, N);
+  int* tick = done_.p + 2 * ns;
+  ++gen_;
+  hipLaunchKernelGGL(k_fwd, dim3(nfwd_), dim3(256), 0, st_, fwd_.p, tick, fofs_.p, nr_.p, nc_.p,
+                     first_.p, rowptr_.p, gptr_.p, gidx_.p, uofs_.p, child_ptr_.p, child_.p, nblk_.p,
+                     flag_ofs_.p, flags_.p, done_.p, gen_, F_.p, xp_.p, U_.p, serr_.p);
+  ++gen_;
+  hipLaunchKernelGGL(k_bwd, dim3(nbwd_), dim3(256), 0, st_, bwd_.p, tick + 1, fofs_.p, nr_.p, nc_.p,
+                     first_.p, rowptr_.p, rows_.p, parent_.p, nblkb_.p, flag_ofs_.p, flags_.p,
+                     done_.p + ns, gen_, F_.p, D_.p, xp_.p, serr_.p);
+  hipLaunchKernelGGL(k_perm_out, dim3(ceil_div(N, 256)), dim3(256), 0, st_, perm_.p, xp_.p, x, N);
+  CX_LAUNCH_CHECK();
+}
+
+void DeviceKKT::spmv_async(const double* Kx, const double* x, double* y) {
+  const int N = P_.N;
+  if (N == 0) return;
+  hipLaunchKernelGGL(k_spmv_sym, dim3(ceil_div(N, 256)), dim3(256), 0, st_, N, kfptr_.p, kfcol_.p,
+                     kfsrc_.p, Kx, x, y);
+  CX_LAUNCH_CHECK();
+}
+
+int DeviceKKT::take_solve_error() {
+  int e = 0;
+  serr_.download(&e, 1, st_);
+  CX_HIP(hipStreamSynchronize(st_));
+  if (e) serr_.zero(st_);
+  return e;
+}
+
+}  // namespace cx

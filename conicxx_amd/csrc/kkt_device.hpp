@@ -1,0 +1,121 @@
+// Device supernodal (multifrontal) LDL^T of the quasi-definite KKT matrix.
+//
+// Numeric counterpart of the reference's LDL^T backends: QdldlLdlt::factorize/
+// solve (src/conicxx/kkt/qdldl_ldlt.cpp:58-75) and RegularizedLdlt::factorize
+// (src/conicxx/kkt/regularized_ldlt.cpp:142-254).  The symbolic plan comes
+// from symbolic.cpp; every launch of a factorization or solve follows a static
+// schedule built once per setup().
+#pragma once
+
+#include <vector>
+
+#include "common.hpp"
+#include "symbolic.hpp"
+
+namespace cx {
+
+struct FactorStats {
+  double min_abs_d = 0;  // NaN if any pivot is NaN
+  int zero_pivot = 0;    // exact zero pivot met (checked pivot mode)
+  int nreg = 0;          // pivots corrected inline (regularizing mode)
+};
+
+struct AsmTask { int s, j0, j1, seg_beg, seg_end; };
+struct AsmSeg { int child, i0, i1; };
+struct PanelTask { int s, a, w, r0, r1; };  // r0 < 0: writer of the factored diagonal block
+struct TrailTask { int s, ti, tj, k0, k1, rlo, chi; };
+struct SolveTask { int s, blk; };
+
+enum LaunchKind { L_ASSEMBLE = 0, L_SMALL = 1, L_PANEL = 2, L_TRAIL = 3 };
+struct Launch { int kind, beg, end; };
+
+// Pivot modes.
+enum PivotMode { PIVOT_CHECK = 0, PIVOT_REGULARIZE = 1 };
+
+class DeviceKKT {
+ public:
+  DeviceKKT() = default;
+  DeviceKKT(const DeviceKKT&) = delete;
+  DeviceKKT& operator=(const DeviceKKT&) = delete;
+
+  // Symbolic analysis + device plan.  K: lower CSC pattern (diagonal present
+  // for every column).  nx: columns < nx expect positive pivots, others negative.
+  void setup(const LowerPattern& K, int nx, hipStream_t stream);
+
+  // Numeric factorization of the K lower values Kx (device, CSC order) with the
+  // diagonal bumped by +extra_p (rows < nx) and -extra_a (rows >= nx).
+  void factorize_async(const double* Kx, double extra_p, double extra_a, int mode, double eps,
+                       double delta);
+  FactorStats factor_stats();  // synchronizes the stream
+
+  // x = K^-1 rhs with the current factorization (device vectors of length N).
+  void solve_async(const double* rhs, double* x);
+  // y = K x, K symmetric from its lower values Kx.
+  void spmv_async(const double* Kx, const double* x, double* y);
+  // nonzero if a bounded wait in a solve sweep timed out (synchronizes)
+  int take_solve_error();
+
+  const SupernodalPlan& plan() const { return P_; }
+  int N() const { return P_.N; }
+  hipStream_t stream() const { return st_; }
+
+  // timing of the dense trailing-update kernel (HIP events around each launch)
+  void enable_trail_timing(bool on) { time_trail_ = on; }
+  void reset_trail_timing() { trail_ms_ = 0; trail_launches_ = 0; trail_flops_ = 0; }
+  double trail_ms() const { return trail_ms_; }
+  int trail_launches() const { return trail_launches_; }
+  double trail_flops() const { return trail_flops_; }
+  double trail_flops_per_factor() const { return trail_flops_static_; }
+
+ private:
+  void build_schedule();
+
+  hipStream_t st_ = nullptr;
+  SupernodalPlan P_;
+  int nx_ = 0;
+  // host copies of the schedule
+  std::vector<Launch> launches_;
+  std::vector<AsmTask> asm_h_;
+  std::vector<AsmSeg> seg_h_;
+  std::vector<int> small_h_;
+  std::vector<PanelTask> panel_h_;
+  std::vector<TrailTask> trail_h_;
+  double trail_flops_static_ = 0;
+  std::vector<double> trail_launch_flops_;
+  // device plan
+  DevBuf<long> fofs_;
+  DevBuf<int> nr_, nc_, first_, rowptr_, rows_, child_ptr_, child_;
+  DevBuf<double> sgn_;
+  DevBuf<int> kcolptr_, ksrc_;
+  DevBuf<long> kdst_;
+  DevBuf<signed char> kdiag_;
+  DevBuf<int> rmap_ofs_, rmap_;
+  DevBuf<AsmTask> asm_;
+  DevBuf<AsmSeg> seg_;
+  DevBuf<int> small_;
+  DevBuf<PanelTask> panel_;
+  DevBuf<TrailTask> trail_;
+  DevBuf<double> F_;   // fronts
+  DevBuf<double> D_;   // pivots, permuted order
+  DevBuf<int> fstat_;  // [0] zero pivot, [1] nreg
+  DevBuf<double> dstat_;
+  // solve plan
+  DevBuf<int> perm_;
+  DevBuf<SolveTask> fwd_, bwd_;
+  int nfwd_ = 0, nbwd_ = 0;
+  DevBuf<int> gptr_, gidx_;
+  DevBuf<long> uofs_;
+  DevBuf<int> flag_ofs_, nblk_, nblkb_, parent_;
+  DevBuf<int> flags_, done_, tickets_, serr_;
+  DevBuf<double> xp_, U_;
+  int gen_ = 0;
+  // K full symmetric CSR for spmv
+  DevBuf<int> kfptr_, kfcol_, kfsrc_;
+  // timing
+  bool time_trail_ = false;
+  double trail_ms_ = 0, trail_flops_ = 0;
+  int trail_launches_ = 0;
+  hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+};
+
+}  // namespace cx

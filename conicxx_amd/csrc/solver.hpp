@@ -1,0 +1,122 @@
+// Device interior-point solver behind the C-ABI (one problem per handle).
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "../../include/conicxx_c.h"
+#include "common.hpp"
+#include "ipm_kernels.hpp"
+#include "kkt_device.hpp"
+
+namespace cx {
+
+struct HostCSC {
+  int rows = 0, cols = 0;
+  std::vector<int> p, i;
+  std::vector<double> x;
+  int nnz() const { return p.empty() ? 0 : p[cols]; }
+};
+
+class Solver {
+ public:
+  Solver();
+  ~Solver();
+  bool setup(int n, int m, const int* Pp, const int* Pi, const double* Px, const double* q,
+             const int* Ap, const int* Ai, const double* Ax, const double* b, int zero_dim,
+             int nonneg_dim, int nsoc, const int* soc_dims, const cx_settings& st);
+  bool update_data(const int* Pp, const int* Pi, const double* Px, const double* q, const int* Ap,
+                   const int* Ai, const double* Ax, const double* b);
+  void set_warm_start(const double* x, const double* s, const double* z);
+  void set_settings(const cx_settings& st) { st_ = st; }
+  const cx_settings& settings() const { return st_; }
+  int solve();
+  int get_solution(double* x, double* s, double* z, double* obj, cx_info* info) const;
+  void dims(int* n, int* m) const { *n = n_; *m = m_; }
+  void kkt_stats(double* out8) const;
+  void profile(double* out8) const;
+
+ private:
+  // KKT (kkt_system.cpp)
+  bool kkt_update_pa(bool p, bool a);
+  bool kkt_update_scaling();
+  bool factor_retry();
+  bool try_factor(double ep, double ea);
+  bool escalate(const double* rhs, double* x);
+  double kkt_solve(const double* rhs, double* x);
+  // IPM pieces (solver_impl.cpp)
+  bool initial_point();
+  void shift_cold(double* v);
+  void ensure_interior_warm(double* v);
+  bool constant_solve();
+  void residuals();
+  bool tau_elim(const double* xv, const double* zv, double dtau_rhs, double dkappa_rhs, double* dtau);
+  bool affine_step();
+  bool combined_step(double sigma, double mu);
+  double step_length(const double* ds, const double* dz, double dtau, double dkappa);
+  void finalize();
+  double mu() const { return (dsz_ + tau_ * kappa_) / (double)(degree_ + 1); }
+  // helpers
+  std::vector<double> reduce(const RedBatch& b);
+  void pmul(const double* x, double* y);
+  void margins(const double* v, double* mn, double* ps);
+  void zero_zero_block(double* s);
+  int count_nonfinite(const double* a, int la, const double* b = nullptr, int lb = 0,
+                      const double* c = nullptr, int lc = 0);
+
+  hipStream_t stream_ = nullptr;
+  cx_settings st_{};
+  bool setup_done_ = false;
+  int n_ = 0, m_ = 0, N_ = 0;
+  int zero_dim_ = 0, nonneg_dim_ = 0, degree_ = 0;
+  std::vector<int> soc_dims_, soc_off_;
+  int P_nnz_ = 0;
+  // pattern fingerprints (kkt_system.cpp:110-113)
+  std::vector<int> Pp_ref_, Pi_ref_, Ap_ref_, Ai_ref_;
+  // equilibration (host) + device copies
+  std::vector<double> ed_, ee_;
+  double ec_ = 1.0;
+  DevBuf<double> d_ed_, d_ee_;
+  // problem data on device (scaled)
+  DevBuf<double> dPx_, dAx_, dq_, db_, dPx_new_, dAx_new_;
+  DevBuf<int> dPp_, dPi_, dAp_, dAi_;
+  DevBuf<int> pf_ptr_, pf_col_, pf_src_;  // full symmetric P (CSR)
+  DevBuf<int> ar_ptr_, ar_col_, ar_src_;  // A by rows
+  DevBuf<int> at_src_;                    // identity map for A' by columns
+  // KKT values and slots
+  DevBuf<double> Kx_;
+  int nnzK_ = 0;
+  DevBuf<int> pd_slot_, pd_src_, po_slot_, po_src_, a_slot_, a_src_;
+  int npd_ = 0, npo_ = 0, na_ = 0;
+  DevBuf<int> hs_slot_, hs_kind_, hs_blk_, hs_a_, hs_b_, hs_row_;
+  int nhs_ = 0;
+  DeviceKKT kkt_;
+  bool factorized_ = false;
+  double last_p_ = 0, last_a_ = 0;
+  // cones
+  ConeDev cone_{};
+  DevBuf<int> c_soc_off_, c_soc_dim_, c_small_, c_large_;
+  DevBuf<double> c_beta_, c_vv_, c_v_, c_w_, c_hsd_;
+  DevBuf<int> c_ident_;
+  DevBuf<double> identity_;
+  // iterate and workspace
+  DevBuf<double> x_, s_, z_, Px_, rx_, rz_, x1_, z1_, Px1_, dxa_, dza_, dsa_, dx_, dz_, ds_;
+  DevBuf<double> wx_, ws_, wz_;
+  DevBuf<double> rhs_, sol_, r_, t_, dsol_, wn_, wn2_, wm_, wm2_, dsk_;
+  DevBuf<double> red_partial_, red_out_;
+  double* host_red_ = nullptr;
+  double tau_ = 1.0, kappa_ = 1.0;
+  double rtau_ = 0, dqx_ = 0, dbz_ = 0, dsz_ = 0, dxPx_ = 0, nrx2_ = 0, nrz2_ = 0;
+  double dtaua_ = 0, dkappaa_ = 0, dtau_ = 0, dkappa_ = 0;
+  bool have_warm_ = false;
+  // solution
+  int status_ = CX_UNSOLVED, iterations_ = 0;
+  std::vector<double> xo_, so_, zo_;
+  double objective_ = 0, pres_ = 0, dres_ = 0, gap_ = 0, mu_out_ = 0;
+  double setup_time_ = 0, solve_time_ = 0;
+  // profile
+  double prof_fact_ms_ = 0, prof_solve_ms_ = 0, prof_total_ms_ = 0;
+  int prof_nfact_ = 0, prof_nsolve_ = 0;
+};
+
+}  // namespace cx

@@ -336,3 +336,27 @@ def kkt_certificate(prob, x, s, z):
     out["margin_s"] = float(mins if np.isfinite(mins) else 0.0)
     out["margin_z"] = float(minz if np.isfinite(minz) else 0.0)
     return out
+
+
+def kkt_lower_pattern(prob):
+    """Lower-triangle pattern of the reference's KKT matrix (kkt_system.cpp:54-107)."""
+    n, m = prob.n, prob.m
+    rows, cols = [np.arange(n)], [np.arange(n)]
+    Pc = prob.P.tocoo()
+    sel = Pc.row < Pc.col
+    rows.append(Pc.col[sel]); cols.append(Pc.row[sel])
+    Ac = prob.A.tocoo()
+    rows.append(n + Ac.row); cols.append(Ac.col)
+    rows.append(n + np.arange(prob.zero_dim + prob.nonneg_dim))
+    cols.append(n + np.arange(prob.zero_dim + prob.nonneg_dim))
+    off = prob.zero_dim + prob.nonneg_dim
+    for d in prob.soc_dims:
+        a, b = np.tril_indices(d)
+        rows.append(n + off + a); cols.append(n + off + b)
+        off += d
+    r = np.concatenate(rows); c = np.concatenate(cols)
+    K = sp.csc_matrix((np.ones(r.size), (r, c)), shape=(n + m, n + m))
+    K.sum_duplicates()
+    K.sort_indices()
+    K.data[:] = 1.0
+    return K

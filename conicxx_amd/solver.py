@@ -1,0 +1,305 @@
+"""Python mirror of the reference's public solver interface over the C-ABI.
+
+Same names, argument meaning and error behaviour as include/conicxx/solver.h,
+settings.h, cone_spec.h and solution.h: ``Solver.setup`` returns False on
+invalid input, ``Solver.updateData`` returns False on a pattern mismatch or a
+failed factorization (pass None for unchanged pieces), ``Solver.solve`` always
+returns a Solution with a definite Status and never raises for numerical
+trouble.  P is the upper triangle (entries below the diagonal are ignored by
+the KKT system), matrices are scipy.sparse CSC (int32 indices, float64
+values), cone rows are ordered Zero, Nonnegative, SOC...
+"""
+import ctypes
+import enum
+import sys
+from dataclasses import dataclass, field, fields
+
+import numpy as np
+import scipy.sparse as sp
+
+from . import _lib
+
+
+class Status(enum.IntEnum):  # include/conicxx/solution.h:9-16
+    Unsolved = 0
+    Solved = 1
+    PrimalInfeasible = 2
+    DualInfeasible = 3
+    MaxIterations = 4
+    NumericalError = 5
+
+
+def toString(status):  # src/conicxx/solution.cpp:5-21
+    return _lib.lib().cx_status_string(int(status)).decode()
+
+
+class LinearSolverBackend(enum.IntEnum):  # include/conicxx/settings.h:8-17
+    Eigen = 0
+    Qdldl = 1
+    RegularizedLdlt = 2
+
+
+class FactorPrecision(enum.IntEnum):
+    FP64 = 0
+    FP32Refined = 1
+
+
+@dataclass
+class Settings:  # include/conicxx/settings.h:21-62 (defaults identical)
+    tol_feas: float = 1e-8
+    tol_gap_abs: float = 1e-8
+    tol_gap_rel: float = 1e-8
+    tol_infeas: float = 1e-7
+    max_iter: int = 200
+    static_reg_P: float = 1e-8
+    static_reg_A: float = 1e-8
+    dynamic_reg_eps: float = 1e-14
+    dynamic_reg_delta: float = 1e-7
+    refine_max_iter: int = 3
+    refine_tol: float = 1e-12
+    max_step_fraction: float = 0.99
+    equilibrate: bool = True
+    equilibrate_max_iter: int = 10
+    equilibrate_min_scale: float = 1e-4
+    equilibrate_max_scale: float = 1e4
+    warm_start: bool = True
+    verbose: int = 0
+    record_timings: bool = False
+    validate_inputs: bool = True
+    linear_solver: LinearSolverBackend = LinearSolverBackend.Qdldl
+    factor_precision: FactorPrecision = FactorPrecision.FP64
+
+    def to_c(self):
+        c = _lib.CxSettings()
+        for name, ctype in _lib.CxSettings._fields_:
+            v = getattr(self, name)
+            setattr(c, name, int(v) if ctype is ctypes.c_int else float(v))
+        return c
+
+    @classmethod
+    def from_c(cls, c):
+        kw = {}
+        for f in fields(cls):
+            v = getattr(c, f.name)
+            if f.name == "linear_solver":
+                v = LinearSolverBackend(v)
+            elif f.name == "factor_precision":
+                v = FactorPrecision(v)
+            elif f.type is bool:
+                v = bool(v)
+            kw[f.name] = v
+        return cls(**kw)
+
+
+@dataclass
+class ConeSpec:  # include/conicxx/cone_spec.h:15-33
+    zero_dim: int = 0
+    nonneg_dim: int = 0
+    soc_dims: list = field(default_factory=list)
+
+    def totalDim(self):
+        return int(self.zero_dim) + int(self.nonneg_dim) + int(sum(self.soc_dims))
+
+    def isValid(self):
+        return self.zero_dim >= 0 and self.nonneg_dim >= 0 and all(d >= 1 for d in self.soc_dims)
+
+
+@dataclass
+class Info:  # include/conicxx/solution.h:20-28
+    iterations: int = 0
+    primal_residual: float = 0.0
+    dual_residual: float = 0.0
+    duality_gap: float = 0.0
+    mu: float = 0.0
+    setup_time_s: float = 0.0
+    solve_time_s: float = 0.0
+
+
+@dataclass
+class Solution:  # include/conicxx/solution.h:30-39
+    status: Status = Status.Unsolved
+    x: np.ndarray = None
+    s: np.ndarray = None
+    z: np.ndarray = None
+    objective: float = 0.0
+    info: Info = field(default_factory=Info)
+
+    def ok(self):
+        return self.status == Status.Solved
+
+
+def _warn_dense(name):
+    # the reference's dense->sparse conversion warns on stderr (src/conicxx/convert.cpp:11-18)
+    print(f"[conicxx] warning: dense matrix '{name}' converted to sparse; it carries no sparsity "
+          "for the solver to exploit", file=sys.stderr)
+
+
+def _as_csc(M, name, upper=False):
+    if sp.issparse(M):
+        C = sp.csc_matrix(M)
+    else:
+        D = np.asarray(M, dtype=np.float64)
+        _warn_dense(name)
+        if upper:
+            D = np.triu(D)
+            r, c = np.triu_indices(D.shape[0], 0, D.shape[1])
+        else:
+            r, c = np.indices(D.shape).reshape(2, -1)
+        C = sp.csc_matrix((D[r, c], (r, c)), shape=D.shape)  # keeps explicit zeros
+    C = C.copy()
+    C.sort_indices()
+    return C
+
+
+class Solver:
+    """conicxx::Solver (include/conicxx/solver.h:33-95) over one device handle."""
+
+    def __init__(self):
+        L = _lib.lib()
+        h = L.cx_create()
+        if not h:
+            raise RuntimeError("cx_create failed: no usable GPU (the solver runs on the device only)")
+        self._h = ctypes.c_void_p(h)
+        self._n = self._m = 0
+        self._P = self._A = None
+        self._solution = Solution()
+        self._settings = Settings()
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            _lib.lib().cx_destroy(h)
+            self._h = None
+
+    # -- setup / update -----------------------------------------------------
+    def setup(self, P, q, A, b, cone_spec, settings=None):
+        settings = settings or Settings()
+        if not isinstance(cone_spec, ConeSpec):
+            cone_spec = ConeSpec(*cone_spec)
+        Pc = _as_csc(P, "P", upper=True)
+        Ac = _as_csc(A, "A")
+        q = np.asarray(q, dtype=np.float64).reshape(-1)
+        b = np.asarray(b, dtype=np.float64).reshape(-1)
+        n = Pc.shape[0]
+        m = cone_spec.totalDim()
+        if (not cone_spec.isValid() or Pc.shape[0] != Pc.shape[1] or Ac.shape != (m, n)
+                or q.size != n or b.size != m):
+            return False
+        self._n, self._m = n, m
+        self._settings = settings
+        keep = []
+        a1, p1 = _lib.i32(Pc.indptr); a2, p2 = _lib.i32(Pc.indices); a3, p3 = _lib.f64(Pc.data)
+        a4, p4 = _lib.i32(Ac.indptr); a5, p5 = _lib.i32(Ac.indices); a6, p6 = _lib.f64(Ac.data)
+        a7, p7 = _lib.f64(q); a8, p8 = _lib.f64(b)
+        sd, sdp = _lib.i32(np.asarray(cone_spec.soc_dims, dtype=np.int32).reshape(-1))
+        keep += [a1, a2, a3, a4, a5, a6, a7, a8, sd]
+        st = settings.to_c()
+        ok = _lib.lib().cx_setup(self._h, n, m, p1, p2, p3, p7, p4, p5, p6, p8,
+                                 int(cone_spec.zero_dim), int(cone_spec.nonneg_dim), len(sd), sdp,
+                                 ctypes.byref(st))
+        self._solution = Solution()
+        return bool(ok)
+
+    def updateData(self, P=None, q=None, A=None, b=None):
+        keep = []
+        p1 = p2 = _lib.NULL_I
+        p3 = _lib.NULL_F
+        p4 = p5 = _lib.NULL_I
+        p6 = _lib.NULL_F
+        p7 = p8 = _lib.NULL_F
+        if P is not None:
+            if not sp.issparse(P) or P.shape != (self._n, self._n):
+                return False
+            Pc = sp.csc_matrix(P)
+            a1, p1 = _lib.i32(Pc.indptr); a2, p2 = _lib.i32(Pc.indices); a3, p3 = _lib.f64(Pc.data)
+            keep += [a1, a2, a3]
+        if A is not None:
+            if not sp.issparse(A) or A.shape != (self._m, self._n):
+                return False
+            Ac = sp.csc_matrix(A)
+            a4, p4 = _lib.i32(Ac.indptr); a5, p5 = _lib.i32(Ac.indices); a6, p6 = _lib.f64(Ac.data)
+            keep += [a4, a5, a6]
+        if q is not None:
+            q = np.asarray(q, dtype=np.float64).reshape(-1)
+            if q.size != self._n:
+                return False
+            a7, p7 = _lib.f64(q); keep.append(a7)
+        if b is not None:
+            b = np.asarray(b, dtype=np.float64).reshape(-1)
+            if b.size != self._m:
+                return False
+            a8, p8 = _lib.f64(b); keep.append(a8)
+        return bool(_lib.lib().cx_update_data(self._h, p1, p2, p3, p7, p4, p5, p6, p8))
+
+    def setWarmStart(self, x, s, z):
+        a, ap = _lib.f64(x); b_, bp = _lib.f64(s); c, cp = _lib.f64(z)
+        _lib.lib().cx_set_warm_start(self._h, ap, bp, cp)
+
+    def setSettings(self, settings):
+        self._settings = settings
+        st = settings.to_c()
+        _lib.lib().cx_set_settings(self._h, ctypes.byref(st))
+
+    def settings(self):
+        st = _lib.CxSettings()
+        _lib.lib().cx_get_settings(self._h, ctypes.byref(st))
+        return Settings.from_c(st)
+
+    # -- solve ----------------------------------------------------------------
+    def solve(self):
+        _lib.lib().cx_solve(self._h)
+        return self._fetch()
+
+    def solution(self):
+        return self._solution
+
+    def _fetch(self):
+        n, m = self._n, self._m
+        x = np.zeros(n); s = np.zeros(m); z = np.zeros(m)
+        obj = ctypes.c_double(0.0)
+        info = _lib.CxInfo()
+        st = _lib.lib().cx_get_solution(self._h, x.ctypes.data_as(_lib._f64p),
+                                        s.ctypes.data_as(_lib._f64p), z.ctypes.data_as(_lib._f64p),
+                                        ctypes.byref(obj), ctypes.byref(info))
+        self._solution = Solution(
+            status=Status(st), x=x, s=s, z=z, objective=obj.value,
+            info=Info(info.iterations, info.primal_residual, info.dual_residual, info.duality_gap,
+                      info.mu, info.setup_time_s, info.solve_time_s))
+        return self._solution
+
+    # -- extensions (no counterpart in the reference) ----------------------------
+    def kkt_stats(self):
+        out = np.zeros(8)
+        _lib.lib().cx_get_kkt_stats(self._h, out.ctypes.data_as(_lib._f64p))
+        keys = ["N", "nnz_K", "supernodes", "levels", "nnz_L", "flops", "max_front", "nnz_L_exact"]
+        return dict(zip(keys, out.tolist()))
+
+    def profile(self):
+        out = np.zeros(8)
+        _lib.lib().cx_get_profile(self._h, out.ctypes.data_as(_lib._f64p))
+        keys = ["factor_ms", "solve_ms", "other_ms", "factorizations", "kkt_solves",
+                "trail_avg_ms", "trail_launches", "trail_flops"]
+        return dict(zip(keys, out.tolist()))
+
+    # snake_case aliases
+    update_data = updateData
+    set_warm_start = setWarmStart
+    set_settings = setSettings
+
+
+def analyze_pattern(K_lower):
+    """Host symbolic analysis of a symmetric pattern given by its lower triangle."""
+    K = sp.csc_matrix(K_lower)
+    K.sort_indices()
+    N = K.shape[0]
+    a1, p1 = _lib.i32(K.indptr)
+    a2, p2 = _lib.i32(K.indices)
+    perm = np.zeros(N, dtype=np.int32)
+    cc = np.zeros(N, dtype=np.int32)
+    st = np.zeros(8)
+    ok = _lib.lib().cx_analyze_pattern(N, p1, p2, perm.ctypes.data_as(_lib._i32p),
+                                       cc.ctypes.data_as(_lib._i32p), st.ctypes.data_as(_lib._f64p))
+    if not ok:
+        raise RuntimeError("symbolic analysis failed")
+    keys = ["N", "nnz_K", "supernodes", "levels", "nnz_L", "flops", "max_front", "nnz_L_exact"]
+    return perm, cc, dict(zip(keys, st.tolist()))

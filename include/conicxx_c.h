@@ -1,0 +1,141 @@
+/*
+ * conicxx_c.h -- C-ABI of the MI355X conic interior-point solver.
+ *
+ * One opaque handle per problem; plain pointers and sizes; CSC matrices with
+ * int32 indices and double values (column-major, P upper triangle only);
+ * cone rows in the fixed order Zero, Nonnegative, SOC...  No exception
+ * crosses this boundary: every call returns a status or an int code.
+ * A handle is not thread-safe; distinct handles are independent.
+ */
+#ifndef CONICXX_C_H
+#define CONICXX_C_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status values in the order of enum class Status (include/conicxx/solution.h:9-16). */
+enum cx_status {
+  CX_UNSOLVED = 0,
+  CX_SOLVED = 1,
+  CX_PRIMAL_INFEASIBLE = 2,
+  CX_DUAL_INFEASIBLE = 3,
+  CX_MAX_ITERATIONS = 4,
+  CX_NUMERICAL_ERROR = 5
+};
+
+/* LinearSolverBackend (include/conicxx/settings.h:8-17).  All three select the
+ * device supernodal LDL^T; Eigen and Qdldl use the whole-matrix retry ladder on a
+ * failed pivot check, RegularizedLdlt corrects each pivot inline (5% cap). */
+enum cx_linear_solver { CX_LS_EIGEN = 0, CX_LS_QDLDL = 1, CX_LS_REGULARIZED = 2 };
+
+/* Numeric factorization precision (no counterpart in the reference, which is FP64 only). */
+enum cx_factor_precision { CX_FACTOR_FP64 = 0, CX_FACTOR_FP32_REFINED = 1 };
+
+/* Field-for-field the Settings aggregate (include/conicxx/settings.h:21-62), same
+ * defaults; the last field extends it. */
+typedef struct cx_settings {
+  double tol_feas;
+  double tol_gap_abs;
+  double tol_gap_rel;
+  double tol_infeas;
+  int max_iter;
+  double static_reg_P;
+  double static_reg_A;
+  double dynamic_reg_eps;
+  double dynamic_reg_delta;
+  int refine_max_iter;
+  double refine_tol;
+  double max_step_fraction;
+  int equilibrate;
+  int equilibrate_max_iter;
+  double equilibrate_min_scale;
+  double equilibrate_max_scale;
+  int warm_start;
+  int verbose;         /* accepted, unused (as in the reference) */
+  int record_timings;  /* accepted, unused */
+  int validate_inputs; /* accepted, unused */
+  int linear_solver;   /* enum cx_linear_solver */
+  int factor_precision; /* enum cx_factor_precision */
+} cx_settings;
+
+/* Info (include/conicxx/solution.h:20-28). */
+typedef struct cx_info {
+  int iterations;
+  double primal_residual;
+  double dual_residual;
+  double duality_gap;
+  double mu;
+  double setup_time_s;
+  double solve_time_s;
+} cx_info;
+
+typedef struct cx_solver cx_solver;
+
+/* Settings{} defaults (settings.h:23-61). */
+void cx_default_settings(cx_settings* s);
+
+/* Solver::Solver / ~Solver (include/conicxx/solver.h:36-37).  NULL if no GPU. */
+cx_solver* cx_create(void);
+void cx_destroy(cx_solver* h);
+
+/* Solver::setup(P, q, A, b, cone_spec, settings) (solver.h:46-47; solver_impl.cpp:8-55).
+ * P: n x n upper-triangular CSC (lower entries ignored by the KKT, as in the
+ * reference); A: m x n CSC; m must equal zero_dim + nonneg_dim + sum(soc_dims).
+ * Copies all data.  Returns 1 on success, 0 on invalid input or KKT setup failure. */
+int cx_setup(cx_solver* h, int n, int m, const int* P_colptr, const int* P_rowind,
+             const double* P_val, const double* q, const int* A_colptr, const int* A_rowind,
+             const double* A_val, const double* b, int zero_dim, int nonneg_dim, int n_soc,
+             const int* soc_dims, const cx_settings* settings);
+
+/* Solver::updateData(P*, q*, A*, b*) (solver.h:77; solver_impl.cpp:57-91;
+ * kkt_system.cpp:10-19,119-137).  A NULL value pointer leaves that piece
+ * unchanged; P_colptr/P_rowind (A_colptr/A_rowind) must accompany P_val (A_val)
+ * for the pattern check.  Returns 0 on pattern mismatch or factorization failure. */
+int cx_update_data(cx_solver* h, const int* P_colptr, const int* P_rowind, const double* P_val,
+                   const double* q, const int* A_colptr, const int* A_rowind,
+                   const double* A_val, const double* b);
+
+/* Solver::setWarmStart(x, s, z) in original units (solver.h:81; solver_impl.cpp:93-100). */
+void cx_set_warm_start(cx_solver* h, const double* x, const double* s, const double* z);
+
+/* Solver::setSettings / settings() (solver.h:83-84). */
+void cx_set_settings(cx_solver* h, const cx_settings* settings);
+void cx_get_settings(const cx_solver* h, cx_settings* out);
+
+/* Solver::solve() (solver.h:89; solver_impl.cpp:367-466).  Returns the status. */
+int cx_solve(cx_solver* h);
+
+/* Solver::solution() (solver.h:91): copies x (n), s (m), z (m), objective, info
+ * (any pointer may be NULL).  Returns the status. */
+int cx_get_solution(const cx_solver* h, double* x, double* s, double* z, double* objective,
+                    cx_info* info);
+
+/* toString(Status) (src/conicxx/solution.cpp:5-21). */
+const char* cx_status_string(int status);
+
+/* Problem sizes of the last setup (n, m) and symbolic statistics of the KKT plan:
+ * stats[0..7] = N, nnz(K lower), supernodes, levels, nnz(L) stored, factor flops,
+ * largest front, nnz(L) exact.  No counterpart in the reference. */
+int cx_get_dims(const cx_solver* h, int* n, int* m);
+void cx_get_kkt_stats(const cx_solver* h, double* stats8);
+
+/* Device-timed breakdown of the last solve (milliseconds): [0] numeric
+ * factorizations, [1] triangular solves, [2] everything else, [3] number of
+ * factorizations, [4] number of KKT solves (incl. refinement steps), [5] HIP-event
+ * average duration of the dominant dense trailing-update kernel (ms),
+ * [6] its launches, [7] sum of its algorithmic flops.  No counterpart in the reference. */
+void cx_get_profile(const cx_solver* h, double* out8);
+
+/* Host-only symbolic analysis of a symmetric pattern (lower CSC, diagonal
+ * included), for testing: writes the final permutation (new->old, N ints) and
+ * the column counts of L (N ints) when non-NULL, and stats8 as above.
+ * Returns 1 on success.  Replaces qdldl_ldlt.cpp:22-56 (ordering + etree + Lnz). */
+int cx_analyze_pattern(int N, const int* colptr, const int* rowind, int* perm, int* colcount,
+                       double* stats8);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CONICXX_C_H */

@@ -1,0 +1,169 @@
+"""GPU parity tests: the device solver (through the C-ABI) against the CPU oracle
+on the same seeded inputs, and against the reference's known answers.
+
+Parity bar (floating point): same Status and same iteration count as the
+oracle, and x, s, z, objective within 1e-8 relative
+(max|a-b| / max(1, max|b|)); at sizes the oracle cannot finish, the KKT
+optimality certificate.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from conicxx_amd import ConeSpec, LinearSolverBackend, Settings, Solver, Status
+from conicxx_amd import problems as PR
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-8
+
+
+def rel(a, b):
+    return float(np.abs(np.asarray(a) - np.asarray(b)).max(initial=0.0) / max(1.0, np.abs(b).max(initial=0.0)))
+
+
+def gpu_solve(p, settings=None):
+    s = Solver()
+    assert s.setup(p.P, p.q, p.A, p.b, ConeSpec(p.zero_dim, p.nonneg_dim, p.soc_dims), settings)
+    return s, s.solve()
+
+
+def assert_parity(sol, ref, tol=TOL, iters=True):
+    assert int(sol.status) == ref["status"], (sol.status, ref["status"])
+    if iters:
+        assert sol.info.iterations == ref["iterations"], (sol.info.iterations, ref["iterations"])
+    errs = {k: rel(getattr(sol, k), ref[k]) for k in ("x", "s", "z")}
+    errs["objective"] = abs(sol.objective - ref["objective"]) / max(1.0, abs(ref["objective"]))
+    print("parity", {k: f"{v:.2e}" for k, v in errs.items()}, "iters", sol.info.iterations)
+    for k, v in errs.items():
+        assert v <= tol, (k, v)
+
+
+@pytest.mark.parametrize("case", PR.kat_problems(), ids=lambda c: c[0])
+def test_known_answers(case, oracle):
+    name, p, status, x, obj, tol = case
+    s, sol = gpu_solve(p)
+    assert int(sol.status) == status
+    if x is not None:
+        np.testing.assert_allclose(sol.x, x, atol=tol)
+        c = PR.kkt_certificate(p, sol.x, sol.s, sol.z)
+        assert c["stationarity"] < tol and c["feasibility"] < tol and c["complementarity"] < tol
+    if obj is not None:
+        assert abs(sol.objective - obj) < tol
+    ref = oracle.solve(p.P, p.q, p.A, p.b, p.zero_dim, p.nonneg_dim, p.soc_dims)
+    assert int(sol.status) == ref["status"]
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_config0_parity(seed, oracle):
+    p = PR.config0(seed=seed)
+    s, sol = gpu_solve(p)
+    ref = oracle.solve(p.P, p.q, p.A, p.b, p.zero_dim, p.nonneg_dim, p.soc_dims)
+    assert_parity(sol, ref)
+
+
+@pytest.mark.parametrize("index", [0, 1, 5])
+def test_contact_qp_parity(index, oracle):
+    p = PR.config2_problem(index)
+    s, sol = gpu_solve(p)
+    ref = oracle.solve(p.P, p.q, p.A, p.b, p.zero_dim, p.nonneg_dim, p.soc_dims)
+    assert_parity(sol, ref)
+
+
+def test_dense_diff_qp_parity(oracle):
+    p = PR.config3_problem(0)
+    s, sol = gpu_solve(p)
+    ref = oracle.solve(p.P, p.q, p.A, p.b, p.zero_dim, p.nonneg_dim, p.soc_dims)
+    assert_parity(sol, ref)
+
+
+@pytest.mark.parametrize("scale", [0.05, 0.1])
+def test_large_sparse_scaled_parity(scale, oracle):
+    """config 1 scaled down: large dense fronts (blocked panels + MFMA updates)."""
+    p = PR.config1(scale=scale)
+    s, sol = gpu_solve(p)
+    st = s.kkt_stats()
+    print("kkt", st)
+    assert st["max_front"] > 256  # the blocked MFMA path is exercised
+    ref = oracle.solve(p.P, p.q, p.A, p.b, p.zero_dim, p.nonneg_dim, p.soc_dims)
+    assert_parity(sol, ref)
+
+
+def test_regularized_backend(oracle):
+    p = PR.config0(seed=2)
+    st = Settings(linear_solver=LinearSolverBackend.RegularizedLdlt)
+    s, sol = gpu_solve(p, st)
+    ref = oracle.solve(p.P, p.q, p.A, p.b, p.zero_dim, p.nonneg_dim, p.soc_dims, linear_solver=2)
+    assert_parity(sol, ref)
+
+
+def test_update_reuse_and_pattern_mismatch(oracle):
+    """test/solver/test_update_reuse.cpp"""
+    targets = [np.array([1.0, 2.0]), np.array([-1.0, 3.0]), np.array([0.5, -0.5])]
+    p0 = PR.box_qp(targets[0])
+    s = Solver()
+    assert s.setup(p0.P, p0.q, p0.A, p0.b, ConeSpec(0, 2, []))
+    o = oracle.OracleSolver()
+    assert o.setup(p0.P, p0.q, p0.A, p0.b, 0, 2, [])
+    for t, c in enumerate(targets):
+        p = PR.box_qp(c)
+        if t:
+            assert s.updateData(q=p.q)
+            assert o.update_data(q=p.q)
+        sol = s.solve()
+        ref = o.solve()
+        assert_parity(sol, ref)
+        np.testing.assert_allclose(sol.x, np.maximum(c, 0), atol=1e-6)
+    A_bad = sp.csc_matrix(([-1.0, 0.3, -1.0], ([0, 0, 1], [0, 1, 1])), shape=(2, 2))
+    A_bad.sort_indices()
+    assert not s.updateData(A=A_bad)
+    assert s.setup(p0.P, p0.q, A_bad, p0.b, ConeSpec(0, 2, []))
+    assert s.solve().status == Status.Solved
+
+
+def test_friction_demo_warm_timesteps(oracle):
+    """examples/friction_contact_demo.cpp: 4 warm-started timesteps."""
+    p0 = PR.friction_demo_problem([5.0, 1.0, 1.0])
+    s = Solver()
+    assert s.setup(p0.P, p0.q, p0.A, p0.b, ConeSpec(0, 0, [3]))
+    o = oracle.OracleSolver()
+    assert o.setup(p0.P, p0.q, p0.A, p0.b, 0, 0, [3])
+    for t in range(4):
+        f = np.array([5.0 - t, 1.0 + 2.0 * t, 1.0 + 0.5 * t])
+        assert s.updateData(q=-f) and o.update_data(q=-f)
+        assert_parity(s.solve(), o.solve())
+
+
+def test_warm_start_user_point(oracle):
+    p = PR.config0(seed=4)
+    s = Solver()
+    assert s.setup(p.P, p.q, p.A, p.b, ConeSpec(p.zero_dim, p.nonneg_dim, p.soc_dims))
+    o = oracle.OracleSolver()
+    assert o.setup(p.P, p.q, p.A, p.b, p.zero_dim, p.nonneg_dim, p.soc_dims)
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal(p.n); sl = np.abs(rng.standard_normal(p.m)); z = np.abs(rng.standard_normal(p.m))
+    s.setWarmStart(x, sl, z)
+    o.set_warm_start(x, sl, z)
+    assert_parity(s.solve(), o.solve())
+
+
+def test_failure_statuses(oracle):
+    p = PR.box_qp([1.0, 2.0])
+    s, sol = gpu_solve(p, Settings(max_iter=1))
+    assert sol.status == Status.MaxIterations
+    q = p.q.copy(); q[0] = np.nan
+    s = Solver()
+    assert s.setup(p.P, q, p.A, p.b, ConeSpec(0, 2, []))
+    assert s.solve().status == Status.NumericalError
+    s = Solver()
+    assert s.solve().status == Status.NumericalError  # solve() before setup()
+    # invalid input is rejected
+    assert not s.setup(p.P, p.q, p.A, p.b, ConeSpec(0, 3, []))
+    assert not s.setup(p.P, p.q, p.A, p.b, ConeSpec(-1, 3, []))
+
+
+def test_no_equilibration_parity(oracle):
+    p = PR.config0(seed=5)
+    s, sol = gpu_solve(p, Settings(equilibrate=False))
+    ref = oracle.solve(p.P, p.q, p.A, p.b, p.zero_dim, p.nonneg_dim, p.soc_dims, equilibrate=0)
+    assert_parity(sol, ref)
