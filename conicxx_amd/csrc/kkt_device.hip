@@ -101,13 +101,10 @@ __device__ void ldl_lds(double* A, int n, int nc, const double* sgn, int mode, d
       A[i * LDSP + k] = c * dinv;
     }
     __syncthreads();
-    const int m = n - k - 1;
-    for (int idx = tid; idx < m * m; idx += blockDim.x) {
-      int ii = idx % m, jj = idx / m;
-      if (ii >= jj) {
-        int i = k + 1 + ii, j = k + 1 + jj;
-        A[i * LDSP + j] -= A[i * LDSP + k] * colk[j];
-      }
+    // trailing update, thread (row i, column group g): columns k+1+g, k+5+g, ...
+    for (int i = k + 1 + (tid & 63); i < n; i += 64) {
+      const double lik = A[i * LDSP + k];
+      for (int j = k + 1 + (tid >> 6); j <= i; j += 4) A[i * LDSP + j] -= lik * colk[j];
     }
     __syncthreads();
   }
@@ -302,7 +299,7 @@ __global__ __launch_bounds__(256) void k_trail(const TrailTask* __restrict__ tas
     }
     __syncthreads();
   }
-  // write back: lane holds rows 4*(lane/16)+r, column lane%16 of each 16x16 tile
+  // write back: v_mfma_f64_16x16x4f64 leaves D[lane/16 + 4r][lane%16] in register r
 #pragma unroll
   for (int mi = 0; mi < 4; ++mi)
 #pragma unroll
@@ -311,7 +308,7 @@ __global__ __launch_bounds__(256) void k_trail(const TrailTask* __restrict__ tas
       if (j >= T.chi) continue;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int i = T.ti + wr + mi * 16 + 4 * (lane >> 4) + r;
+        const int i = T.ti + wr + mi * 16 + (lane >> 4) + 4 * r;
         if (i < nr && i >= j && i >= T.rlo) F[base + (long)j * nr + i] -= acc[mi][ni][r];
       }
     }
@@ -360,7 +357,11 @@ __global__ void k_perm_out(const int* __restrict__ perm, const double* __restric
 }
 
 // forward: L y = b on the permuted vector; pivot rows of each front get y,
-// non-pivot rows produce the update vector U_s for the parent.
+// non-pivot rows produce the update vector U_s for the parent.  One task per
+// 64-row block; each wave owns the column blocks cb = wv (mod 4) and polls only
+// the flag of the block it needs next, so far-away blocks (solved long ago)
+// are consumed without waiting.  The diagonal block is staged in LDS before
+// any wait, and the in-block substitution reads it from LDS.
 __global__ __launch_bounds__(256) void k_fwd(
     const SolveTask* __restrict__ tasks, int* ticket, const long* __restrict__ fofs,
     const int* __restrict__ nrv, const int* __restrict__ ncv, const int* __restrict__ firstv,
@@ -370,7 +371,7 @@ __global__ __launch_bounds__(256) void k_fwd(
     const double* __restrict__ F, double* xp, double* U, int* err) {
   __shared__ int tk;
   __shared__ double part[4][SB];
-  __shared__ double yblk[SB];
+  __shared__ double Ld[SB * LDSP];  // Ld[row][col] of the diagonal block
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   if (tid == 0) tk = atomicAdd(ticket, 1);
   __syncthreads();
@@ -379,6 +380,39 @@ __global__ __launch_bounds__(256) void k_fwd(
   const long base = fofs[s];
   const int r0 = rb * SB;
   const int ncb = (nc + SB - 1) / SB;
+  const int r = r0 + lane;
+  const bool diag = rb < ncb;
+  const int kc = diag ? min(SB, nc - r0) : 0;
+  // stage the diagonal block (independent of every wait)
+  for (int c = wv; c < kc; c += 4) Ld[lane * LDSP + c] = (r < nr) ? F[base + (long)(r0 + c) * nr + r] : 0.0;
+  double acc = 0.0;
+  const int ncbk = min(rb, ncb);
+  for (int cb = wv; cb < ncbk; cb += 4) {
+    if (lane == 0) wait_flag(&flags[flag_ofs[s] + cb], gen, err);
+    __builtin_amdgcn_wave_barrier();
+    const int cbase = cb * SB;
+    const double yv = (cbase + lane < nc) ? load_acq(&xp[first + cbase + lane]) : 0.0;
+    const int cend = min(SB, nc - cbase);
+    // every lane runs the shuffles (sources must be active); rows past nr read
+    // a clamped, in-bounds address and contribute nothing
+    const bool rv = r < nr;
+    const double* Fc = F + base + (long)cbase * nr + (rv ? r : nr - 1);
+    if (cend == SB) {
+#pragma unroll 16
+      for (int c = 0; c < SB; ++c) {
+        const double yc = __shfl(yv, c, 64);
+        const double f = Fc[(long)c * nr];
+        acc += rv ? f * yc : 0.0;
+      }
+    } else {
+      for (int c = 0; c < cend; ++c) {
+        const double yc = __shfl(yv, c, 64);
+        const double f = Fc[(long)c * nr];
+        acc += rv ? f * yc : 0.0;
+      }
+    }
+  }
+  part[wv][lane] = acc;
   if (tid == 0) {
     for (int t = child_ptr[s]; t < child_ptr[s + 1]; ++t) {
       int c = child[t];
@@ -386,42 +420,17 @@ __global__ __launch_bounds__(256) void k_fwd(
     }
   }
   __syncthreads();
-  const int r = r0 + lane;
-  double acc = 0.0;
-  const int ncbk = min(rb, ncb);
-  for (int cb = 0; cb < ncbk; ++cb) {
-    if (tid == 0) wait_flag(&flags[flag_ofs[s] + cb], gen, err);
-    __syncthreads();
-    if (tid < SB) {
-      int c = cb * SB + tid;
-      yblk[tid] = (c < nc) ? load_acq(&xp[first + c]) : 0.0;
-    }
-    __syncthreads();
-    if (r < nr) {
-      const int cend = min(SB, nc - cb * SB);
-      for (int q = wv; q < cend; q += 4) {
-        const int c = cb * SB + q;
-        acc += F[base + (long)c * nr + r] * yblk[q];
-      }
-    }
-    __syncthreads();
-  }
-  part[wv][lane] = acc;
-  __syncthreads();
   if (wv == 0) {
     double val = 0.0;
     if (r < nr) {
       val = (r < nc) ? xp[first + r] : 0.0;
       const int gr = rowptr[s] + r;
       for (int g = gptr[gr]; g < gptr[gr + 1]; ++g) val += load_acq(&U[gidx[g]]);
-      val -= part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane];
+      val -= (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
     }
-    if (rb < ncb) {
-      const int kc = min(SB, nc - r0);
-      for (int k = 0; k < kc; ++k) {
-        double yk = __shfl(val, k, 64);
-        if (lane > k && r < nr) val -= F[base + (long)(r0 + k) * nr + r] * yk;
-      }
+    for (int k = 0; k < kc; ++k) {
+      const double yk = __shfl(val, k, 64);
+      if (lane > k) val -= Ld[lane * LDSP + k] * yk;
     }
     if (r < nr) {
       if (r < nc)
@@ -433,13 +442,16 @@ __global__ __launch_bounds__(256) void k_fwd(
   __threadfence();
   __syncthreads();
   if (tid == 0) {
-    if (rb < ncb)
+    if (diag)
       __hip_atomic_store(&flags[flag_ofs[s] + rb], gen, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_fetch_add(&done[s], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
-// backward: D^-1 then L^T x = y, one column block of pivots per task.
+// backward: D^-1 then L^T x = y, one column block of pivots per task.  Rows
+// below the block are visited farthest first (non-pivot rows, final once the
+// parent front is done, then later pivot blocks in descending order), with
+// lane = row: every lane keeps 64 column partials, reduced across lanes once.
 __global__ __launch_bounds__(256) void k_bwd(
     const SolveTask* __restrict__ tasks, int* ticket, const long* __restrict__ fofs,
     const int* __restrict__ nrv, const int* __restrict__ ncv, const int* __restrict__ firstv,
@@ -447,8 +459,8 @@ __global__ __launch_bounds__(256) void k_bwd(
     const int* __restrict__ nblkb, const int* __restrict__ flag_ofs, int* flags, int* done, int gen,
     const double* __restrict__ F, const double* __restrict__ Dg, double* xp, int* err) {
   __shared__ int tk;
-  __shared__ double xs[4][SB];
   __shared__ double part[4][SB];
+  __shared__ double Ld[SB * LDSP];  // Ld[row][col] of the diagonal block
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   if (tid == 0) tk = atomicAdd(ticket, 1);
   __syncthreads();
@@ -457,54 +469,58 @@ __global__ __launch_bounds__(256) void k_bwd(
   const long base = fofs[s];
   const int c0 = cb * SB, kc = min(SB, nc - c0);
   const int ncb = (nc + SB - 1) / SB;
+  for (int c = wv; c < kc; c += 4) Ld[lane * LDSP + c] = (lane < kc) ? F[base + (long)(c0 + c) * nr + c0 + lane] : 0.0;
   if (tid == 0 && parent[s] >= 0) wait_count(&done[parent[s]], nblkb[parent[s]], err);
   __syncthreads();
-  // acc[c] = sum over rows below this block of L[r][c0+c] x[r]
-  double acc = 0.0;
-  const int col = c0 + lane;
-  const int rstart = c0 + kc;  // first row below the diagonal block
+  double acc[SB];
+#pragma unroll
+  for (int c = 0; c < SB; ++c) acc[c] = 0.0;
+  const int rstart = c0 + kc;
   const int nchunks = (nr - rstart + SB - 1) / SB;
-  for (int q = wv; q < nchunks; q += 4) {
+  for (int q = nchunks - 1 - wv; q >= 0; q -= 4) {
     const int R = rstart + q * SB;
-    // pivot rows of later column blocks of this front must be final
     if (lane == 0 && R < nc) {
-      const int b0 = R / SB, b1 = min(R + SB, nc) - 1;
-      for (int b = b0; b <= b1 / SB; ++b)
+      // pivot rows [R, min(R+SB, nc)) belong to blocks b > cb of this front
+      const int b0 = R / SB, b1 = (min(R + SB, nc) - 1) / SB;
+      for (int b = b1; b >= b0; --b)
         if (b > cb && b < ncb) wait_flag(&flags[flag_ofs[s] + b], gen, err);
     }
     __builtin_amdgcn_wave_barrier();
-    {
-      const int rr = R + lane;
-      double xv = 0.0;
-      if (rr < nr) {
-        int gidxr = (rr < nc) ? first + rr : rows[rowptr[s] + rr];
-        xv = load_acq(&xp[gidxr]);
-      }
-      xs[wv][lane] = xv;
+    const int rr = R + lane;
+    double xv = 0.0;
+    if (rr < nr) {
+      const int g = (rr < nc) ? first + rr : rows[rowptr[s] + rr];
+      xv = load_acq(&xp[g]);
     }
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
-    if (lane < kc) {
-      const int rlen = min(SB, nr - R);
-      const double* Lc = F + base + (long)col * nr + R;
-      for (int t = 0; t < rlen; ++t) acc += Lc[t] * xs[wv][t];
+    if (rr < nr) {
+      const double* Lr = F + base + (long)c0 * nr + rr;
+#pragma unroll
+      for (int c = 0; c < SB; ++c)
+        if (c < kc) acc[c] += Lr[(long)c * nr] * xv;
     }
-    __builtin_amdgcn_wave_barrier();
   }
-  part[wv][lane] = acc;
+  // reduce each column partial across the wave: lane c ends with column c
+  double mine = 0.0;
+#pragma unroll
+  for (int c = 0; c < SB; ++c) {
+    double v = acc[c];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (lane == c) mine = v;
+  }
+  part[wv][lane] = mine;
   __syncthreads();
   if (wv == 0) {
     double y = 0.0;
     if (lane < kc) {
-      const double dv = Dg[first + col];
-      const double dinv = 1.0 / dv;
-      y = xp[first + col] * dinv - (part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane]);
+      const double dinv = 1.0 / Dg[first + c0 + lane];
+      y = xp[first + c0 + lane] * dinv - ((part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]));
     }
     for (int k = kc - 1; k >= 0; --k) {
-      double xk = __shfl(y, k, 64);
-      if (lane < k) y -= F[base + (long)(c0 + lane) * nr + (c0 + k)] * xk;
+      const double xk = __shfl(y, k, 64);
+      if (lane < k) y -= Ld[k * LDSP + lane] * xk;
     }
-    if (lane < kc) xp[first + col] = y;
+    if (lane < kc) xp[first + c0 + lane] = y;
   }
   __threadfence();
   __syncthreads();

@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <map>
 #include <numeric>
@@ -747,6 +748,12 @@ int Solver::solve() {  // solver_impl.cpp:367-466
       }
       residuals();
       const double mu_now = mu();
+      if (st_.verbose >= 1) {
+        std::fprintf(stderr, "[conicxx_amd] iter %3d mu %.3e |rx| %.3e |rz| %.3e t %.3fs (fact %.1f ms, solve %.1f ms)\n",
+                     iter, mu_now, std::sqrt(nrx2_) / tau_, std::sqrt(nrz2_) / tau_, now_s() - t0,
+                     prof_fact_ms_, prof_solve_ms_);
+        std::fflush(stderr);
+      }
       const double nrx = std::sqrt(nrx2_) / tau_;
       const double nrz = std::sqrt(nrz2_) / tau_;
       const double pobj = (0.5 * dxPx_ / tau_ + dqx_) / tau_;
