@@ -9,8 +9,8 @@
 //   k_assemble      zero-fill, scatter K entries (+ diagonal bump), extend-add
 //                   of the children's update matrices (pull, deterministic)
 //   k_factor_small  fronts with nr <= 64, factored whole in LDS
-//   k_panel         64-column sub-panel: diagonal block LDL^T in LDS plus the
-//                   triangular solve of 64 rows below it (LDS, 4 lanes per row)
+//   k_diag          64-column sub-panel: LDL^T of the diagonal block in LDS
+//   k_trsm          the rows below it, L21 = R L11^-T D^-1 (64-row chunks, LDS)
 //   k_trail         lower-triangle trailing update C -= L D L^T on FP64 MFMA
 //                   (v_mfma_f64_16x16x4f64), 128x128 tiles
 // Triangular solves: one ticketed launch per sweep over the whole tree with
@@ -37,7 +37,7 @@ constexpr int TM = 128;       // trailing-update tile
 constexpr int KC = 16;        // K chunk staged in LDS per step
 constexpr int SB = 64;        // solve block
 constexpr int LDSP = NB + 1;  // padded LDS row stride for 64x64 blocks
-constexpr long SPIN_LIMIT = 1L << 22;  // ~0.2 s of polling
+constexpr long SPIN_LIMIT = 1L << 21;  // ~0.2 s of polling
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
@@ -59,29 +59,39 @@ __device__ inline double load_acq(const double* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Bounded waits: give up (and flag the sweep as failed) after SPIN_LIMIT polls,
-// or at once when another workgroup has already flagged a failure.
+// Bounded waits: poll with relaxed loads (an acquire per poll would
+// invalidate the CU's L1 for every co-resident wave), back off between polls,
+// and issue one acquire fence once the flag is seen.  Give up (flagging the
+// sweep as failed) after SPIN_LIMIT polls, or at once when another workgroup
+// has already flagged a failure.  Flags and counters live one per 128-byte
+// line (FSTRIDE ints apart) so pollers of different blocks do not share lines.
+constexpr int FSTRIDE = 32;
+
 __device__ inline bool wait_flag(int* f, int want, int* err) {
   long spins = 0;
-  while (__hip_atomic_load(f, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != want) {
-    if (++spins > SPIN_LIMIT || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+  while (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != want) {
+    if (++spins > SPIN_LIMIT ||
+        ((spins & 63) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
       atomicOr(err, 1);
       return false;
     }
-    __builtin_amdgcn_s_sleep(1);
+    __builtin_amdgcn_s_sleep(2);
   }
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   return true;
 }
 
 __device__ inline bool wait_count(int* f, int want, int* err) {
   long spins = 0;
-  while (__hip_atomic_load(f, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < want) {
-    if (++spins > SPIN_LIMIT || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+  while (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < want) {
+    if (++spins > SPIN_LIMIT ||
+        ((spins & 63) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
       atomicOr(err, 1);
       return false;
     }
-    __builtin_amdgcn_s_sleep(1);
+    __builtin_amdgcn_s_sleep(2);
   }
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   return true;
 }
 
@@ -186,18 +196,16 @@ __global__ __launch_bounds__(256) void k_factor_small(
 }
 
 // ---------------------------------------------------------------------------
-// sub-panel: diagonal block LDL^T (redundant per task) + TRSM of 64 rows
+// sub-panel, step 1: LDL^T of the diagonal block, one workgroup per front
 // ---------------------------------------------------------------------------
 constexpr int PANEL_ROWS = 64;
 
-__global__ __launch_bounds__(256) void k_panel(
+__global__ __launch_bounds__(256) void k_diag(
     const PanelTask* __restrict__ tasks, const long* __restrict__ fofs, const int* __restrict__ nrv,
     const int* __restrict__ firstv, const double* __restrict__ sgn, double* F, double* Dg, int mode,
     double eps, double delta, int* stats) {
   __shared__ double L11[NB * LDSP];
-  __shared__ double X[PANEL_ROWS * LDSP];
   __shared__ double colk[NB];
-  __shared__ double dinv[NB];
   const PanelTask T = tasks[blockIdx.x];
   const int s = T.s, nr = nrv[s], first = firstv[s], a = T.a, w = T.w;
   const long base = fofs[s];
@@ -207,23 +215,37 @@ __global__ __launch_bounds__(256) void k_panel(
     if (i >= j) L11[i * LDSP + j] = F[base + (long)(a + j) * nr + (a + i)];
   }
   __syncthreads();
-  const bool writer = T.r0 < 0;
-  ldl_lds(L11, w, w, sgn + first + a, mode, eps, delta, stats, writer, colk);
-  if (writer) {
-    for (int idx = tid; idx < w * w; idx += blockDim.x) {
-      int j = idx / w, i = idx % w;
-      if (i >= j) F[base + (long)(a + j) * nr + (a + i)] = L11[i * LDSP + j];
-    }
-    for (int k = tid; k < w; k += blockDim.x) Dg[first + a + k] = L11[k * LDSP + k];
-    return;
+  ldl_lds(L11, w, w, sgn + first + a, mode, eps, delta, stats, true, colk);
+  for (int idx = tid; idx < w * w; idx += blockDim.x) {
+    int j = idx / w, i = idx % w;
+    if (i >= j) F[base + (long)(a + j) * nr + (a + i)] = L11[i * LDSP + j];
+  }
+  for (int k = tid; k < w; k += blockDim.x) Dg[first + a + k] = L11[k * LDSP + k];
+}
+
+// sub-panel, step 2: rows below the diagonal block, L21 = R L11^-T D^-1, one
+// 64-row chunk per workgroup (the factored block is read back from F)
+__global__ __launch_bounds__(256) void k_trsm(const PanelTask* __restrict__ tasks,
+                                              const long* __restrict__ fofs,
+                                              const int* __restrict__ nrv, double* F) {
+  __shared__ double L11[NB * LDSP];
+  __shared__ double X[PANEL_ROWS * LDSP];
+  __shared__ double dinv[NB];
+  const PanelTask T = tasks[blockIdx.x];
+  const int s = T.s, nr = nrv[s], a = T.a, w = T.w;
+  const long base = fofs[s];
+  const int tid = threadIdx.x;
+  for (int idx = tid; idx < w * w; idx += blockDim.x) {
+    int j = idx / w, i = idx % w;
+    if (i >= j) L11[i * LDSP + j] = F[base + (long)(a + j) * nr + (a + i)];
   }
   const int rows = T.r1 - T.r0;
-  for (int k = tid; k < w; k += blockDim.x) dinv[k] = 1.0 / L11[k * LDSP + k];
   for (int idx = tid; idx < rows * w; idx += blockDim.x) {
     int k = idx / rows, rr = idx % rows;
     X[rr * LDSP + k] = F[base + (long)(a + k) * nr + T.r0 + rr];
   }
   __syncthreads();
+  for (int k = tid; k < w; k += blockDim.x) dinv[k] = 1.0 / L11[k * LDSP + k];
   // X L11^T = R (unit lower L11): four lanes per row share the inner sums
   const int rr = tid >> 2, q = tid & 3;
   if (rr < rows) {
@@ -388,7 +410,7 @@ __global__ __launch_bounds__(256) void k_fwd(
   double acc = 0.0;
   const int ncbk = min(rb, ncb);
   for (int cb = wv; cb < ncbk; cb += 4) {
-    if (lane == 0) wait_flag(&flags[flag_ofs[s] + cb], gen, err);
+    if (lane == 0) wait_flag(&flags[(flag_ofs[s] + cb) * FSTRIDE], gen, err);
     __builtin_amdgcn_wave_barrier();
     const int cbase = cb * SB;
     const double yv = (cbase + lane < nc) ? load_acq(&xp[first + cbase + lane]) : 0.0;
@@ -416,7 +438,7 @@ __global__ __launch_bounds__(256) void k_fwd(
   if (tid == 0) {
     for (int t = child_ptr[s]; t < child_ptr[s + 1]; ++t) {
       int c = child[t];
-      wait_count(&done[c], nblk[c], err);
+      wait_count(&done[c * FSTRIDE], nblk[c], err);
     }
   }
   __syncthreads();
@@ -443,8 +465,8 @@ __global__ __launch_bounds__(256) void k_fwd(
   __syncthreads();
   if (tid == 0) {
     if (diag)
-      __hip_atomic_store(&flags[flag_ofs[s] + rb], gen, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_fetch_add(&done[s], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&flags[(flag_ofs[s] + rb) * FSTRIDE], gen, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_add(&done[s * FSTRIDE], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
@@ -470,7 +492,7 @@ __global__ __launch_bounds__(256) void k_bwd(
   const int c0 = cb * SB, kc = min(SB, nc - c0);
   const int ncb = (nc + SB - 1) / SB;
   for (int c = wv; c < kc; c += 4) Ld[lane * LDSP + c] = (lane < kc) ? F[base + (long)(c0 + c) * nr + c0 + lane] : 0.0;
-  if (tid == 0 && parent[s] >= 0) wait_count(&done[parent[s]], nblkb[parent[s]], err);
+  if (tid == 0 && parent[s] >= 0) wait_count(&done[parent[s] * FSTRIDE], nblkb[parent[s]], err);
   __syncthreads();
   double acc[SB];
 #pragma unroll
@@ -483,7 +505,7 @@ __global__ __launch_bounds__(256) void k_bwd(
       // pivot rows [R, min(R+SB, nc)) belong to blocks b > cb of this front
       const int b0 = R / SB, b1 = (min(R + SB, nc) - 1) / SB;
       for (int b = b1; b >= b0; --b)
-        if (b > cb && b < ncb) wait_flag(&flags[flag_ofs[s] + b], gen, err);
+        if (b > cb && b < ncb) wait_flag(&flags[(flag_ofs[s] + b) * FSTRIDE], gen, err);
     }
     __builtin_amdgcn_wave_barrier();
     const int rr = R + lane;
@@ -525,8 +547,8 @@ __global__ __launch_bounds__(256) void k_bwd(
   __threadfence();
   __syncthreads();
   if (tid == 0) {
-    __hip_atomic_store(&flags[flag_ofs[s] + cb], gen, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_fetch_add(&done[s], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&flags[(flag_ofs[s] + cb) * FSTRIDE], gen, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_add(&done[s * FSTRIDE], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
@@ -700,9 +722,9 @@ void DeviceKKT::setup(const LowerPattern& K, int nx, hipStream_t stream) {
   nblk_.upload(nblk, st_);
   nblkb_.upload(nblkb, st_);
   parent_.upload(P.sn_parent, st_);
-  flags_.alloc(std::max(1, flag_ofs[ns]));
+  flags_.alloc((size_t)std::max(1, flag_ofs[ns]) * FSTRIDE);
   flags_.zero(st_);
-  done_.alloc(2 * ns + 2);
+  done_.alloc((size_t)(2 * ns + 2) * FSTRIDE);
   serr_.alloc(1);
   serr_.zero(st_);
   xp_.alloc(N);
@@ -819,11 +841,19 @@ void DeviceKKT::build_schedule() {
         int tbeg = (int)trail_h_.size();
         double fl = 0;
         for (int s : bigs) {
-          int nc = P.sn_ncol[s], nr = P.nr(s);
+          int nc = P.sn_ncol[s];
           if (a0 >= nc) continue;
           int pend = std::min(P0 + PW, nc);
           int w = std::min(NB, pend - a0);
           panel_h_.push_back({s, a0, w, -1, -1});
+        }
+        if ((int)panel_h_.size() > pbeg) launches_.push_back({L_DIAG, pbeg, (int)panel_h_.size()});
+        const int rbeg = (int)panel_h_.size();
+        for (int s : bigs) {
+          int nc = P.sn_ncol[s], nr = P.nr(s);
+          if (a0 >= nc) continue;
+          int pend = std::min(P0 + PW, nc);
+          int w = std::min(NB, pend - a0);
           for (int r0 = a0 + w; r0 < nr; r0 += PANEL_ROWS) panel_h_.push_back({s, a0, w, r0, std::min(nr, r0 + PANEL_ROWS)});
           int b = a0 + w;
           if (b < pend)
@@ -831,7 +861,7 @@ void DeviceKKT::build_schedule() {
               for (int tj = b; tj < pend; tj += TM)
                 if (ti + TM - 1 >= tj) push_trail(s, ti, tj, a0, b, b, pend, nr, fl);
         }
-        if ((int)panel_h_.size() > pbeg) launches_.push_back({L_PANEL, pbeg, (int)panel_h_.size()});
+        if ((int)panel_h_.size() > rbeg) launches_.push_back({L_PANEL, rbeg, (int)panel_h_.size()});
         if ((int)trail_h_.size() > tbeg) {
           launches_.push_back({L_TRAIL, tbeg, (int)trail_h_.size()});
           trail_launch_flops_.push_back(fl);
@@ -880,9 +910,12 @@ void DeviceKKT::factorize_async(const double* Kx, double extra_p, double extra_a
         hipLaunchKernelGGL(k_factor_small, dim3(n), dim3(256), 0, st_, small_.p + L.beg, fofs_.p,
                            nr_.p, nc_.p, first_.p, sgn_.p, F_.p, D_.p, mode, eps, delta, fstat_.p);
         break;
-      case L_PANEL:
-        hipLaunchKernelGGL(k_panel, dim3(n), dim3(256), 0, st_, panel_.p + L.beg, fofs_.p, nr_.p,
+      case L_DIAG:
+        hipLaunchKernelGGL(k_diag, dim3(n), dim3(256), 0, st_, panel_.p + L.beg, fofs_.p, nr_.p,
                            first_.p, sgn_.p, F_.p, D_.p, mode, eps, delta, fstat_.p);
+        break;
+      case L_PANEL:
+        hipLaunchKernelGGL(k_trsm, dim3(n), dim3(256), 0, st_, panel_.p + L.beg, fofs_.p, nr_.p, F_.p);
         break;
       case L_TRAIL:
         if (time_trail_) CX_HIP(hipEventRecord(ev0_, st_));
@@ -922,17 +955,18 @@ FactorStats DeviceKKT::factor_stats() {
 void DeviceKKT::solve_async(const double* rhs, double* x) {
   const int N = P_.N, ns = P_.ns;
   if (N == 0) return;
-  hipLaunchKernelGGL(k_zero_int, dim3(ceil_div(2 * ns + 2, 256)), dim3(256), 0, st_, done_.p, 2 * ns + 2);
+  const int nctr = (2 * ns + 2) * FSTRIDE;
+  hipLaunchKernelGGL(k_zero_int, dim3(std::min(1024, ceil_div(nctr, 256))), dim3(256), 0, st_, done_.p, nctr);
   hipLaunchKernelGGL(k_perm_in, dim3(ceil_div(N, 256)), dim3(256), 0, st_, perm_.p, rhs, xp_.p, N);
-  int* tick = done_.p + 2 * ns;
+  int* tick = done_.p + (size_t)2 * ns * FSTRIDE;
   ++gen_;
   hipLaunchKernelGGL(k_fwd, dim3(nfwd_), dim3(256), 0, st_, fwd_.p, tick, fofs_.p, nr_.p, nc_.p,
                      first_.p, rowptr_.p, gptr_.p, gidx_.p, uofs_.p, child_ptr_.p, child_.p, nblk_.p,
                      flag_ofs_.p, flags_.p, done_.p, gen_, F_.p, xp_.p, U_.p, serr_.p);
   ++gen_;
-  hipLaunchKernelGGL(k_bwd, dim3(nbwd_), dim3(256), 0, st_, bwd_.p, tick + 1, fofs_.p, nr_.p, nc_.p,
-                     first_.p, rowptr_.p, rows_.p, parent_.p, nblkb_.p, flag_ofs_.p, flags_.p,
-                     done_.p + ns, gen_, F_.p, D_.p, xp_.p, serr_.p);
+  hipLaunchKernelGGL(k_bwd, dim3(nbwd_), dim3(256), 0, st_, bwd_.p, tick + FSTRIDE, fofs_.p, nr_.p,
+                     nc_.p, first_.p, rowptr_.p, rows_.p, parent_.p, nblkb_.p, flag_ofs_.p, flags_.p,
+                     done_.p + (size_t)ns * FSTRIDE, gen_, F_.p, D_.p, xp_.p, serr_.p);
   hipLaunchKernelGGL(k_perm_out, dim3(ceil_div(N, 256)), dim3(256), 0, st_, perm_.p, xp_.p, x, N);
   CX_LAUNCH_CHECK();
 }

@@ -22,11 +22,11 @@ struct FactorStats {
 
 struct AsmTask { int s, j0, j1, seg_beg, seg_end; };
 struct AsmSeg { int child, i0, i1; };
-struct PanelTask { int s, a, w, r0, r1; };  // r0 < 0: writer of the factored diagonal block
+struct PanelTask { int s, a, w, r0, r1; };  // r0 < 0: diagonal-block task
 struct TrailTask { int s, ti, tj, k0, k1, rlo, chi; };
 struct SolveTask { int s, blk; };
 
-enum LaunchKind { L_ASSEMBLE = 0, L_SMALL = 1, L_PANEL = 2, L_TRAIL = 3 };
+enum LaunchKind { L_ASSEMBLE = 0, L_SMALL = 1, L_DIAG = 2, L_PANEL = 3, L_TRAIL = 4 };
 struct Launch { int kind, beg, end; };
 
 // Pivot modes.

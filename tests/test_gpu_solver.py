@@ -204,3 +204,20 @@ def test_no_equilibration_parity(oracle):
     p = PR.config0(seed=5)
     s, sol = gpu_solve(p, Settings(equilibrate=False))
     assert_parity(sol, oracle_runs(oracle, p, equilibrate=0), p)
+
+
+def test_large_sparse_certificate_many_panel_tasks():
+    """config 1 at 1/4 scale (largest front ~3.7k, ~58 row tasks per sub-panel
+    launch): the oracle would need minutes, so the bar is the KKT optimality
+    certificate and a converged status within the reference's iteration range."""
+    p = PR.config1(scale=0.25)
+    s, sol = gpu_solve(p, Settings(warm_start=False))
+    st = s.kkt_stats()
+    print("kkt", st, "iters", sol.info.iterations, "time", sol.info.solve_time_s)
+    assert st["max_front"] > 3000
+    assert sol.status == Status.Solved
+    assert sol.info.iterations <= 40
+    c = PR.kkt_certificate(p, sol.x, sol.s, sol.z)
+    scale = 1.0 + np.abs(p.q).max() + np.abs(p.b).max()
+    assert c["stationarity"] < 1e-6 * scale and c["feasibility"] < 1e-6 * scale, c
+    assert c["margin_s"] > -1e-7 and c["margin_z"] > -1e-7, c
