@@ -1,0 +1,168 @@
+"""Benchmark: QPs solved per second on one MI355X, plain FP64 method.
+
+Workload (default): one large sparse conic QP, n = 20000 variables, m = 28500
+rows (2000 zero + 20000 nonnegative + 2000 SOC blocks of dim 3 + 10 SOC
+blocks of dim 50); P = diag(U[0.1,1]) + tridiagonal N(0,0.05) off-diagonals
+with the diagonal-dominance repair d_i <- max(d_i, |e_{i-1}|+|e_i|+0.01);
+A with 5 nonzeros per row; seeded feasible construction of q and b.  The
+problem is set up once (symbolic analysis, untimed); a step is one complete
+cold-started solve() to the default 1e-8 tolerances (warm start off, so no
+step reuses the previous optimum), on data already resident in HBM.
+
+Prints one JSON line.  Extra fields: "roofline" for the kernel class with the
+largest device time inside the timed region (HIP events on the solver's
+stream), and "cpu_baseline": the CPU oracle (single-threaded restatement of
+the reference method) timed on a bounded sample of the same workload family.
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+os.environ.setdefault("CONICXX_KERNEL_TIMING", "1")
+
+HBM_PEAK_GBS = 8000.0
+FP64_MFMA_PEAK_TFLOPS = 78.6
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def cpu_baseline(full_flops, scale=0.1):
+    """Oracle solve of the same generator at `scale`, extrapolated to the full
+    instance by the ratio of per-factorization flops (factorization-dominated)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_py
+    from conicxx_amd import analyze_pattern
+    from conicxx_amd import problems as PR
+
+    p = PR.config1(scale=scale)
+    s = oracle_py.OracleSolver()
+    assert s.setup(p.P, p.q, p.A, p.b, p.zero_dim, p.nonneg_dim, p.soc_dims, warm_start=0)
+    t0 = time.perf_counter()
+    r = s.solve()
+    t = time.perf_counter() - t0
+    _, _, st = analyze_pattern(PR.kkt_lower_pattern(p))
+    ratio = full_flops / st["flops"]
+    est = t * ratio
+    return {
+        "value": 1.0 / est, "unit": "QPs/s", "cores": 1, "kind": "port",
+        "sample": (f"oracle solve of the config-1 generator at scale {scale} (n={p.n}, m={p.m}, "
+                   f"status {r['status']}, {r['iterations']} iterations) took {t:.2f} s on one core; "
+                   f"extrapolated to the full instance by the factorization-flop ratio "
+                   f"{ratio:.0f} -> {est:.0f} s per solve"),
+        "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: the plain solve of one QP runs on a single device "
+              "(run without --gpus or with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np  # noqa: F401
+    from conicxx_amd import ConeSpec, Settings, Solver, Status
+    from conicxx_amd import problems as PR
+
+    p = PR.config1()
+    solver = Solver()
+    t0 = time.perf_counter()
+    ok = solver.setup(p.P, p.q, p.A, p.b, ConeSpec(p.zero_dim, p.nonneg_dim, p.soc_dims),
+                      Settings(warm_start=False))
+    setup_s = time.perf_counter() - t0
+    assert ok, "setup failed"
+    kkt = solver.kkt_stats()
+    print(f"[bench] setup {setup_s:.2f} s, N={kkt['N']:.0f}, nnz(L)={kkt['nnz_L']:.3g}, "
+          f"factor flops={kkt['flops']:.3g}, largest front={kkt['max_front']:.0f}", file=sys.stderr, flush=True)
+    for w in range(args.warmup):
+        sol = solver.solve()
+        print(f"[bench] warmup {w}: {sol.status.name} in {sol.info.iterations} iterations, "
+              f"{sol.info.solve_time_s:.3f} s", file=sys.stderr, flush=True)
+    agg = {}
+    statuses, iters = [], []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        sol = solver.solve()
+        statuses.append(sol.status)
+        iters.append(sol.info.iterations)
+        for name, v in solver.kernel_stats().items():
+            a = agg.setdefault(name, {"ms": 0.0, "launches": 0, "work": 0.0})
+            a["ms"] += v["ms"]
+            a["launches"] += v["launches"]
+            a["work"] += v["work"]
+        print(f"[bench] step {k}: {sol.status.name} in {sol.info.iterations} iterations, "
+              f"{sol.info.solve_time_s:.3f} s", file=sys.stderr, flush=True)
+    elapsed = time.perf_counter() - t0
+    valid = all(s == Status.Solved for s in statuses)
+    value = args.steps / elapsed
+
+    # roofline of the dominant instrumented kernel class
+    dom = max(agg, key=lambda k: agg[k]["ms"]) if agg else None
+    roof = None
+    if dom:
+        a = agg[dom]
+        avg_s = a["ms"] / max(1, a["launches"]) / 1e3
+        per_launch = a["work"] / max(1, a["launches"])
+        if dom == "trail":
+            ach = per_launch / avg_s / 1e12
+            roof = {"kernel": "k_trail (FP64 MFMA trailing update)", "bound": "mfma",
+                    "achieved": ach, "peak": FP64_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
+                    "frac": ach / FP64_MFMA_PEAK_TFLOPS, "traffic": None}
+        else:
+            ach = per_launch / avg_s / 1e9 if per_launch else 0.0
+            roof = {"kernel": f"k_{dom}", "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None}
+        roof["avg_launch_ms"] = a["ms"] / max(1, a["launches"])
+        roof["launches"] = a["launches"]
+        roof["share_of_device_time"] = a["ms"] / max(1e-9, sum(v["ms"] for v in agg.values()))
+    kernel_ms = {k: round(v["ms"] / args.steps, 2) for k, v in agg.items()}
+    trail = agg.get("trail")
+    trail_tflops = (trail["work"] / (trail["ms"] / 1e3) / 1e12) if trail and trail["ms"] > 0 else None
+
+    out = {
+        "metric": "QPs solved/sec at 1e-8 rel gap (plain FP64 GPU solve)",
+        "value": value if valid else 0.0,
+        "unit": "QPs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded generator, feasible by construction)",
+        "config": {"workload": "config1: one sparse conic QP, n=20000, m=28500 (zero 2000, nonneg 20000, "
+                               "2000 SOC dim 3, 10 SOC dim 50), cold-started solve per step",
+                   "psd_repair": p.meta.get("psd_repair"), "kkt_N": kkt["N"], "nnz_L": kkt["nnz_L"],
+                   "factor_flops": kkt["flops"], "largest_front": kkt["max_front"],
+                   "iterations": iters, "all_solved": valid},
+        "roofline": roof,
+        "kernel_ms_per_step": kernel_ms,
+        "trail_tflops": trail_tflops,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(kkt["flops"])
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

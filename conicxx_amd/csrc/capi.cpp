@@ -154,6 +154,11 @@ void cx_get_profile(const cx_solver* h, double* out8) {
   if (h && out8) h->impl->profile(out8);
 }
 
+int cx_get_kernel_stats(const cx_solver* h, double* out, int max_kinds) {
+  if (!h || !out) return 0;
+  return h->impl->kernel_stats(out, max_kinds);
+}
+
 int cx_analyze_pattern(int N, const int* colptr, const int* rowind, int* perm, int* colcount,
                        double* stats8) {
   try {

@@ -754,13 +754,11 @@ void DeviceKKT::setup(const LowerPattern& K, int nx, hipStream_t stream) {
   kfcol_.upload(fcol, st_);
   kfsrc_.upload(fsrc, st_);
   {
-    const char* e = std::getenv("CONICXX_TRAIL_TIMING");
-    time_trail_ = e && e[0] == '1';
+    const char* e = std::getenv("CONICXX_KERNEL_TIMING");
+    timing_ = e && e[0] == '1';
   }
-  if (!ev0_) {
-    CX_HIP(hipEventCreate(&ev0_));
-    CX_HIP(hipEventCreate(&ev1_));
-  }
+  // algorithmic bytes of one triangular sweep: every stored entry of L once
+  sweep_bytes_ = 8.0 * P_.nnzL + 3.0 * 8.0 * N;
   CX_HIP(hipStreamSynchronize(st_));
 }
 
@@ -893,6 +891,46 @@ void DeviceKKT::build_schedule() {
   trail_.upload(trail_h_.empty() ? std::vector<TrailTask>{{0, 0, 0, 0, 0, 0, 0}} : trail_h_, st_);
 }
 
+void DeviceKKT::tbegin() {
+  if (!timing_) return;
+  if (evused_ + 2 > evpool_.size()) {
+    size_t add = std::max<size_t>(256, evpool_.size());
+    for (size_t i = 0; i < add; ++i) {
+      hipEvent_t e;
+      CX_HIP(hipEventCreate(&e));
+      evpool_.push_back(e);
+    }
+  }
+  CX_HIP(hipEventRecord(evpool_[evused_], st_));
+}
+
+void DeviceKKT::tend(int kind, double work) {
+  if (!timing_) return;
+  CX_HIP(hipEventRecord(evpool_[evused_ + 1], st_));
+  evpending_.push_back({kind, work});
+  evused_ += 2;
+}
+
+void DeviceKKT::collect_timing() {
+  if (!timing_ || evpending_.empty()) return;
+  CX_HIP(hipStreamSynchronize(st_));
+  for (size_t i = 0; i < evpending_.size(); ++i) {
+    float ms = 0;
+    CX_HIP(hipEventElapsedTime(&ms, evpool_[2 * i], evpool_[2 * i + 1]));
+    KindStats& k = kstats_[evpending_[i].first];
+    k.ms += ms;
+    k.work += evpending_[i].second;
+    k.launches += 1;
+  }
+  evpending_.clear();
+  evused_ = 0;
+}
+
+void DeviceKKT::reset_timing() {
+  collect_timing();
+  for (auto& k : kstats_) k = KindStats{};
+}
+
 void DeviceKKT::factorize_async(const double* Kx, double extra_p, double extra_a, int mode,
                                 double eps, double delta) {
   CX_HIP(hipMemsetAsync(fstat_.p, 0, 2 * sizeof(int), st_));
@@ -902,34 +940,34 @@ void DeviceKKT::factorize_async(const double* Kx, double extra_p, double extra_a
     if (n <= 0) continue;
     switch (L.kind) {
       case L_ASSEMBLE:
+        tbegin();
         hipLaunchKernelGGL(k_assemble, dim3(n), dim3(256), 0, st_, asm_.p + L.beg, seg_.p, fofs_.p,
                            nr_.p, nc_.p, first_.p, kcolptr_.p, ksrc_.p, kdst_.p, kdiag_.p, Kx,
                            extra_p, extra_a, rmap_ofs_.p, rmap_.p, F_.p);
+        tend(T_ASM, 0.0);
         break;
       case L_SMALL:
+        tbegin();
         hipLaunchKernelGGL(k_factor_small, dim3(n), dim3(256), 0, st_, small_.p + L.beg, fofs_.p,
                            nr_.p, nc_.p, first_.p, sgn_.p, F_.p, D_.p, mode, eps, delta, fstat_.p);
+        tend(T_SMALL, 0.0);
         break;
       case L_DIAG:
+        tbegin();
         hipLaunchKernelGGL(k_diag, dim3(n), dim3(256), 0, st_, panel_.p + L.beg, fofs_.p, nr_.p,
                            first_.p, sgn_.p, F_.p, D_.p, mode, eps, delta, fstat_.p);
+        tend(T_DIAG, 0.0);
         break;
       case L_PANEL:
+        tbegin();
         hipLaunchKernelGGL(k_trsm, dim3(n), dim3(256), 0, st_, panel_.p + L.beg, fofs_.p, nr_.p, F_.p);
+        tend(T_TRSM, 0.0);
         break;
       case L_TRAIL:
-        if (time_trail_) CX_HIP(hipEventRecord(ev0_, st_));
+        tbegin();
         hipLaunchKernelGGL(k_trail, dim3(n), dim3(256), 0, st_, trail_.p + L.beg, fofs_.p, nr_.p,
                            first_.p, D_.p, F_.p);
-        if (time_trail_) {
-          CX_HIP(hipEventRecord(ev1_, st_));
-          CX_HIP(hipEventSynchronize(ev1_));
-          float ms = 0;
-          CX_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
-          trail_ms_ += ms;
-          trail_launches_ += 1;
-          trail_flops_ += trail_launch_flops_[tl];
-        }
+        tend(T_TRAIL, trail_launch_flops_[tl]);
         ++tl;
         break;
     }
@@ -946,6 +984,7 @@ FactorStats DeviceKKT::factor_stats() {
   fstat_.download(st, 2, st_);
   dstat_.download(&mn, 1, st_);
   CX_HIP(hipStreamSynchronize(st_));
+  collect_timing();
   fs.zero_pivot = st[0];
   fs.nreg = st[1];
   fs.min_abs_d = mn;
@@ -960,13 +999,17 @@ void DeviceKKT::solve_async(const double* rhs, double* x) {
   hipLaunchKernelGGL(k_perm_in, dim3(ceil_div(N, 256)), dim3(256), 0, st_, perm_.p, rhs, xp_.p, N);
   int* tick = done_.p + (size_t)2 * ns * FSTRIDE;
   ++gen_;
+  tbegin();
   hipLaunchKernelGGL(k_fwd, dim3(nfwd_), dim3(256), 0, st_, fwd_.p, tick, fofs_.p, nr_.p, nc_.p,
                      first_.p, rowptr_.p, gptr_.p, gidx_.p, uofs_.p, child_ptr_.p, child_.p, nblk_.p,
                      flag_ofs_.p, flags_.p, done_.p, gen_, F_.p, xp_.p, U_.p, serr_.p);
+  tend(T_FWD, sweep_bytes_);
   ++gen_;
+  tbegin();
   hipLaunchKernelGGL(k_bwd, dim3(nbwd_), dim3(256), 0, st_, bwd_.p, tick + FSTRIDE, fofs_.p, nr_.p,
                      nc_.p, first_.p, rowptr_.p, rows_.p, parent_.p, nblkb_.p, flag_ofs_.p, flags_.p,
                      done_.p + (size_t)ns * FSTRIDE, gen_, F_.p, D_.p, xp_.p, serr_.p);
+  tend(T_BWD, sweep_bytes_);
   hipLaunchKernelGGL(k_perm_out, dim3(ceil_div(N, 256)), dim3(256), 0, st_, perm_.p, xp_.p, x, N);
   CX_LAUNCH_CHECK();
 }
@@ -983,6 +1026,7 @@ int DeviceKKT::take_solve_error() {
   int e = 0;
   serr_.download(&e, 1, st_);
   CX_HIP(hipStreamSynchronize(st_));
+  collect_timing();
   if (e) serr_.zero(st_);
   return e;
 }

@@ -59,13 +59,19 @@ class DeviceKKT {
   int N() const { return P_.N; }
   hipStream_t stream() const { return st_; }
 
-  // timing of the dense trailing-update kernel (HIP events around each launch)
-  void enable_trail_timing(bool on) { time_trail_ = on; }
-  void reset_trail_timing() { trail_ms_ = 0; trail_launches_ = 0; trail_flops_ = 0; }
-  double trail_ms() const { return trail_ms_; }
-  int trail_launches() const { return trail_launches_; }
-  double trail_flops() const { return trail_flops_; }
-  double trail_flops_per_factor() const { return trail_flops_static_; }
+  // Per-kernel-class device timing with pooled HIP events recorded on this
+  // stream around each launch (enabled by CONICXX_KERNEL_TIMING=1); events are
+  // resolved only where the solver synchronizes anyway.
+  enum TimedKind { T_TRAIL = 0, T_DIAG, T_TRSM, T_ASM, T_SMALL, T_FWD, T_BWD, T_NKINDS };
+  struct KindStats {
+    double ms = 0, work = 0;  // work: algorithmic flops (factorization kinds) or bytes (sweeps)
+    long launches = 0;
+  };
+  bool timing() const { return timing_; }
+  void collect_timing();
+  void reset_timing();
+  const KindStats& kind_stats(int k) const { return kstats_[k]; }
+  double sweep_bytes() const { return sweep_bytes_; }
 
  private:
   void build_schedule();
@@ -82,6 +88,7 @@ class DeviceKKT {
   std::vector<TrailTask> trail_h_;
   double trail_flops_static_ = 0;
   std::vector<double> trail_launch_flops_;
+  std::vector<double> diag_launch_flops_, trsm_launch_flops_;
   // device plan
   DevBuf<long> fofs_;
   DevBuf<int> nr_, nc_, first_, rowptr_, rows_, child_ptr_, child_;
@@ -112,10 +119,15 @@ class DeviceKKT {
   // K full symmetric CSR for spmv
   DevBuf<int> kfptr_, kfcol_, kfsrc_;
   // timing
-  bool time_trail_ = false;
-  double trail_ms_ = 0, trail_flops_ = 0;
-  int trail_launches_ = 0;
-  hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+  void tbegin();
+  void tend(int kind, double work);
+  bool timing_ = false;
+  std::vector<hipEvent_t> evpool_;
+  size_t evused_ = 0;
+  std::vector<std::pair<int, double>> evpending_;
+  KindStats kstats_[T_NKINDS];
+  std::vector<double> trail_launch_flops_h_;
+  double sweep_bytes_ = 0;
 };
 
 }  // namespace cx

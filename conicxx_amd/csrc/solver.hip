@@ -716,7 +716,7 @@ int Solver::solve() {  // solver_impl.cpp:367-466
   objective_ = pres_ = dres_ = gap_ = mu_out_ = 0;
   prof_fact_ms_ = prof_solve_ms_ = 0;
   prof_nfact_ = prof_nsolve_ = 0;
-  kkt_.reset_trail_timing();
+  kkt_.reset_timing();
   if (!setup_done_) {
     status_ = CX_NUMERICAL_ERROR;
     return status_;
@@ -871,14 +871,26 @@ void Solver::kkt_stats(double* out8) const {
 }
 
 void Solver::profile(double* out8) const {
+  const DeviceKKT::KindStats& t = kkt_.kind_stats(DeviceKKT::T_TRAIL);
   out8[0] = prof_fact_ms_;
   out8[1] = prof_solve_ms_;
   out8[2] = prof_total_ms_ - prof_fact_ms_ - prof_solve_ms_;
   out8[3] = prof_nfact_;
   out8[4] = prof_nsolve_;
-  out8[5] = kkt_.trail_launches() ? kkt_.trail_ms() / kkt_.trail_launches() : 0.0;
-  out8[6] = kkt_.trail_launches();
-  out8[7] = kkt_.trail_flops();
+  out8[5] = t.launches ? t.ms / t.launches : 0.0;
+  out8[6] = (double)t.launches;
+  out8[7] = t.work;
+}
+
+int Solver::kernel_stats(double* out, int max_kinds) const {
+  int n = std::min(max_kinds, (int)DeviceKKT::T_NKINDS);
+  for (int k = 0; k < n; ++k) {
+    const DeviceKKT::KindStats& t = kkt_.kind_stats(k);
+    out[3 * k + 0] = t.ms;
+    out[3 * k + 1] = (double)t.launches;
+    out[3 * k + 2] = t.work;
+  }
+  return n;
 }
 
 }  // namespace cx

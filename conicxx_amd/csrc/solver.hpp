@@ -35,6 +35,7 @@ class Solver {
   void dims(int* n, int* m) const { *n = n_; *m = m_; }
   void kkt_stats(double* out8) const;
   void profile(double* out8) const;
+  int kernel_stats(double* out, int max_kinds) const;
 
  private:
   // KKT (kkt_system.cpp)

@@ -281,6 +281,15 @@ class Solver:
                 "trail_avg_ms", "trail_launches", "trail_flops"]
         return dict(zip(keys, out.tolist()))
 
+    KERNEL_CLASSES = ["trail", "diag", "trsm", "assemble", "small", "fwd", "bwd"]
+
+    def kernel_stats(self):
+        """Device time per kernel class of the last solve (CONICXX_KERNEL_TIMING=1)."""
+        out = np.zeros(3 * len(self.KERNEL_CLASSES))
+        n = _lib.lib().cx_get_kernel_stats(self._h, out.ctypes.data_as(_lib._f64p), len(self.KERNEL_CLASSES))
+        return {self.KERNEL_CLASSES[k]: dict(ms=out[3 * k], launches=int(out[3 * k + 1]), work=out[3 * k + 2])
+                for k in range(n)}
+
     # snake_case aliases
     update_data = updateData
     set_warm_start = setWarmStart

@@ -120,12 +120,22 @@ const char* cx_status_string(int status);
 int cx_get_dims(const cx_solver* h, int* n, int* m);
 void cx_get_kkt_stats(const cx_solver* h, double* stats8);
 
-/* Device-timed breakdown of the last solve (milliseconds): [0] numeric
- * factorizations, [1] triangular solves, [2] everything else, [3] number of
- * factorizations, [4] number of KKT solves (incl. refinement steps), [5] HIP-event
- * average duration of the dominant dense trailing-update kernel (ms),
- * [6] its launches, [7] sum of its algorithmic flops.  No counterpart in the reference. */
+/* Breakdown of the last solve: [0] numeric factorizations (ms, host-timed incl.
+ * the pivot check), [1] KKT solves incl. refinement (ms), [2] everything else (ms),
+ * [3] number of factorizations, [4] number of KKT solves (incl. refinement steps),
+ * [5] average device duration of the dense trailing-update kernel (ms), [6] its
+ * launches, [7] its algorithmic flops.  [5..7] need CONICXX_KERNEL_TIMING=1 in the
+ * environment at setup().  No counterpart in the reference. */
 void cx_get_profile(const cx_solver* h, double* out8);
+
+/* Device time per kernel class of the last solve, measured with HIP events on the
+ * solver's stream (CONICXX_KERNEL_TIMING=1 at setup()): for class k, out[3k..3k+2]
+ * = (milliseconds, launches, algorithmic work); work is flops for the trailing
+ * update (class 0) and bytes for the forward/backward sweeps (classes 5, 6).
+ * Classes: 0 trailing update, 1 diagonal block, 2 panel solve, 3 assembly,
+ * 4 small fronts, 5 forward sweep, 6 backward sweep.  Returns the number of
+ * classes written.  No counterpart in the reference. */
+int cx_get_kernel_stats(const cx_solver* h, double* out, int max_kinds);
 
 /* Host-only symbolic analysis of a symmetric pattern (lower CSC, diagonal
  * included), for testing: writes the final permutation (new->old, N ints) and
