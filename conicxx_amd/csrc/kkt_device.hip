@@ -378,12 +378,17 @@ __global__ void k_perm_out(const int* __restrict__ perm, const double* __restric
   if (k < N) x[perm[k]] = xp[k];
 }
 
+__device__ inline double bcast(double v, int l) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                          __builtin_amdgcn_readlane(__double2loint(v), l));
+}
+
 // forward: L y = b on the permuted vector; pivot rows of each front get y,
 // non-pivot rows produce the update vector U_s for the parent.  One task per
-// 64-row block; each wave owns the column blocks cb = wv (mod 4) and polls only
-// the flag of the block it needs next, so far-away blocks (solved long ago)
-// are consumed without waiting.  The diagonal block is staged in LDS before
-// any wait, and the in-block substitution reads it from LDS.
+// 64-row block (lane = row).  Each wave owns the column blocks cb = wv (mod 4):
+// it loads the 64x64 block of L into registers *before* polling the block's
+// flag, so once y_cb is published only a broadcast-and-FMA pass remains.  The
+// diagonal block is staged in LDS up front.
 __global__ __launch_bounds__(256) void k_fwd(
     const SolveTask* __restrict__ tasks, int* ticket, const long* __restrict__ fofs,
     const int* __restrict__ nrv, const int* __restrict__ ncv, const int* __restrict__ firstv,
@@ -403,38 +408,27 @@ __global__ __launch_bounds__(256) void k_fwd(
   const int r0 = rb * SB;
   const int ncb = (nc + SB - 1) / SB;
   const int r = r0 + lane;
+  const bool rv = r < nr;
+  const int rc = rv ? r : nr - 1;  // clamped, always in bounds
   const bool diag = rb < ncb;
   const int kc = diag ? min(SB, nc - r0) : 0;
-  // stage the diagonal block (independent of every wait)
-  for (int c = wv; c < kc; c += 4) Ld[lane * LDSP + c] = (r < nr) ? F[base + (long)(r0 + c) * nr + r] : 0.0;
+  for (int c = wv; c < kc; c += 4) Ld[lane * LDSP + c] = F[base + (long)(r0 + c) * nr + rc];
   double acc = 0.0;
   const int ncbk = min(rb, ncb);
   for (int cb = wv; cb < ncbk; cb += 4) {
+    const int cbase = cb * SB;
+    const int cend = min(SB, nc - cbase);
+    const double* Fc = F + base + (long)cbase * nr + rc;
+    double Lb[SB];
+#pragma unroll
+    for (int c = 0; c < SB; ++c) Lb[c] = (c < cend) ? Fc[(long)c * nr] : 0.0;
     if (lane == 0) wait_flag(&flags[(flag_ofs[s] + cb) * FSTRIDE], gen, err);
     __builtin_amdgcn_wave_barrier();
-    const int cbase = cb * SB;
-    const double yv = (cbase + lane < nc) ? load_acq(&xp[first + cbase + lane]) : 0.0;
-    const int cend = min(SB, nc - cbase);
-    // every lane runs the shuffles (sources must be active); rows past nr read
-    // a clamped, in-bounds address and contribute nothing
-    const bool rv = r < nr;
-    const double* Fc = F + base + (long)cbase * nr + (rv ? r : nr - 1);
-    if (cend == SB) {
-#pragma unroll 16
-      for (int c = 0; c < SB; ++c) {
-        const double yc = __shfl(yv, c, 64);
-        const double f = Fc[(long)c * nr];
-        acc += rv ? f * yc : 0.0;
-      }
-    } else {
-      for (int c = 0; c < cend; ++c) {
-        const double yc = __shfl(yv, c, 64);
-        const double f = Fc[(long)c * nr];
-        acc += rv ? f * yc : 0.0;
-      }
-    }
+    const double yv = (lane < cend) ? load_acq(&xp[first + cbase + lane]) : 0.0;
+#pragma unroll
+    for (int c = 0; c < SB; ++c) acc += Lb[c] * bcast(yv, c);
   }
-  part[wv][lane] = acc;
+  part[wv][lane] = rv ? acc : 0.0;
   if (tid == 0) {
     for (int t = child_ptr[s]; t < child_ptr[s + 1]; ++t) {
       int c = child[t];
@@ -444,17 +438,20 @@ __global__ __launch_bounds__(256) void k_fwd(
   __syncthreads();
   if (wv == 0) {
     double val = 0.0;
-    if (r < nr) {
+    if (rv) {
       val = (r < nc) ? xp[first + r] : 0.0;
       const int gr = rowptr[s] + r;
       for (int g = gptr[gr]; g < gptr[gr + 1]; ++g) val += load_acq(&U[gidx[g]]);
       val -= (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
     }
-    for (int k = 0; k < kc; ++k) {
-      const double yk = __shfl(val, k, 64);
-      if (lane > k) val -= Ld[lane * LDSP + k] * yk;
+#pragma unroll
+    for (int k = 0; k < SB; ++k) {
+      if (k < kc) {
+        const double yk = bcast(val, k);
+        if (lane > k) val -= Ld[lane * LDSP + k] * yk;
+      }
     }
-    if (r < nr) {
+    if (rv) {
       if (r < nc)
         xp[first + r] = val;
       else
@@ -470,10 +467,35 @@ __global__ __launch_bounds__(256) void k_fwd(
   }
 }
 
+// Reduce-scatter of 64 per-lane partials over the wave: afterwards lane c holds
+// sum over lanes of a[c].  Six halving steps, 63 shuffles in total.
+// Value-level select (a ternary on array elements would be rewritten into a
+// per-lane dynamic index, which forces the array into scratch memory).
+__device__ inline double bitsel(double x, double y, long long mask) {
+  const long long xi = __double_as_longlong(x), yi = __double_as_longlong(y);
+  return __longlong_as_double((xi & mask) | (yi & ~mask));
+}
+
+__device__ inline double wave_reduce_scatter64(double (&a)[SB], int lane) {
+#pragma unroll
+  for (int h = 32; h >= 1; h >>= 1) {
+    const long long up = -(long long)((lane & h) != 0);
+#pragma unroll
+    for (int j = 0; j < h; ++j) {
+      const double lo = a[j], hi = a[j + h];
+      const double keep = bitsel(hi, lo, up);
+      const double send = bitsel(lo, hi, up);
+      a[j] = keep + __shfl_xor(send, h, 64);
+    }
+  }
+  return a[0];
+}
+
 // backward: D^-1 then L^T x = y, one column block of pivots per task.  Rows
 // below the block are visited farthest first (non-pivot rows, final once the
-// parent front is done, then later pivot blocks in descending order), with
-// lane = row: every lane keeps 64 column partials, reduced across lanes once.
+// parent front is done, then later pivot blocks in descending order) with
+// lane = row: every lane keeps 64 column partials (reduce-scattered once at
+// the end); the L block is loaded into registers before the flag is polled.
 __global__ __launch_bounds__(256) void k_bwd(
     const SolveTask* __restrict__ tasks, int* ticket, const long* __restrict__ fofs,
     const int* __restrict__ nrv, const int* __restrict__ ncv, const int* __restrict__ firstv,
@@ -501,6 +523,13 @@ __global__ __launch_bounds__(256) void k_bwd(
   const int nchunks = (nr - rstart + SB - 1) / SB;
   for (int q = nchunks - 1 - wv; q >= 0; q -= 4) {
     const int R = rstart + q * SB;
+    const int rr = R + lane;
+    const bool rv = rr < nr;
+    const double* Lr = F + base + (long)c0 * nr + (rv ? rr : nr - 1);
+    constexpr int H = SB / 2;
+    double Lb[H];  // first half of the block, loaded before the flag is polled
+#pragma unroll
+    for (int c = 0; c < H; ++c) Lb[c] = (c < kc) ? Lr[(long)c * nr] : 0.0;
     if (lane == 0 && R < nc) {
       // pivot rows [R, min(R+SB, nc)) belong to blocks b > cb of this front
       const int b0 = R / SB, b1 = (min(R + SB, nc) - 1) / SB;
@@ -508,29 +537,19 @@ __global__ __launch_bounds__(256) void k_bwd(
         if (b > cb && b < ncb) wait_flag(&flags[(flag_ofs[s] + b) * FSTRIDE], gen, err);
     }
     __builtin_amdgcn_wave_barrier();
-    const int rr = R + lane;
     double xv = 0.0;
-    if (rr < nr) {
+    if (rv) {
       const int g = (rr < nc) ? first + rr : rows[rowptr[s] + rr];
       xv = load_acq(&xp[g]);
     }
-    if (rr < nr) {
-      const double* Lr = F + base + (long)c0 * nr + rr;
 #pragma unroll
-      for (int c = 0; c < SB; ++c)
-        if (c < kc) acc[c] += Lr[(long)c * nr] * xv;
-    }
+    for (int c = 0; c < H; ++c) acc[c] += Lb[c] * xv;
+#pragma unroll
+    for (int c = 0; c < H; ++c) Lb[c] = (H + c < kc) ? Lr[(long)(H + c) * nr] : 0.0;
+#pragma unroll
+    for (int c = 0; c < H; ++c) acc[H + c] += Lb[c] * xv;
   }
-  // reduce each column partial across the wave: lane c ends with column c
-  double mine = 0.0;
-#pragma unroll
-  for (int c = 0; c < SB; ++c) {
-    double v = acc[c];
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    if (lane == c) mine = v;
-  }
-  part[wv][lane] = mine;
+  part[wv][lane] = wave_reduce_scatter64(acc, lane);
   __syncthreads();
   if (wv == 0) {
     double y = 0.0;
@@ -538,9 +557,12 @@ __global__ __launch_bounds__(256) void k_bwd(
       const double dinv = 1.0 / Dg[first + c0 + lane];
       y = xp[first + c0 + lane] * dinv - ((part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]));
     }
-    for (int k = kc - 1; k >= 0; --k) {
-      const double xk = __shfl(y, k, 64);
-      if (lane < k) y -= Ld[k * LDSP + lane] * xk;
+#pragma unroll
+    for (int k = SB - 1; k >= 0; --k) {
+      if (k < kc) {
+        const double xk = bcast(y, k);
+        if (lane < k) y -= Ld[k * LDSP + lane] * xk;
+      }
     }
     if (lane < kc) xp[first + c0 + lane] = y;
   }
