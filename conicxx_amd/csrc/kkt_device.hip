@@ -59,13 +59,26 @@ __device__ inline double load_acq(const double* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Bounded waits: poll with relaxed loads (an acquire per poll would
-// invalidate the CU's L1 for every co-resident wave), back off between polls,
-// and issue one acquire fence once the flag is seen.  Give up (flagging the
-// sweep as failed) after SPIN_LIMIT polls, or at once when another workgroup
-// has already flagged a failure.  Flags and counters live one per 128-byte
-// line (FSTRIDE ints apart) so pollers of different blocks do not share lines.
+// Cross-workgroup protocol of the solve sweeps.  Values that other
+// workgroups of the same launch read (xp, U) are written with agent-coherent
+// atomic stores; the writer then waits for its stores to complete
+// (s_waitcnt) before it publishes a flag or bumps a counter.  Readers poll the
+// flag with relaxed loads and read the values with agent-coherent loads.  No
+// agent-scope acquire/release fence is issued (on this GPU each one writes back
+// or invalidates the XCD's L2).  Waits are bounded: after SPIN_LIMIT polls, or
+// at once when another workgroup has flagged a failure, the wait gives up and
+// flags the sweep as failed.  Flags and counters live one per 128-byte line
+// (FSTRIDE ints apart).
 constexpr int FSTRIDE = 32;
+
+__device__ inline void store_shared(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ inline void stores_complete() {
+  __builtin_amdgcn_s_waitcnt(0);
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
 
 __device__ inline bool wait_flag(int* f, int want, int* err) {
   long spins = 0;
@@ -77,7 +90,7 @@ __device__ inline bool wait_flag(int* f, int want, int* err) {
     }
     __builtin_amdgcn_s_sleep(2);
   }
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
   return true;
 }
 
@@ -91,7 +104,7 @@ __device__ inline bool wait_count(int* f, int want, int* err) {
     }
     __builtin_amdgcn_s_sleep(2);
   }
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
   return true;
 }
 
@@ -415,6 +428,9 @@ __global__ __launch_bounds__(256) void k_fwd(
   for (int c = wv; c < kc; c += 4) Ld[lane * LDSP + c] = F[base + (long)(r0 + c) * nr + rc];
   double acc = 0.0;
   const int ncbk = min(rb, ncb);
+  // a published block implies every earlier block of the front is published
+  // (its task read them all), so a wave only polls past what it already knows
+  int ready = -1;
   for (int cb = wv; cb < ncbk; cb += 4) {
     const int cbase = cb * SB;
     const int cend = min(SB, nc - cbase);
@@ -422,8 +438,15 @@ __global__ __launch_bounds__(256) void k_fwd(
     double Lb[SB];
 #pragma unroll
     for (int c = 0; c < SB; ++c) Lb[c] = (c < cend) ? Fc[(long)c * nr] : 0.0;
-    if (lane == 0) wait_flag(&flags[(flag_ofs[s] + cb) * FSTRIDE], gen, err);
-    __builtin_amdgcn_wave_barrier();
+    if (cb > ready) {
+      if (lane == 0) {
+        int* fl = &flags[(flag_ofs[s] + ncbk - 1) * FSTRIDE];  // newest block this task needs
+        if (__hip_atomic_load(fl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != gen)
+          wait_flag(&flags[(flag_ofs[s] + cb) * FSTRIDE], gen, err);
+      }
+      __builtin_amdgcn_wave_barrier();
+      ready = cb;
+    }
     const double yv = (lane < cend) ? load_acq(&xp[first + cbase + lane]) : 0.0;
 #pragma unroll
     for (int c = 0; c < SB; ++c) acc += Lb[c] * bcast(yv, c);
@@ -453,17 +476,15 @@ __global__ __launch_bounds__(256) void k_fwd(
     }
     if (rv) {
       if (r < nc)
-        xp[first + r] = val;
+        store_shared(&xp[first + r], val);
       else
-        U[uofs[s] + (r - nc)] = val;
+        store_shared(&U[uofs[s] + (r - nc)], val);
     }
-  }
-  __threadfence();
-  __syncthreads();
-  if (tid == 0) {
-    if (diag)
-      __hip_atomic_store(&flags[(flag_ofs[s] + rb) * FSTRIDE], gen, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_fetch_add(&done[s * FSTRIDE], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    stores_complete();
+    if (lane == 0) {
+      if (diag) __hip_atomic_store(&flags[(flag_ofs[s] + rb) * FSTRIDE], gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_add(&done[s * FSTRIDE], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
   }
 }
 
@@ -564,13 +585,12 @@ __global__ __launch_bounds__(256) void k_bwd(
         if (lane < k) y -= Ld[k * LDSP + lane] * xk;
       }
     }
-    if (lane < kc) xp[first + c0 + lane] = y;
-  }
-  __threadfence();
-  __syncthreads();
-  if (tid == 0) {
-    __hip_atomic_store(&flags[(flag_ofs[s] + cb) * FSTRIDE], gen, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_fetch_add(&done[s * FSTRIDE], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    if (lane < kc) store_shared(&xp[first + c0 + lane], y);
+    stores_complete();
+    if (lane == 0) {
+      __hip_atomic_store(&flags[(flag_ofs[s] + cb) * FSTRIDE], gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_add(&done[s * FSTRIDE], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
   }
 }
 
