@@ -34,7 +34,7 @@ constexpr int SMALL_NR = 64;  // fronts up to this size are factored in LDS
 constexpr int NB = 64;        // sub-panel width
 constexpr int PW = 256;       // outer panel width
 constexpr int TM = 128;       // trailing-update tile
-constexpr int KC = 16;        // K chunk staged in LDS per step
+constexpr int KC = 32;        // K chunk staged in LDS per step
 constexpr int SB = 64;        // solve block
 constexpr int LDSP = NB + 1;  // padded LDS row stride for 64x64 blocks
 constexpr long SPIN_LIMIT = 1L << 21;  // ~0.2 s of polling
@@ -287,6 +287,7 @@ __global__ __launch_bounds__(256) void k_trail(const TrailTask* __restrict__ tas
                                                const int* __restrict__ firstv,
                                                const double* __restrict__ Dg, double* F) {
   constexpr int LDA = TM + 4;
+  constexpr int PER = KC * TM / 256;  // staged elements per thread and operand
   __shared__ double As[KC * LDA];
   __shared__ double Bs[KC * LDA];
   const TrailTask T = tasks[blockIdx.x];
@@ -299,25 +300,35 @@ __global__ __launch_bounds__(256) void k_trail(const TrailTask* __restrict__ tas
   for (int mi = 0; mi < 4; ++mi)
 #pragma unroll
     for (int ni = 0; ni < 4; ++ni) acc[mi][ni] = (v4d){0.0, 0.0, 0.0, 0.0};
-  const int lrow = tid & (TM - 1);  // 0..127
-  const int lk = tid >> 7;          // 0..1
-  for (int k0 = T.k0; k0 < T.k1; k0 += KC) {
-    const int kw = min(KC, T.k1 - k0);
-    // stage A (rows ti..) and B (rows tj.., scaled by D) for this K chunk
+  const int lrow = tid & (TM - 1);  // staged row (0..127)
+  const int lk = tid >> 7;          // staged column parity (0..1)
+  const int gi = T.ti + lrow, gj = T.tj + lrow;
+  const bool iok = gi < nr, jok = gj < nr;
+  const double* Fa = F + base + (iok ? gi : 0);
+  const double* Fb = F + base + (jok ? gj : 0);
+  // register prefetch of the first chunk
+  double ra[PER], rb[PER];
+  auto load_chunk = [&](int k0) {
 #pragma unroll
-    for (int q = 0; q < KC / 2; ++q) {
+    for (int q = 0; q < PER; ++q) {
       const int kk = lk + 2 * q;
-      const int gi = T.ti + lrow, gj = T.tj + lrow;
-      double av = 0.0, bv = 0.0;
-      if (kk < kw) {
-        const long colb = base + (long)(k0 + kk) * nr;
-        if (gi < nr) av = F[colb + gi];
-        if (gj < nr) bv = F[colb + gj] * Dg[first + k0 + kk];
-      }
-      As[kk * LDA + lrow] = av;
-      Bs[kk * LDA + lrow] = bv;
+      const int k = k0 + kk;
+      const bool kin = k < T.k1;
+      const long col = (long)(kin ? k : T.k0) * nr;
+      ra[q] = (kin && iok) ? Fa[col] : 0.0;
+      rb[q] = (kin && jok) ? Fb[col] * Dg[first + (kin ? k : T.k0)] : 0.0;
+    }
+  };
+  load_chunk(T.k0);
+  for (int k0 = T.k0; k0 < T.k1; k0 += KC) {
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int kk = lk + 2 * q;
+      As[kk * LDA + lrow] = ra[q];
+      Bs[kk * LDA + lrow] = rb[q];
     }
     __syncthreads();
+    if (k0 + KC < T.k1) load_chunk(k0 + KC);  // in flight while the MFMAs run
 #pragma unroll
     for (int kk = 0; kk < KC; kk += 4) {
       double af[4], bf[4];
