@@ -308,15 +308,25 @@ __global__ __launch_bounds__(256) void k_trail(const TrailTask* __restrict__ tas
   const double* Fb = F + base + (jok ? gj : 0);
   // register prefetch of the first chunk
   double ra[PER], rb[PER];
+  // Branch-free staging: every load goes to a clamped, valid address and the
+  // out-of-range values are zeroed by a select afterwards, so all PER loads of
+  // a chunk are in flight together (a guarded load would make the compiler
+  // wait for each one at the join of its branch).
   auto load_chunk = [&](int k0) {
+    double va[PER], vb[PER], vd[PER];
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
-      const int kk = lk + 2 * q;
-      const int k = k0 + kk;
-      const bool kin = k < T.k1;
-      const long col = (long)(kin ? k : T.k0) * nr;
-      ra[q] = (kin && iok) ? Fa[col] : 0.0;
-      rb[q] = (kin && jok) ? Fb[col] * Dg[first + (kin ? k : T.k0)] : 0.0;
+      const int kq = min(k0 + lk + 2 * q, T.k1 - 1);
+      const long col = (long)kq * nr;
+      va[q] = Fa[col];
+      vb[q] = Fb[col];
+      vd[q] = Dg[first + kq];
+    }
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const bool kin = k0 + lk + 2 * q < T.k1;
+      ra[q] = (kin && iok) ? va[q] : 0.0;
+      rb[q] = (kin && jok) ? vb[q] * vd[q] : 0.0;
     }
   };
   load_chunk(T.k0);
@@ -345,19 +355,31 @@ __global__ __launch_bounds__(256) void k_trail(const TrailTask* __restrict__ tas
     }
     __syncthreads();
   }
-  // write back: v_mfma_f64_16x16x4f64 leaves D[lane/16 + 4r][lane%16] in register r
+  // write back: v_mfma_f64_16x16x4f64 leaves D[lane/16 + 4r][lane%16] in
+  // register r.  Per row block mi, the 16 old values are loaded together from
+  // clamped addresses, then the valid ones are stored.
 #pragma unroll
-  for (int mi = 0; mi < 4; ++mi)
+  for (int mi = 0; mi < 4; ++mi) {
+    double old[4][4];
+    long adr[4][4];
+    bool ok[4][4];
 #pragma unroll
     for (int ni = 0; ni < 4; ++ni) {
       const int j = T.tj + wc + ni * 16 + (lane & 15);
-      if (j >= T.chi) continue;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int i = T.ti + wr + mi * 16 + (lane >> 4) + 4 * r;
-        if (i < nr && i >= j && i >= T.rlo) F[base + (long)j * nr + i] -= acc[mi][ni][r];
+        ok[ni][r] = j < T.chi && i < nr && i >= j && i >= T.rlo;
+        adr[ni][r] = ok[ni][r] ? base + (long)j * nr + i : base;
+        old[ni][r] = F[adr[ni][r]];
       }
     }
+#pragma unroll
+    for (int ni = 0; ni < 4; ++ni)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        if (ok[ni][r]) F[adr[ni][r]] = old[ni][r] - acc[mi][ni][r];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -437,7 +459,20 @@ __global__ __launch_bounds__(256) void k_fwd(
   const bool diag = rb < ncb;
   const int kc = diag ? min(SB, nc - r0) : 0;
   for (int c = wv; c < kc; c += 4) Ld[lane * LDSP + c] = F[base + (long)(r0 + c) * nr + rc];
+  // Children's update vectors first: they do not depend on this front's own
+  // blocks, so gathering them here keeps them off the front's dependency
+  // chain.  Every thread waits for a share of the children, every wave sums a
+  // share of each row's gather list.
+  for (int t = child_ptr[s] + tid; t < child_ptr[s + 1]; t += 256) {
+    const int c = child[t];
+    wait_count(&done[c * FSTRIDE], nblk[c], err);
+  }
+  __syncthreads();
   double acc = 0.0;
+  if (rv) {
+    const int gr = rowptr[s] + r;
+    for (int g = gptr[gr] + wv; g < gptr[gr + 1]; g += 4) acc -= load_acq(&U[gidx[g]]);
+  }
   const int ncbk = min(rb, ncb);
   // a published block implies every earlier block of the front is published
   // (its task read them all), so a wave only polls past what it already knows
@@ -463,19 +498,11 @@ __global__ __launch_bounds__(256) void k_fwd(
     for (int c = 0; c < SB; ++c) acc += Lb[c] * bcast(yv, c);
   }
   part[wv][lane] = rv ? acc : 0.0;
-  if (tid == 0) {
-    for (int t = child_ptr[s]; t < child_ptr[s + 1]; ++t) {
-      int c = child[t];
-      wait_count(&done[c * FSTRIDE], nblk[c], err);
-    }
-  }
   __syncthreads();
   if (wv == 0) {
     double val = 0.0;
     if (rv) {
       val = (r < nc) ? xp[first + r] : 0.0;
-      const int gr = rowptr[s] + r;
-      for (int g = gptr[gr]; g < gptr[gr + 1]; ++g) val += load_acq(&U[gidx[g]]);
       val -= (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
     }
 #pragma unroll

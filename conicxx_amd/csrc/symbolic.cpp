@@ -282,12 +282,22 @@ std::vector<int> etree_of(const LowerPattern& K, const std::vector<int>& perm) {
   return parent;
 }
 
-std::vector<int> postorder(const std::vector<int>& parent) {
+std::vector<int> postorder(const std::vector<int>& parent, const std::vector<int>& key) {
   const int N = (int)parent.size();
   std::vector<int> head(N, -1), next(N, -1), post;
   post.reserve(N);
-  for (int j = N - 1; j >= 0; --j)
-    if (parent[j] != -1) next[j] = head[parent[j]], head[parent[j]] = j;
+  if (key.empty()) {
+    for (int j = N - 1; j >= 0; --j)
+      if (parent[j] != -1) next[j] = head[parent[j]], head[parent[j]] = j;
+  } else {
+    // children of every node linked in ascending (key, index) order: the child
+    // with the largest key is visited last and ends up adjacent to its parent
+    std::vector<int> order(N);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key[a] > key[b]; });
+    for (int j : order)
+      if (parent[j] != -1) next[j] = head[parent[j]], head[parent[j]] = j;
+  }
   std::vector<int> stack;
   for (int r = 0; r < N; ++r) {
     if (parent[r] != -1) continue;
@@ -360,7 +370,12 @@ SupernodalPlan plan_supernodes(const LowerPattern& K, const RelaxParams& rp) {
   if (N == 0) return P;
   std::vector<int> perm0 = amd_order(K);
   std::vector<int> par0 = etree_of(K, perm0);
-  std::vector<int> post = postorder(par0);
+  // Column counts do not depend on which postorder is used, so count once in a
+  // plain postorder, then visit children by ascending count: the child whose
+  // front is closest to its parent's sits next to it, where relaxed
+  // amalgamation can absorb it (dense separators then form one wide front).
+  std::vector<int> cc0 = column_counts(K, perm0, par0, postorder(par0));
+  std::vector<int> post = postorder(par0, cc0);
   // relabel so that the elimination tree is postordered
   std::vector<int> perm(N);
   for (int k = 0; k < N; ++k) perm[k] = perm0[post[k]];
@@ -368,9 +383,8 @@ SupernodalPlan plan_supernodes(const LowerPattern& K, const RelaxParams& rp) {
   for (int k = 0; k < N; ++k) ipost[post[k]] = k;
   std::vector<int> parent(N);
   for (int k = 0; k < N; ++k) parent[k] = par0[post[k]] == -1 ? -1 : ipost[par0[post[k]]];
-  std::vector<int> ident(N);
-  std::iota(ident.begin(), ident.end(), 0);
-  std::vector<int> cc = column_counts(K, perm, parent, ident);
+  std::vector<int> cc(N);
+  for (int k = 0; k < N; ++k) cc[k] = cc0[post[k]];
   P.nnzL_exact = 0;
   for (int j = 0; j < N; ++j) P.nnzL_exact += cc[j];
 

@@ -26,7 +26,9 @@ std::vector<int> amd_order(const LowerPattern& K);
 // parent[] in permuted labels, -1 for roots.
 std::vector<int> etree_of(const LowerPattern& K, const std::vector<int>& perm);
 
-std::vector<int> postorder(const std::vector<int>& parent);
+// Postorder of a forest; with `key`, the children of every node are visited in
+// ascending key order.
+std::vector<int> postorder(const std::vector<int>& parent, const std::vector<int>& key = {});
 
 // Column counts of L (diagonal included) for the permuted matrix
 // (Gilbert-Ng-Peyton; parent/post in permuted labels).
@@ -58,7 +60,11 @@ struct SupernodalPlan {
 struct RelaxParams {
   int nrelax0 = 4, nrelax1 = 16, nrelax2 = 48;
   double zrelax0 = 0.8, zrelax1 = 0.1, zrelax2 = 0.05;
-  int max_merge_cols = 256;  // do not create supernodes wider than this by amalgamation
+  // Width cap for amalgamation.  Wide supernodes are factored in 256-column
+  // panels inside one front, so a dense separator is best kept as a single
+  // supernode: splitting it into a chain makes every link re-assemble an
+  // almost equally large contribution block into the next.
+  int max_merge_cols = 1 << 30;
 };
 
 SupernodalPlan plan_supernodes(const LowerPattern& K, const RelaxParams& rp = RelaxParams());
