@@ -53,7 +53,24 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_facade_test()
     return LIB
+
+
+FACADE_SRC = os.path.join(ROOT, "tests", "cpp", "facade_qp.cpp")
+FACADE_BIN = os.path.join(ROOT, "tests", "cpp", "facade_qp")
+
+
+def build_facade_test():
+    """C++ program that drives include/conicxx_amd.hpp over the built library."""
+    if not os.path.exists(FACADE_SRC):
+        return None
+    cmd = ["g++", "-std=c++17", "-O1", "-Wall", "-I", os.path.join(ROOT, "include"), FACADE_SRC,
+           "-L", HERE, "-lconicxx_hip", "-Wl,-rpath,$ORIGIN/../../conicxx_amd", "-o", FACADE_BIN]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"facade test build failed:\n{r.stdout}\n{r.stderr}")
+    return FACADE_BIN
 
 
 if __name__ == "__main__":
