@@ -73,6 +73,7 @@ SIGNATURES = [
     ("cx_get_kkt_stats", None, [ctypes.c_void_p, _f64p]),
     ("cx_get_profile", None, [ctypes.c_void_p, _f64p]),
     ("cx_get_kernel_stats", ctypes.c_int, [ctypes.c_void_p, _f64p, ctypes.c_int]),
+    ("cx_backward", ctypes.c_int, [ctypes.c_void_p, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p]),
     ("cx_analyze_pattern", ctypes.c_int, [ctypes.c_int, _i32p, _i32p, _i32p, _i32p, _f64p]),
 ]
 

@@ -159,6 +159,17 @@ int cx_get_kernel_stats(const cx_solver* h, double* out, int max_kinds) {
   return h->impl->kernel_stats(out, max_kinds);
 }
 
+int cx_backward(cx_solver* h, const double* dl_dx, const double* dl_ds, const double* dl_dz,
+                double* dP, double* dq, double* dA, double* db) {
+  if (!h) return 0;
+  try {
+    return h->impl->backward(dl_dx, dl_ds, dl_dz, dP, dq, dA, db);
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "[conicxx_amd] backward failed: %s\n", e.what());
+    return 0;
+  }
+}
+
 int cx_analyze_pattern(int N, const int* colptr, const int* rowind, int* perm, int* colcount,
                        double* stats8) {
   try {

@@ -549,7 +549,8 @@ __global__ void k_scatter(int n, const int* slot, const int* src, const double* 
 }
 
 __global__ void k_hs_scatter(ConeDev C, int nhs, const int* slot, const int* kind, const int* blk,
-                             const int* ia, const int* ib, const int* row, double sreg_a, double* Kx) {
+                             const int* ia, const int* ib, const int* row, double sreg_a,
+                             const double* tcorr, const double* tdir, double* Kx) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nhs) return;
   const int a = ia[t], b = ib[t];
@@ -567,6 +568,10 @@ __global__ void k_hs_scatter(ConeDev C, int nhs, const int* slot, const int* kin
       h = be * be *
           (4.0 * C.vv[k] * va * vb - 2.0 * va * (jsign(b) * vb) - 2.0 * (jsign(a) * va) * vb +
            (a == b ? 1.0 : 0.0));
+      if (tcorr && a > 0 && b > 0) {
+        const double* u = tdir + C.soc_off[k];
+        h += tcorr[k] * ((a == b ? 1.0 : 0.0) - u[a] * u[b]);
+      }
     }
   }
   double val = -h;
@@ -593,6 +598,74 @@ __global__ void k_vmuldiv(int n, const double* in, const double* mul, const doub
     if (div) v = v / div[i];
     out[i] = v * c;
   }
+}
+
+// Adjoint gradient on a CSC pattern, one thread per stored entry (column found
+// by binary search).  u, v: adjoint solution (x part, cone part); xh, zh: the
+// optimum; all in the equilibrated space, xh = xs * inv_tau, zh = zs * inv_tau.
+// mode 0, A (m x n):      g = -(zh[i] u[j] + v[i] xh[j]) * drow[i] * dcol[j]
+// mode 1, P upper (n x n): i < j: -(u[i] xh[j] + u[j] xh[i]) * c * drow[i] * dcol[j]
+//                          i == j: -u[i] xh[i] * c * drow[i]^2;  i > j (ignored entry): 0
+__global__ void k_grad_csc(int cols, int nnz, const int* __restrict__ ptr, const int* __restrict__ row,
+                           int mode, const double* __restrict__ u, const double* __restrict__ v,
+                           const double* __restrict__ xs, const double* __restrict__ zs, double inv_tau,
+                           const double* __restrict__ drow, const double* __restrict__ dcol, double c,
+                           double* __restrict__ g) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nnz) return;
+  int lo = 0, hi = cols;  // largest j with ptr[j] <= k
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (ptr[mid] <= k) lo = mid; else hi = mid;
+  }
+  const int j = lo, i = row[k];
+  double val;
+  if (mode == 0) {
+    val = -(zs[i] * inv_tau * u[j] + v[i] * xs[j] * inv_tau) * drow[i] * dcol[j];
+  } else if (i < j) {
+    val = -(u[i] * xs[j] + u[j] * xs[i]) * inv_tau * c * drow[i] * dcol[j];
+  } else if (i == j) {
+    val = -u[i] * xs[i] * inv_tau * c * drow[i] * dcol[j];
+  } else {
+    val = 0.0;
+  }
+  g[k] = val;
+}
+
+// Tangent correction of the SOC scaling at a (near-)complementary pair, for
+// the adjoint system.  At s o z = 0 with both s and z on the cone boundary the
+// exact linearization arw(z) ds + arw(s) dz = 0 gives ds_T = -(s0/z0) dz_T on
+// the directions T = {(0, t): t orthogonal to z1}; the NT scaling W^2 has the
+// eigenvalue beta^2 = sqrt((s0^2-|s1|^2)/(z0^2-|z1|^2)) there, which equals
+// s0/z0 only on the central path.  Per block: tcorr = s0/z0 - beta^2 and
+// tdir = (0, z1/|z1|); tcorr = 0 where |z1| = 0 or the scaling fell back to I.
+__global__ void k_soc_tangent(ConeDev C, const double* __restrict__ s, const double* __restrict__ z,
+                              double* tcorr, double* tdir) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= C.nsoc) return;
+  const int o = C.soc_off[k], d = C.soc_dim[k];
+  double zn = 0.0;
+  for (int j = 1; j < d; ++j) zn += z[o + j] * z[o + j];
+  zn = sqrt(zn);
+  const bool ok = zn > 0.0 && z[o] > 0.0 && !C.ident[k];
+  tdir[o] = 0.0;
+  for (int j = 1; j < d; ++j) tdir[o + j] = ok ? z[o + j] / zn : 0.0;
+  tcorr[k] = ok ? s[o] / z[o] - C.beta[k] * C.beta[k] : 0.0;
+}
+
+// out -= tcorr_k * P_T x on every SOC block (the correction term of k_soc_tangent
+// applied to a vector; P_T = projection onto (0, t), t orthogonal to tdir).
+__global__ void k_soc_tangent_apply(ConeDev C, const double* __restrict__ tcorr,
+                                    const double* __restrict__ tdir, const double* __restrict__ x,
+                                    double* out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= C.nsoc) return;
+  const int o = C.soc_off[k], d = C.soc_dim[k];
+  const double c = tcorr[k];
+  if (c == 0.0) return;
+  double ux = 0.0;
+  for (int j = 1; j < d; ++j) ux += tdir[o + j] * x[o + j];
+  for (int j = 1; j < d; ++j) out[o + j] -= c * (x[o + j] - tdir[o + j] * ux);
 }
 
 inline int g1(long n) { return ceil_div(n > 0 ? n : 1, 256); }
@@ -707,10 +780,22 @@ void launch_scatter(int n, const int* slot, const int* src, const double* val, d
 }
 void launch_hs_scatter(const ConeDev& C, int nhs, const int* slot, const int* kind, const int* blk,
                        const int* ia, const int* ib, const int* row, double sreg_a, double* Kx,
-                       hipStream_t st) {
+                       hipStream_t st, const double* tcorr, const double* tdir) {
   if (nhs <= 0) return;
   hipLaunchKernelGGL(k_hs_scatter, dim3(g1(nhs)), dim3(256), 0, st, C, nhs, slot, kind, blk, ia, ib,
-                     row, sreg_a, Kx);
+                     row, sreg_a, tcorr, tdir, Kx);
+  CX_LAUNCH_CHECK();
+}
+void launch_soc_tangent_apply(const ConeDev& C, const double* tcorr, const double* tdir,
+                              const double* x, double* out, hipStream_t st) {
+  if (C.nsoc <= 0) return;
+  hipLaunchKernelGGL(k_soc_tangent_apply, dim3(g1(C.nsoc)), dim3(256), 0, st, C, tcorr, tdir, x, out);
+  CX_LAUNCH_CHECK();
+}
+void launch_soc_tangent(const ConeDev& C, const double* s, const double* z, double* tcorr, double* tdir,
+                        hipStream_t st) {
+  if (C.nsoc <= 0) return;
+  hipLaunchKernelGGL(k_soc_tangent, dim3(g1(C.nsoc)), dim3(256), 0, st, C, s, z, tcorr, tdir);
   CX_LAUNCH_CHECK();
 }
 void launch_scale_csc(int cols, const int* ptr, const int* row, const double* in, double c,
@@ -729,6 +814,15 @@ void launch_vmuldiv(int n, const double* in, const double* mul, const double* di
                     double* out, hipStream_t st) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_vmuldiv, dim3(g1(n)), dim3(256), 0, st, n, in, mul, div, c, out);
+  CX_LAUNCH_CHECK();
+}
+
+void launch_grad_csc(int cols, int nnz, const int* ptr, const int* row, int mode, const double* u,
+                     const double* v, const double* xs, const double* zs, double inv_tau,
+                     const double* drow, const double* dcol, double c, double* g, hipStream_t st) {
+  if (nnz <= 0 || cols <= 0) return;
+  hipLaunchKernelGGL(k_grad_csc, dim3(g1(nnz)), dim3(256), 0, st, cols, nnz, ptr, row, mode, u, v, xs,
+                     zs, inv_tau, drow, dcol, c, g);
   CX_LAUNCH_CHECK();
 }
 

@@ -86,7 +86,13 @@ void launch_scatter(int n, const int* slot, const int* src, const double* val, d
 // Hs slots of the KKT (2,2) block
 void launch_hs_scatter(const ConeDev& C, int nhs, const int* slot, const int* kind, const int* blk,
                        const int* ia, const int* ib, const int* row, double sreg_a, double* Kx,
-                       hipStream_t st);
+                       hipStream_t st, const double* tcorr = nullptr, const double* tdir = nullptr);
+// out -= tcorr_k P_T x per SOC block (see k_soc_tangent_apply)
+void launch_soc_tangent_apply(const ConeDev& C, const double* tcorr, const double* tdir,
+                              const double* x, double* out, hipStream_t st);
+// SOC tangent correction for the adjoint system (see k_soc_tangent)
+void launch_soc_tangent(const ConeDev& C, const double* s, const double* z, double* tcorr, double* tdir,
+                        hipStream_t st);
 // equilibration of values on device: out[k] = in[k] * c * d[r[k]] * d[col(k)]
 void launch_scale_csc(int cols, const int* ptr, const int* row, const double* in, double c,
                       const double* drow, const double* dcol, double* out, hipStream_t st);
@@ -95,5 +101,10 @@ void launch_vscale(int n, const double* d, double c, const double* in, double* o
 // out[i] = in[i] * mul[i] / div[i]
 void launch_vmuldiv(int n, const double* in, const double* mul, const double* div, double c,
                     double* out, hipStream_t st);
+
+// Adjoint gradients on a CSC pattern (mode 0: A, mode 1: P upper); see k_grad_csc.
+void launch_grad_csc(int cols, int nnz, const int* ptr, const int* row, int mode, const double* u,
+                     const double* v, const double* xs, const double* zs, double inv_tau,
+                     const double* drow, const double* dcol, double c, double* g, hipStream_t st);
 
 }  // namespace cx

@@ -72,6 +72,7 @@ bool Solver::setup(int n, int m, const int* Pp, const int* Pi, const double* Px,
                    int nonneg_dim, int nsoc, const int* soc_dims, const cx_settings& st) {
   const double t0 = now_s();
   setup_done_ = false;
+  bw_valid_ = false;
   if (zero_dim < 0 || nonneg_dim < 0 || n < 0 || m < 0 || nsoc < 0) return false;
   long total = (long)zero_dim + nonneg_dim;
   for (int k = 0; k < nsoc; ++k) {
@@ -456,6 +457,7 @@ double Solver::kkt_solve(const double* rhs, double* x) {
 bool Solver::update_data(const int* Pp, const int* Pi, const double* Px, const double* q,
                          const int* Ap, const int* Ai, const double* Ax, const double* b) {
   if (!setup_done_) return false;
+  bw_valid_ = false;
   const bool hasP = Px != nullptr, hasA = Ax != nullptr;
   const int nP = (int)Pi_ref_.size(), nA = (int)Ai_ref_.size();
   auto same = [](const int* p, const int* i, int cols, const std::vector<int>& pr,
@@ -523,6 +525,73 @@ void Solver::set_warm_start(const double* x, const double* s, const double* z) {
   wz_.upload(zv, stream_);
   CX_HIP(hipStreamSynchronize(stream_));
   have_warm_ = true;
+}
+
+// Backward (adjoint) pass at the optimum of the last solve.  The solution
+// map's differential follows from the KKT conditions with complementarity
+// linearized as ds = -H dz:
+//   [P, A'; A, -H] [dx; dz] = [-(dP x + dq + dA' z); db - dA x].
+// H is the final NT scaling Hs with, on every SOC block, the eigenvalue of the
+// directions orthogonal to z1 set to s0/z0 (the limit of the exact
+// linearization arw(z) ds + arw(s) dz = 0; NT gives it only on the central
+// path).  Nonnegative and zero rows are exact already.  The corrected KKT
+// matrix (static regularization included) is factored once per solved problem,
+// on the first backward call; for L with gradients (wx, ws, wz) in original
+// units the adjoint solve K [u; v] = [D wx; E wz / c - H (ws / E)] (plus the
+// usual refinement) then gives
+//   dL/dq = -c D u,  dL/db = E v,  dL/dA_ij = -(z_i u_j + v_i x_j) E_i D_j,
+//   dL/dP_ij (upper) = -(u_i x_j + u_j x_i) c D_i D_j, diagonal -u_i x_i c D_i^2,
+// with x, z the normalized optimum in the equilibrated space.
+int Solver::backward(const double* wx, const double* ws, const double* wz, double* gP, double* gq,
+                     double* gA, double* gb) {
+  if (!setup_done_ || !bw_valid_ || status_ != CX_SOLVED || !factorized_) return 0;
+  const int nA = Ap_ref_[n_];
+  std::vector<double> tx(n_, 0.0), us(m_, 0.0), ntz(m_, 0.0);
+  for (int i = 0; i < n_; ++i) tx[i] = wx ? wx[i] * ed_[i] : 0.0;
+  for (int i = 0; i < m_; ++i) {
+    us[i] = ws ? ws[i] / ee_[i] : 0.0;
+    ntz[i] = wz ? -wz[i] * ee_[i] / ec_ : 0.0;
+  }
+  if (!bw_factored_) {
+    if (!soc_dims_.empty()) {
+      bw_tcorr_.alloc(soc_dims_.size());
+      bw_tdir_.alloc(m_);
+      launch_soc_tangent(cone_, s_.p, z_.p, bw_tcorr_.p, bw_tdir_.p, stream_);
+      launch_hs_scatter(cone_, nhs_, hs_slot_.p, hs_kind_.p, hs_blk_.p, hs_a_.p, hs_b_.p, hs_row_.p,
+                        st_.static_reg_A, Kx_.p, stream_, bw_tcorr_.p, bw_tdir_.p);
+      if (!factor_retry()) return 0;
+    }
+    bw_factored_ = true;
+  }
+  bw_rhs_.alloc(N_);
+  bw_sol_.alloc(N_);
+  bw_us_.upload(us, stream_);
+  bw_ntz_.upload(ntz, stream_);
+  if (n_) CX_HIP(hipMemcpyAsync(bw_rhs_.p, tx.data(), n_ * sizeof(double), hipMemcpyHostToDevice, stream_));
+  if (m_) launch_neg_hs_plus(cone_, m_, bw_us_.p, bw_ntz_.p, bw_rhs_.p + n_, stream_);
+  if (m_ && ws && !soc_dims_.empty())
+    launch_soc_tangent_apply(cone_, bw_tcorr_.p, bw_tdir_.p, bw_us_.p, bw_rhs_.p + n_, stream_);
+  const double rel = kkt_solve(bw_rhs_.p, bw_sol_.p);
+  if (!std::isfinite(rel)) return 0;
+  const double* u = bw_sol_.p;
+  const double* v = bw_sol_.p + n_;
+  const double inv_tau = 1.0 / std::max(tau_, 1e-30);
+  bw_gP_.alloc(std::max(1, P_nnz_));
+  bw_gA_.alloc(std::max(1, nA));
+  bw_gq_.alloc(std::max(1, n_));
+  bw_gb_.alloc(std::max(1, m_));
+  launch_grad_csc(n_, P_nnz_, dPp_.p, dPi_.p, 1, u, v, x_.p, z_.p, inv_tau, d_ed_.p, d_ed_.p, ec_,
+                  bw_gP_.p, stream_);
+  launch_grad_csc(n_, nA, dAp_.p, dAi_.p, 0, u, v, x_.p, z_.p, inv_tau, d_ee_.p, d_ed_.p, 1.0,
+                  bw_gA_.p, stream_);
+  launch_vmuldiv(n_, u, d_ed_.p, nullptr, -ec_, bw_gq_.p, stream_);
+  launch_vmuldiv(m_, v, d_ee_.p, nullptr, 1.0, bw_gb_.p, stream_);
+  if (gP) bw_gP_.download(gP, P_nnz_, stream_);
+  if (gA) bw_gA_.download(gA, nA, stream_);
+  if (gq) bw_gq_.download(gq, n_, stream_);
+  if (gb) bw_gb_.download(gb, m_, stream_);
+  CX_HIP(hipStreamSynchronize(stream_));
+  return 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -712,6 +781,7 @@ void Solver::finalize() {  // solver_impl.cpp:342-365
 int Solver::solve() {  // solver_impl.cpp:367-466
   const double t0 = now_s();
   status_ = CX_UNSOLVED;
+  bw_valid_ = false;
   iterations_ = 0;
   objective_ = pres_ = dres_ = gap_ = mu_out_ = 0;
   prof_fact_ms_ = prof_solve_ms_ = 0;
@@ -825,6 +895,8 @@ int Solver::solve() {  // solver_impl.cpp:367-466
     }
     status_ = final_status;
     iterations_ = iterations;
+    bw_valid_ = final_status == CX_SOLVED;
+    bw_factored_ = false;
     finalize();
     if (final_status == CX_SOLVED && st_.warm_start) {
       launch_copy(n_, x_.p, wx_.p, stream_);

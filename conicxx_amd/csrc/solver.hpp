@@ -36,6 +36,8 @@ class Solver {
   void kkt_stats(double* out8) const;
   void profile(double* out8) const;
   int kernel_stats(double* out, int max_kinds) const;
+  int backward(const double* wx, const double* ws, const double* wz, double* gP, double* gq,
+               double* gA, double* gb);
 
  private:
   // KKT (kkt_system.cpp)
@@ -110,6 +112,11 @@ class Solver {
   double rtau_ = 0, dqx_ = 0, dbz_ = 0, dsz_ = 0, dxPx_ = 0, nrx2_ = 0, nrz2_ = 0;
   double dtaua_ = 0, dkappaa_ = 0, dtau_ = 0, dkappa_ = 0;
   bool have_warm_ = false;
+  // backward pass
+  bool bw_valid_ = false;  // the factorization in memory belongs to the last Solved solve
+  bool bw_factored_ = false;  // the adjoint (tangent-corrected) matrix is factored
+  DevBuf<double> bw_tcorr_, bw_tdir_;
+  DevBuf<double> bw_rhs_, bw_sol_, bw_us_, bw_ntz_, bw_gP_, bw_gA_, bw_gq_, bw_gb_;
   // solution
   int status_ = CX_UNSOLVED, iterations_ = 0;
   std::vector<double> xo_, so_, zo_;

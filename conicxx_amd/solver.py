@@ -187,6 +187,7 @@ class Solver:
             return False
         self._n, self._m = n, m
         self._settings = settings
+        self._P, self._A = Pc, Ac  # patterns (the gradients of backward() live on them)
         keep = []
         a1, p1 = _lib.i32(Pc.indptr); a2, p2 = _lib.i32(Pc.indices); a3, p3 = _lib.f64(Pc.data)
         a4, p4 = _lib.i32(Ac.indptr); a5, p5 = _lib.i32(Ac.indices); a6, p6 = _lib.f64(Ac.data)
@@ -289,6 +290,32 @@ class Solver:
         n = _lib.lib().cx_get_kernel_stats(self._h, out.ctypes.data_as(_lib._f64p), len(self.KERNEL_CLASSES))
         return {self.KERNEL_CLASSES[k]: dict(ms=out[3 * k], launches=int(out[3 * k + 1]), work=out[3 * k + 2])
                 for k in range(n)}
+
+    def backward(self, dl_dx=None, dl_ds=None, dl_dz=None):
+        """Adjoint pass of the last Solved solve: gradients of a loss L given
+        dL/dx, dL/ds, dL/dz (None = zero).  Returns dict(P=csc on P's pattern,
+        q=ndarray, A=csc on A's pattern, b=ndarray).  One adjoint KKT solve on
+        the final factorization (no counterpart in the reference)."""
+        n, m = self._n, self._m
+        keep = []
+
+        def vec(v, k):
+            if v is None:
+                return _lib.NULL_F
+            a, p = _lib.f64(np.asarray(v, dtype=np.float64).reshape(-1))
+            if a.size != k:
+                raise ValueError("gradient has the wrong length")
+            keep.append(a)
+            return p
+        gP = np.zeros(self._P.nnz); gq = np.zeros(n); gA = np.zeros(self._A.nnz); gb = np.zeros(m)
+        ok = _lib.lib().cx_backward(self._h, vec(dl_dx, n), vec(dl_ds, m), vec(dl_dz, m),
+                                    gP.ctypes.data_as(_lib._f64p), gq.ctypes.data_as(_lib._f64p),
+                                    gA.ctypes.data_as(_lib._f64p), gb.ctypes.data_as(_lib._f64p))
+        if not ok:
+            raise RuntimeError("backward needs a Solved solve() immediately before it")
+        P = sp.csc_matrix((gP, self._P.indices.copy(), self._P.indptr.copy()), shape=self._P.shape)
+        A = sp.csc_matrix((gA, self._A.indices.copy(), self._A.indptr.copy()), shape=self._A.shape)
+        return {"P": P, "q": gq, "A": A, "b": gb}
 
     # snake_case aliases
     update_data = updateData

@@ -137,6 +137,17 @@ void cx_get_profile(const cx_solver* h, double* out8);
  * classes written.  No counterpart in the reference. */
 int cx_get_kernel_stats(const cx_solver* h, double* out, int max_kinds);
 
+/* Backward (adjoint) pass of the last solve, which must have returned
+ * CX_SOLVED.  Given dL/dx (n), dL/ds (m), dL/dz (m) in original units (any may be
+ * NULL = zero), writes dL/dP on P's stored pattern (nnz(P) values in setup's CSC
+ * order; entries below the diagonal get 0), dL/dq (n), dL/dA on A's pattern
+ * (nnz(A)) and dL/db (m); any output may be NULL.  One adjoint KKT solve that
+ * reuses the final factorization, then the rank-one gradient assembly on the
+ * device.  Returns 1 on success, 0 if there is no solved problem or the solve
+ * failed.  No counterpart in the reference (the reference has no derivatives). */
+int cx_backward(cx_solver* h, const double* dl_dx, const double* dl_ds, const double* dl_dz,
+                double* dP, double* dq, double* dA, double* db);
+
 /* Host-only symbolic analysis of a symmetric pattern (lower CSC, diagonal
  * included), for testing: writes the final permutation (new->old, N ints) and
  * the column counts of L (N ints) when non-NULL, and stats8 as above.
