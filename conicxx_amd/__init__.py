@@ -5,3 +5,4 @@ ConeSpec / Solution interface; see conicxx_amd.solver.
 """
 from .solver import (ConeSpec, FactorPrecision, Info, LinearSolverBackend, Settings, Solution,  # noqa: F401
                      Solver, Status, analyze_pattern, toString)
+from .batch import BatchSolver  # noqa: F401,E402

@@ -74,6 +74,21 @@ SIGNATURES = [
     ("cx_get_profile", None, [ctypes.c_void_p, _f64p]),
     ("cx_get_kernel_stats", ctypes.c_int, [ctypes.c_void_p, _f64p, ctypes.c_int]),
     ("cx_backward", ctypes.c_int, [ctypes.c_void_p, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p]),
+    ("cx_batch_create", ctypes.c_void_p, []),
+    ("cx_batch_destroy", None, [ctypes.c_void_p]),
+    ("cx_batch_setup", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _i32p, _i32p, ctypes.POINTER(_i32p),
+                                      ctypes.POINTER(_i32p), ctypes.POINTER(_f64p), ctypes.POINTER(_f64p),
+                                      ctypes.POINTER(_i32p), ctypes.POINTER(_i32p), ctypes.POINTER(_f64p),
+                                      ctypes.POINTER(_f64p), _i32p, _i32p, _i32p, ctypes.POINTER(_i32p),
+                                      ctypes.POINTER(CxSettings)]),
+    ("cx_batch_update_data", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_f64p), ctypes.POINTER(_f64p)]),
+    ("cx_batch_solve", ctypes.c_int, [ctypes.c_void_p, _i32p]),
+    ("cx_batch_get_solution", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _f64p, _f64p, _f64p, _f64p,
+                                             ctypes.POINTER(CxInfo)]),
+    ("cx_batch_set_settings", None, [ctypes.c_void_p, ctypes.POINTER(CxSettings)]),
+    ("cx_batch_autotune", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("cx_batch_get_tuning", None, [ctypes.c_void_p, _f64p]),
+    ("cx_batch_last_solve_ms", ctypes.c_double, [ctypes.c_void_p]),
     ("cx_analyze_pattern", ctypes.c_int, [ctypes.c_int, _i32p, _i32p, _i32p, _i32p, _f64p]),
 ]
 

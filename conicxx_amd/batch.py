@@ -1,0 +1,117 @@
+"""Batched form over the C-ABI (cx_batch_*): many independent conic QPs in one
+persistent launch per solve, replayed as a HIP graph across update_data()
+calls.  Each problem follows the same method as conicxx_amd.Solver."""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .solver import ConeSpec, Info, Settings, Solution, Status, _as_csc
+
+
+class BatchSolver:
+    def __init__(self):
+        h = _lib.lib().cx_batch_create()
+        if not h:
+            raise RuntimeError("cx_batch_create failed: no usable GPU (the solver runs on the device only)")
+        self._h = ctypes.c_void_p(h)
+        self._dims = []
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            _lib.lib().cx_batch_destroy(h)
+            self._h = None
+
+    def setup(self, problems, settings=None):
+        """problems: sequence of (P, q, A, b, ConeSpec)."""
+        settings = settings or Settings()
+        K = len(problems)
+        keep = []
+        arrs = {k: [] for k in ("Pp", "Pi", "Px", "q", "Ap", "Ai", "Ax", "b", "sd")}
+        n, m, zd, nd, ns = (np.zeros(K, np.int32) for _ in range(5))
+        self._dims = []
+        for k, (P, q, A, b, cone) in enumerate(problems):
+            if not isinstance(cone, ConeSpec):
+                cone = ConeSpec(*cone)
+            Pc = _as_csc(P, "P", upper=True)
+            Ac = _as_csc(A, "A")
+            q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1)
+            b = np.ascontiguousarray(b, dtype=np.float64).reshape(-1)
+            n[k], m[k] = Pc.shape[0], Ac.shape[0]
+            zd[k], nd[k], ns[k] = cone.zero_dim, cone.nonneg_dim, len(cone.soc_dims)
+            self._dims.append((int(n[k]), int(m[k])))
+            for key, a, conv in (("Pp", Pc.indptr, _lib.i32), ("Pi", Pc.indices, _lib.i32),
+                                 ("Px", Pc.data, _lib.f64), ("q", q, _lib.f64),
+                                 ("Ap", Ac.indptr, _lib.i32), ("Ai", Ac.indices, _lib.i32),
+                                 ("Ax", Ac.data, _lib.f64), ("b", b, _lib.f64),
+                                 ("sd", np.asarray(cone.soc_dims, np.int32).reshape(-1), _lib.i32)):
+                arr, ptr = conv(a if len(a) else np.zeros(1))
+                keep.append(arr)
+                arrs[key].append(ptr)
+
+        def parr(lst, t):
+            return (t * K)(*lst)
+        ok = _lib.lib().cx_batch_setup(
+            self._h, K, n.ctypes.data_as(_lib._i32p), m.ctypes.data_as(_lib._i32p),
+            parr(arrs["Pp"], _lib._i32p), parr(arrs["Pi"], _lib._i32p), parr(arrs["Px"], _lib._f64p),
+            parr(arrs["q"], _lib._f64p), parr(arrs["Ap"], _lib._i32p), parr(arrs["Ai"], _lib._i32p),
+            parr(arrs["Ax"], _lib._f64p), parr(arrs["b"], _lib._f64p), zd.ctypes.data_as(_lib._i32p),
+            nd.ctypes.data_as(_lib._i32p), ns.ctypes.data_as(_lib._i32p), parr(arrs["sd"], _lib._i32p),
+            ctypes.byref(settings.to_c()))
+        return bool(ok)
+
+    def update_data(self, qs=None, bs=None):
+        """New q and/or b for every problem (a list; None entries = unchanged)."""
+        K = len(self._dims)
+        keep = []
+
+        def parr(vals, idx):
+            if vals is None:
+                return None
+            out = (_lib._f64p * K)()
+            for k, v in enumerate(vals):
+                if v is None:
+                    out[k] = _lib.NULL_F
+                    continue
+                a, p = _lib.f64(np.asarray(v, dtype=np.float64).reshape(-1))
+                if a.size != self._dims[k][idx]:
+                    raise ValueError(f"problem {k}: wrong length")
+                keep.append(a)
+                out[k] = p
+            return out
+        return bool(_lib.lib().cx_batch_update_data(self._h, parr(qs, 0), parr(bs, 1)))
+
+    def solve(self):
+        """Solves every problem; returns the array of statuses."""
+        st = np.zeros(len(self._dims), np.int32)
+        r = _lib.lib().cx_batch_solve(self._h, st.ctypes.data_as(_lib._i32p))
+        if r < 0:
+            raise RuntimeError("batched solve failed on the device")
+        return st
+
+    def solution(self, k):
+        n, m = self._dims[k]
+        x = np.zeros(n); s = np.zeros(m); z = np.zeros(m)
+        obj = ctypes.c_double(0.0)
+        info = _lib.CxInfo()
+        st = _lib.lib().cx_batch_get_solution(self._h, k, x.ctypes.data_as(_lib._f64p),
+                                              s.ctypes.data_as(_lib._f64p), z.ctypes.data_as(_lib._f64p),
+                                              ctypes.byref(obj), ctypes.byref(info))
+        return Solution(status=Status(st), x=x, s=s, z=z, objective=obj.value,
+                        info=Info(info.iterations, info.primal_residual, info.dual_residual,
+                                  info.duality_gap, info.mu, info.setup_time_s, info.solve_time_s))
+
+    def set_settings(self, settings):
+        _lib.lib().cx_batch_set_settings(self._h, ctypes.byref(settings.to_c()))
+
+    def autotune(self, reps=1):
+        return int(_lib.lib().cx_batch_autotune(self._h, reps))
+
+    def tuning(self):
+        out = np.zeros(4)
+        _lib.lib().cx_batch_get_tuning(self._h, out.ctypes.data_as(_lib._f64p))
+        return dict(lds_doubles=int(out[0]), grid=int(out[1]), max_packed=int(out[2]), autotuned=bool(out[3]))
+
+    def last_solve_ms(self):
+        return float(_lib.lib().cx_batch_last_solve_ms(self._h))

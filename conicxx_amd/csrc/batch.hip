@@ -1,0 +1,1218 @@
+// Batched persistent interior-point solver (see batch.hpp).
+//
+// One workgroup (256 threads) owns one problem from its initial point to its
+// final status, then takes the next problem from a shared counter; problems
+// are handed out largest first.  The method is the single-problem path's
+// (solver_impl.cpp:104-466 restated step by step: initial point, NT scaling,
+// constant solve, residuals, tau elimination, Mehrotra predictor-corrector,
+// step length, rescaling, convergence and infeasibility tests, finalize,
+// warm-start carry), with the KKT factorization specialised as described in
+// batch.hpp.  Every loop is bounded (max_iter, retry ladders); there are no
+// waits on other workgroups.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <stdexcept>
+
+#include "batch.hpp"
+#include "cone_device.hpp"
+#include "equilibrate.hpp"
+
+namespace cx {
+
+namespace {
+
+using namespace conedev;
+
+constexpr int BG = 256;  // threads per problem
+constexpr int BATCH_REFINE_MAX = 40;
+
+enum { PV_CHECK = 0, PV_REGULARIZE = 1 };
+
+struct BatchArgs {
+  int nprob;
+  const BatchDesc* desc;
+  const int* order;
+  int* counter;
+  // settings
+  double tol_feas, tol_gap_abs, tol_gap_rel, tol_infeas, sreg_p, sreg_a, dyn_eps, dyn_delta;
+  double refine_tol, step_frac;
+  int max_iter, refine_max_iter, mode, warm_start;
+  int lds_doubles;
+  // structure (entry indices global, row/column values problem-relative)
+  const int *pf_ptr, *pf_col, *pf_src;
+  const int *ac_ptr, *ac_row;
+  const int *ar_ptr, *ar_col, *ar_src;
+  const int *soc_off, *soc_dim;
+  const int *rk, *kr;  // per row: index among the kept rows or -1 (bound row); kept rows
+  // values
+  const double *Pv, *Av, *q, *b, *ed, *ee;
+  double *W, *Mg;
+  double *beta, *vv;
+  int* ident;
+  // warm start and results
+  double *wx, *ws, *wz;
+  int* have_warm;
+  double *xo, *so, *zo, *info;
+  int *status, *iters;
+};
+
+// ---------------------------------------------------------------------------
+// workgroup reductions (all threads receive the result)
+// ---------------------------------------------------------------------------
+template <int K>
+__device__ inline void gsum(double (&v)[K], double* red) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int c = 0; c < K; ++c) {
+    double t = v[c];
+    for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+    if (lane == 0) red[c * 4 + wv] = t;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < K; ++c) v[c] = (red[c * 4] + red[c * 4 + 1]) + (red[c * 4 + 2] + red[c * 4 + 3]);
+  __syncthreads();
+}
+__device__ inline double gsum1(double v, double* red) {
+  double a[1] = {v};
+  gsum<1>(a, red);
+  return a[0];
+}
+__device__ inline double gmin1(double v, double* red) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+  if (lane == 0) red[wv] = v;
+  __syncthreads();
+  const double r = fmin(fmin(red[0], red[1]), fmin(red[2], red[3]));
+  __syncthreads();
+  return r;
+}
+__device__ inline double dmax(double a, double b) { return a > b ? a : b; }
+__device__ inline double dmin(double a, double b) { return a < b ? a : b; }
+
+// ---------------------------------------------------------------------------
+// per-problem view
+// ---------------------------------------------------------------------------
+struct PV {
+  int n, m, nz, nn, nsoc, nk, N, nr, pnnz;
+  int co, so, ko, pfp, acp, arp;
+  double cur_a;  // (2,2) regularization of the current factorization
+  double ec;
+  const double *q, *b;
+  double *x, *Px, *rx, *x1, *Px1, *dxa, *dx, *wn, *wn2;
+  double *s, *z, *rz, *z1, *dza, *dsa, *dz, *ds, *dsk, *wm, *wm2;
+  double *rhs, *sol, *res, *tmp, *dsol;
+  double* M;
+  ConeDev C;
+  // scalar state
+  double tau, kappa, rtau, dqx, dbz, dsz, dxPx, nrx2, nrz2;
+  double dtaua, dkappaa, dtau, dkappa;
+  double last_p, last_a;
+};
+
+__device__ inline long packed_size(int nr) { return (long)nr * (nr + 1) / 2; }
+__device__ inline long colbase(int j, int nr) { return (long)j * nr - (long)j * (j - 1) / 2; }
+
+// y = P x (full symmetric CSR)
+__device__ void pmul(const BatchArgs& A, const PV& p, const double* x, double* y) {
+  for (int i = threadIdx.x; i < p.n; i += BG) {
+    double acc = 0.0;
+    for (int e = A.pf_ptr[p.pfp + i]; e < A.pf_ptr[p.pfp + i + 1]; ++e) acc += A.Pv[A.pf_src[e]] * x[A.pf_col[e]];
+    y[i] = acc;
+  }
+  __syncthreads();
+}
+// y = A x (all rows)
+__device__ void amul(const BatchArgs& A, const PV& p, const double* x, double* y) {
+  for (int i = threadIdx.x; i < p.m; i += BG) {
+    double acc = 0.0;
+    for (int e = A.ar_ptr[p.arp + i]; e < A.ar_ptr[p.arp + i + 1]; ++e) acc += A.Av[A.ar_src[e]] * x[A.ar_col[e]];
+    y[i] = acc;
+  }
+  __syncthreads();
+}
+// y = A' z
+__device__ void atmul(const BatchArgs& A, const PV& p, const double* z, double* y) {
+  for (int j = threadIdx.x; j < p.n; j += BG) {
+    double acc = 0.0;
+    for (int e = A.ac_ptr[p.acp + j]; e < A.ac_ptr[p.acp + j + 1]; ++e) acc += A.Av[e] * z[A.ac_row[e]];
+    y[j] = acc;
+  }
+  __syncthreads();
+}
+__device__ inline void vcopy(int len, const double* x, double* y) {
+  for (int i = threadIdx.x; i < len; i += BG) y[i] = x[i];
+  __syncthreads();
+}
+__device__ inline void vfill(int len, double v, double* y) {
+  for (int i = threadIdx.x; i < len; i += BG) y[i] = v;
+  __syncthreads();
+}
+__device__ inline void vlin2(int len, double a, const double* x, double b, const double* y, double* out) {
+  for (int i = threadIdx.x; i < len; i += BG) out[i] = a * x[i] + b * y[i];
+  __syncthreads();
+}
+__device__ inline void vlin3(int len, double a, const double* x, double b, const double* y, double c,
+                             const double* z, double* out) {
+  for (int i = threadIdx.x; i < len; i += BG) out[i] = a * x[i] + b * y[i] + c * z[i];
+  __syncthreads();
+}
+__device__ inline void vaxpy(int len, double a, const double* x, double* y) {
+  for (int i = threadIdx.x; i < len; i += BG) y[i] += a * x[i];
+  __syncthreads();
+}
+__device__ inline void vscale(int len, double a, double* x) {
+  for (int i = threadIdx.x; i < len; i += BG) x[i] *= a;
+  __syncthreads();
+}
+__device__ double count_nonfinite(int len, const double* x, double* red) {
+  double c = 0.0;
+  for (int i = threadIdx.x; i < len; i += BG) c += isfinite(x[i]) ? 0.0 : 1.0;
+  return gsum1(c, red);
+}
+
+// ---------------------------------------------------------------------------
+// cones (ConeSet over one problem)
+// ---------------------------------------------------------------------------
+__device__ void cone_scaling(const PV& p, const double* s, const double* z) {
+  for (int i = threadIdx.x; i < p.nn; i += BG) {
+    const int r = p.nz + i;
+    const double w = sqrt(s[r] / z[r]);
+    p.C.w[r] = w;
+    p.C.hsd[r] = w * w;
+  }
+  for (int k = threadIdx.x; k < p.nsoc; k += BG)
+    soc_scaling<1>(0, p.C.soc_off[k], p.C.soc_dim[k], s, z, p.C.v, &p.C.beta[k], &p.C.vv[k], &p.C.ident[k]);
+  __syncthreads();
+}
+// out = -(Hs x + y)
+__device__ void neg_hs_plus(const PV& p, const double* x, const double* y, double* out) {
+  for (int i = threadIdx.x; i < p.nz; i += BG) out[i] = -y[i];
+  for (int i = threadIdx.x; i < p.nn; i += BG) {
+    const int r = p.nz + i;
+    out[r] = -(p.C.hsd[r] * x[r] + y[r]);
+  }
+  for (int k = threadIdx.x; k < p.nsoc; k += BG) soc_neg_hs_plus<1>(0, p.C, k, x, y, out);
+  __syncthreads();
+}
+__device__ void combined_rhs(const PV& p, const double* z, const double* dsa, const double* dza,
+                             const double* rz, double smu, double oms, double* dsc, double* tail) {
+  for (int i = threadIdx.x; i < p.nz; i += BG) {
+    dsc[i] = 0.0;
+    tail[i] = 0.0 - oms * rz[i];
+  }
+  for (int i = threadIdx.x; i < p.nn; i += BG) {
+    const int r = p.nz + i;
+    const double w = p.C.w[r];
+    const double lam = w * z[r];
+    const double a1 = dsa[r] / w;
+    const double a2 = w * dza[r];
+    const double dc = lam * lam + a1 * a2 - smu;
+    const double lid = dc / lam;
+    const double dsv = w * lid;
+    dsc[r] = dsv;
+    tail[r] = dsv - oms * rz[r];
+  }
+  for (int k = threadIdx.x; k < p.nsoc; k += BG) soc_combined<1>(0, p.C, k, z, dsa, dza, rz, smu, oms, dsc, tail);
+  __syncthreads();
+}
+__device__ double cone_max_step(const PV& p, const double* x, const double* dx, double* red) {
+  double a = INFINITY;
+  for (int i = threadIdx.x; i < p.nn; i += BG) {
+    const int r = p.nz + i;
+    if (dx[r] < 0) a = fmin(a, -x[r] / dx[r]);
+  }
+  for (int k = threadIdx.x; k < p.nsoc; k += BG)
+    a = fmin(a, soc_max_step<1>(0, p.C.soc_off[k], p.C.soc_dim[k], x, dx));
+  return gmin1(a, red);
+}
+// ConeSet::margins: (min margin, sum of positive margins), the orthant counted as one block
+__device__ void margins(const PV& p, const double* x, double* mn, double* ps, double* red) {
+  double nmin = INFINITY, smin = INFINITY, spos = 0.0;
+  for (int i = threadIdx.x; i < p.nn; i += BG) nmin = fmin(nmin, x[p.nz + i]);
+  for (int k = threadIdx.x; k < p.nsoc; k += BG) {
+    const int off = p.C.soc_off[k], d = p.C.soc_dim[k];
+    double ss = 0.0;
+    for (int i = 1; i < d; ++i) ss += x[off + i] * x[off + i];
+    const double mg = x[off] - sqrt(ss);
+    if (isfinite(mg)) smin = fmin(smin, mg), spos += fmax(mg, 0.0);
+  }
+  nmin = gmin1(nmin, red);
+  smin = gmin1(smin, red);
+  spos = gsum1(spos, red);
+  double m0 = smin, p0 = spos;
+  if (p.nn > 0 && isfinite(nmin)) {
+    m0 = fmin(m0, nmin);
+    p0 += fmax(nmin, 0.0);
+  }
+  if (!isfinite(m0)) m0 = 0.0;
+  *mn = m0;
+  *ps = p0;
+}
+__device__ void unit_shift(const PV& p, double* x, double alpha) {
+  for (int i = threadIdx.x; i < p.nn; i += BG) x[p.nz + i] += alpha;
+  for (int k = threadIdx.x; k < p.nsoc; k += BG) x[p.C.soc_off[k]] += alpha;
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// KKT: M assembly (bound rows folded into the P block, every other row kept)
+// ---------------------------------------------------------------------------
+__device__ void assemble_M(const BatchArgs& A, const PV& p, double dp, double da) {
+  const int nr = p.nr, n = p.n;
+  double* M = p.M;
+  const long tot = packed_size(nr);
+  for (long e = threadIdx.x; e < tot; e += BG) M[e] = 0.0;
+  __syncthreads();
+  // variable columns: P + dp I + sum over bound rows a^2 / (hs + da), kept rows of A
+  for (int j = threadIdx.x; j < n; j += BG) {
+    const long cb = colbase(j, nr);
+    for (int e = A.pf_ptr[p.pfp + j]; e < A.pf_ptr[p.pfp + j + 1]; ++e) {
+      const int l = A.pf_col[e];
+      if (l >= j) M[cb + (l - j)] += A.Pv[A.pf_src[e]];
+    }
+    double dg = dp;
+    for (int e = A.ac_ptr[p.acp + j]; e < A.ac_ptr[p.acp + j + 1]; ++e) {
+      const int i = A.ac_row[e];
+      const double aij = A.Av[e];
+      const int kk = A.rk[p.co + i];
+      if (kk < 0)
+        dg += aij * aij / (p.C.hsd[i] + da);
+      else
+        M[cb + (n + kk - j)] = aij;
+    }
+    M[cb] += dg;
+  }
+  // constraint block: -(H + da I) on the kept rows (zero: 0, nonnegative: Hs_ii, SOC: Hs block)
+  for (int kk = threadIdx.x; kk < p.nk; kk += BG) {
+    const int i = A.kr[p.ko + kk];
+    if (i < p.nz)
+      M[colbase(n + kk, nr)] = -da;
+    else if (i < p.nz + p.nn)
+      M[colbase(n + kk, nr)] = -(p.C.hsd[i] + da);
+  }
+  for (int k = threadIdx.x; k < p.nsoc; k += BG) {
+    const int off = p.C.soc_off[k], d = p.C.soc_dim[k];
+    const int k0 = n + A.rk[p.co + off];
+    const double* v = p.C.v + off;
+    const double be = p.C.beta[k], vvk = p.C.vv[k];
+    const bool id = p.C.ident[k] != 0;
+    for (int c = 0; c < d; ++c) {
+      const long cb = colbase(k0 + c, nr);
+      for (int a = c; a < d; ++a) {
+        double h;
+        if (id)
+          h = (a == c) ? 1.0 : 0.0;
+        else
+          h = be * be * (4.0 * vvk * v[a] * v[c] - 2.0 * v[a] * (jsign(c) * v[c]) -
+                         2.0 * (jsign(a) * v[a]) * v[c] + (a == c ? 1.0 : 0.0));
+        M[cb + (a - c)] = -(h + (a == c ? da : 0.0));
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// LDL^T of M in place (packed lower); pivot rule of the reference's backends.
+// Returns acceptance as in try_factor; colk: LDS scratch of nr doubles.
+__device__ bool factor_M(const BatchArgs& A, const PV& p, double* colk, double* red) {
+  const int nr = p.nr;
+  double* M = p.M;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  int zero_piv = 0, nreg = 0;
+  for (int k = 0; k < nr; ++k) {
+    const long ck = colbase(k, nr);
+    if (tid == 0) {
+      double d = M[ck];
+      const double sg = k < p.n ? 1.0 : -1.0;
+      if (A.mode == PV_REGULARIZE) {
+        if (!(d * sg >= A.dyn_eps)) {  // regularized_ldlt.cpp:132-139
+          d += sg * A.dyn_delta;
+          if (d * sg < A.dyn_eps) d = sg * A.dyn_delta;
+          ++nreg;
+        }
+      } else if (d == 0.0) {
+        zero_piv = 1;
+      }
+      M[ck] = d;
+    }
+    __syncthreads();
+    const double dinv = 1.0 / M[ck];
+    for (int i = k + 1 + tid; i < nr; i += BG) {
+      const double c = M[ck + (i - k)];
+      colk[i] = c;
+      M[ck + (i - k)] = c * dinv;
+    }
+    __syncthreads();
+    for (int j = k + 1 + wv; j < nr; j += 4) {
+      const long cj = colbase(j, nr);
+      const double cjv = colk[j];
+      for (int i = j + lane; i < nr; i += 64) M[cj + (i - j)] -= M[ck + (i - k)] * cjv;
+    }
+    __syncthreads();
+  }
+  double mind = INFINITY, nanf = 0.0;
+  for (int k = tid; k < nr; k += BG) {
+    const double d = M[colbase(k, nr)];
+    if (isnan(d)) nanf = 1.0;
+    mind = fmin(mind, fabs(d));
+  }
+  mind = gmin1(mind, red);
+  nanf = gsum1(nanf, red);
+  double zr[2] = {(double)zero_piv, (double)nreg};
+  gsum<2>(zr, red);
+  if (nanf > 0) mind = NAN;
+  if (A.mode == PV_CHECK && zr[0] > 0) return false;
+  if (A.mode == PV_REGULARIZE && zr[1] > 0.05 * (double)p.N) return false;
+  return isfinite(mind) && mind > A.dyn_eps;
+}
+
+__device__ bool try_factor(const BatchArgs& A, PV& p, double ep, double ea, double* colk, double* red) {
+  p.cur_a = A.sreg_a + ea;
+  assemble_M(A, p, A.sreg_p + ep, p.cur_a);
+  return factor_M(A, p, colk, red);
+}
+
+// factorizeWithRetry (kkt_system.cpp:242-264)
+__device__ bool factor_retry(const BatchArgs& A, PV& p, double* colk, double* red) {
+  double ep = 0.0, ea = 0.0;
+  for (int att = 0; att < 18; ++att) {
+    if (try_factor(A, p, ep, ea, colk, red)) {
+      p.last_p = ep, p.last_a = ea;
+      return true;
+    }
+    ep = dmax(ep * 10.0, A.dyn_delta);
+    ea = dmax(ea * 10.0, A.dyn_delta);
+  }
+  return false;
+}
+
+// x = K^-1 rhs through the factored M (no refinement).  Bound rows:
+// y_i = (a_i x_j - r2_i) / (hs_i + da) after the solve of M.
+__device__ void reduced_solve(const BatchArgs& A, const PV& p, const double* rhs, double* x) {
+  const int n = p.n, nr = p.nr, tid = threadIdx.x;
+  const double* M = p.M;
+  const double da = p.cur_a;
+  double* t = p.tmp;
+  for (int j = tid; j < n; j += BG) {
+    double acc = rhs[j];
+    for (int e = A.ac_ptr[p.acp + j]; e < A.ac_ptr[p.acp + j + 1]; ++e) {
+      const int i = A.ac_row[e];
+      if (A.rk[p.co + i] < 0) acc += A.Av[e] * rhs[n + i] / (p.C.hsd[i] + da);
+    }
+    t[j] = acc;
+  }
+  for (int kk = tid; kk < p.nk; kk += BG) t[n + kk] = rhs[n + A.kr[p.ko + kk]];
+  __syncthreads();
+  // L y = t
+  for (int k = 0; k < nr - 1; ++k) {
+    const long ck = colbase(k, nr);
+    const double yk = t[k];
+    for (int i = k + 1 + tid; i < nr; i += BG) t[i] -= M[ck + (i - k)] * yk;
+    __syncthreads();
+  }
+  for (int k = tid; k < nr; k += BG) t[k] /= M[colbase(k, nr)];
+  __syncthreads();
+  // L' x = y
+  for (int k = nr - 1; k > 0; --k) {
+    const double xk = t[k];
+    for (int j = tid; j < k; j += BG) t[j] -= M[colbase(j, nr) + (k - j)] * xk;
+    __syncthreads();
+  }
+  for (int j = tid; j < n; j += BG) x[j] = t[j];
+  for (int kk = tid; kk < p.nk; kk += BG) x[n + A.kr[p.ko + kk]] = t[n + kk];
+  for (int i = tid; i < p.m; i += BG) {
+    if (A.rk[p.co + i] >= 0) continue;
+    const int e = A.ar_ptr[p.arp + i];
+    x[n + i] = (A.Av[A.ar_src[e]] * t[A.ar_col[e]] - rhs[n + i]) / (p.C.hsd[i] + da);
+  }
+  __syncthreads();
+}
+
+// res = rhs - K x with K the statically regularized KKT matrix (the refinement target)
+__device__ void kkt_residual(const BatchArgs& A, const PV& p, const double* rhs, const double* x, double* res) {
+  const int n = p.n;
+  pmul(A, p, x, p.wn);
+  atmul(A, p, x + n, p.wn2);
+  amul(A, p, x, p.wm);
+  vfill(p.m, 0.0, p.wm2);
+  neg_hs_plus(p, x + n, p.wm2, p.wm2);  // -Hs y (each entry read and written by one thread)
+  for (int j = threadIdx.x; j < n; j += BG) res[j] = rhs[j] - p.wn[j] - A.sreg_p * x[j] - p.wn2[j];
+  for (int i = threadIdx.x; i < p.m; i += BG) res[n + i] = rhs[n + i] - p.wm[i] - p.wm2[i] + A.sreg_a * x[n + i];
+  __syncthreads();
+}
+
+// escalateAndResolve (kkt_system.cpp:266-294)
+__device__ bool escalate(const BatchArgs& A, PV& p, const double* rhs, double* x, double* colk, double* red) {
+  double ep = p.last_p, ea = p.last_a;
+  for (int att = 0; att < 10; ++att) {
+    ep = dmax(ep * 1000.0, A.dyn_delta);
+    ea = dmax(ea * 1000.0, A.dyn_delta);
+    if (!try_factor(A, p, ep, ea, colk, red)) continue;
+    reduced_solve(A, p, rhs, x);
+    if (count_nonfinite(p.N, x, red) == 0) {
+      p.last_p = ep, p.last_a = ea;
+      return true;
+    }
+  }
+  return false;
+}
+
+// KktSystem::solve (kkt_system.cpp:296-311)
+__device__ double kkt_solve(const BatchArgs& A, PV& p, const double* rhs, double* x, double* colk, double* red) {
+  reduced_solve(A, p, rhs, x);
+  const double nonfin = count_nonfinite(p.N, x, red);
+  kkt_residual(A, p, rhs, x, p.res);
+  double v[2] = {0.0, 0.0};
+  for (int i = threadIdx.x; i < p.N; i += BG) v[0] += rhs[i] * rhs[i], v[1] += p.res[i] * p.res[i];
+  gsum<2>(v, red);
+  const double rn = dmax(sqrt(v[0]), 1e-30);
+  double rr = v[1];
+  if (nonfin != 0.0) {
+    escalate(A, p, rhs, x, colk, red);
+    kkt_residual(A, p, rhs, x, p.res);
+    double r = 0.0;
+    for (int i = threadIdx.x; i < p.N; i += BG) r += p.res[i] * p.res[i];
+    rr = gsum1(r, red);
+  }
+  double rel = sqrt(rr) / rn;
+  // The reduced factorization is a weaker preconditioner than a full LDL^T
+  // once (Hs + eI)^-1 has entries near 1/e: refinement then continues past
+  // refine_max_iter (up to BATCH_REFINE_MAX steps) while it still reduces the
+  // residual, so each solve meets refine_tol as the reference's solves do.
+  const int rmax = A.refine_max_iter > 0 ? max(A.refine_max_iter, BATCH_REFINE_MAX) : 0;
+  for (int it = 0; it < rmax && rel > A.refine_tol; ++it) {
+    reduced_solve(A, p, p.res, p.dsol);
+    vaxpy(p.N, 1.0, p.dsol, x);
+    kkt_residual(A, p, rhs, x, p.res);
+    double r = 0.0;
+    for (int i = threadIdx.x; i < p.N; i += BG) r += p.res[i] * p.res[i];
+    const double rel_new = sqrt(gsum1(r, red)) / rn;
+    const bool stalled = it >= A.refine_max_iter && rel_new > 0.5 * rel;
+    rel = rel_new;
+    if (stalled) break;
+  }
+  return rel;
+}
+
+// ---------------------------------------------------------------------------
+// IPM pieces (solver_impl.cpp)
+// ---------------------------------------------------------------------------
+__device__ void shift_cold(const PV& p, double* v, double* red) {  // solver_impl.cpp:104-116
+  double mn, ps;
+  margins(p, v, &mn, &ps, red);
+  const int deg = p.nn + p.nsoc;
+  const double target = dmax(1.0, 0.1 * ps / (double)(deg > 1 ? deg : 1));
+  if (mn <= 0.0) {
+    unit_shift(p, v, -mn);
+    unit_shift(p, v, target);
+  } else if (mn < target) {
+    unit_shift(p, v, target - mn);
+  }
+}
+__device__ void ensure_interior_warm(const PV& p, double* v, double* red) {  // solver_impl.cpp:118-124
+  double mn, ps;
+  margins(p, v, &mn, &ps, red);
+  if (mn <= 1e-8) unit_shift(p, v, -mn + 1e-6);
+}
+
+__device__ bool initial_point(const BatchArgs& A, PV& p, double* colk, double* red) {  // solver_impl.cpp:126-158
+  const int n = p.n, m = p.m;
+  // scaling at the identity element e
+  double* e = p.dsk;
+  for (int i = threadIdx.x; i < m; i += BG) e[i] = (i < p.nz) ? 0.0 : (i < p.nz + p.nn ? 1.0 : 0.0);
+  __syncthreads();
+  for (int k = threadIdx.x; k < p.nsoc; k += BG) e[p.C.soc_off[k]] = 1.0;
+  __syncthreads();
+  cone_scaling(p, e, e);
+  if (!factor_retry(A, p, colk, red)) return false;
+  if (p.pnnz == 0) {
+    vfill(n, 0.0, p.rhs);
+    vcopy(m, p.b, p.rhs + n);
+    kkt_solve(A, p, p.rhs, p.sol, colk, red);
+    vcopy(n, p.sol, p.x);
+    vlin2(m, -1.0, p.sol + n, 0.0, p.sol + n, p.s);
+    vlin2(n, -1.0, p.q, 0.0, p.q, p.rhs);
+    vfill(m, 0.0, p.rhs + n);
+    kkt_solve(A, p, p.rhs, p.sol, colk, red);
+    vcopy(m, p.sol + n, p.z);
+  } else {
+    vlin2(n, -1.0, p.q, 0.0, p.q, p.rhs);
+    vcopy(m, p.b, p.rhs + n);
+    kkt_solve(A, p, p.rhs, p.sol, colk, red);
+    vcopy(n, p.sol, p.x);
+    vcopy(m, p.sol + n, p.z);
+    vlin2(m, -1.0, p.sol + n, 0.0, p.sol + n, p.s);
+  }
+  vfill(p.nz, 0.0, p.s);
+  shift_cold(p, p.s, red);
+  shift_cold(p, p.z, red);
+  p.tau = 1.0;
+  p.kappa = 1.0;
+  return count_nonfinite(n, p.x, red) + count_nonfinite(m, p.s, red) + count_nonfinite(m, p.z, red) == 0;
+}
+
+__device__ bool constant_solve(const BatchArgs& A, PV& p, double* colk, double* red) {  // solver_impl.cpp:165-175
+  vlin2(p.n, -1.0, p.q, 0.0, p.q, p.rhs);
+  vcopy(p.m, p.b, p.rhs + p.n);
+  kkt_solve(A, p, p.rhs, p.sol, colk, red);
+  vcopy(p.n, p.sol, p.x1);
+  vcopy(p.m, p.sol + p.n, p.z1);
+  pmul(A, p, p.x1, p.Px1);
+  return count_nonfinite(p.n, p.x1, red) + count_nonfinite(p.m, p.z1, red) == 0;
+}
+
+__device__ void residuals(const BatchArgs& A, PV& p, double* red) {  // solver_impl.cpp:177-186
+  const int n = p.n, m = p.m;
+  pmul(A, p, p.x, p.Px);
+  atmul(A, p, p.z, p.wn);
+  vlin3(n, -1.0, p.wn, -1.0, p.Px, -p.tau, p.q, p.rx);
+  amul(A, p, p.x, p.wm);
+  vlin3(m, 1.0, p.wm, 1.0, p.s, -p.tau, p.b, p.rz);
+  double v[6] = {0, 0, 0, 0, 0, 0};
+  for (int i = threadIdx.x; i < n; i += BG) {
+    v[0] += p.q[i] * p.x[i];
+    v[3] += p.x[i] * p.Px[i];
+    v[4] += p.rx[i] * p.rx[i];
+  }
+  for (int i = threadIdx.x; i < m; i += BG) {
+    v[1] += p.b[i] * p.z[i];
+    v[2] += p.s[i] * p.z[i];
+    v[5] += p.rz[i] * p.rz[i];
+  }
+  gsum<6>(v, red);
+  p.dqx = v[0], p.dbz = v[1], p.dsz = v[2], p.dxPx = v[3], p.nrx2 = v[4], p.nrz2 = v[5];
+  p.rtau = p.dqx + p.dbz + p.kappa + p.dxPx / p.tau;
+}
+
+__device__ bool tau_elim(const BatchArgs& A, PV& p, const double* xv, const double* zv, double dtau_rhs,
+                         double dkappa_rhs, double* dtau, double* red) {
+  const int n = p.n, m = p.m;
+  pmul(A, p, xv, p.wn);
+  vlin2(n, 1.0 / p.tau, p.x, -1.0, p.x1, p.wn2);
+  pmul(A, p, p.wn2, p.tmp);
+  double v[7] = {0, 0, 0, 0, 0, 0, 0};
+  for (int i = threadIdx.x; i < n; i += BG) {
+    v[0] += p.q[i] * xv[i];
+    v[2] += p.x[i] * p.wn[i];
+    v[3] += p.q[i] * p.x1[i];
+    v[5] += p.wn2[i] * p.tmp[i];
+    v[6] += p.x1[i] * p.Px1[i];
+  }
+  for (int i = threadIdx.x; i < m; i += BG) {
+    v[1] += p.b[i] * zv[i];
+    v[4] += p.b[i] * p.z1[i];
+  }
+  gsum<7>(v, red);
+  const double num = dtau_rhs - dkappa_rhs / p.tau + v[0] + v[1] + 2.0 * (v[2] / p.tau);
+  const double den = p.kappa / p.tau - v[3] - v[4] + v[5] - v[6];
+  if (den == 0.0 || !isfinite(num) || !isfinite(den)) return false;
+  *dtau = num / den;
+  return true;
+}
+
+__device__ bool affine_step(const BatchArgs& A, PV& p, double* colk, double* red) {  // solver_impl.cpp:192-231
+  const int n = p.n, m = p.m;
+  vcopy(n, p.rx, p.rhs);
+  vlin2(m, 1.0, p.s, -1.0, p.rz, p.rhs + n);
+  kkt_solve(A, p, p.rhs, p.sol, colk, red);
+  if (count_nonfinite(p.N, p.sol, red) != 0) return false;
+  const double dtau_rhs = p.rtau, dkappa_rhs = p.tau * p.kappa;
+  if (!tau_elim(A, p, p.sol, p.sol + n, dtau_rhs, dkappa_rhs, &p.dtaua, red)) return false;
+  vlin2(n, 1.0, p.sol, p.dtaua, p.x1, p.dxa);
+  vlin2(m, 1.0, p.sol + n, p.dtaua, p.z1, p.dza);
+  neg_hs_plus(p, p.dza, p.s, p.dsa);
+  p.dkappaa = -(dkappa_rhs + p.kappa * p.dtaua) / p.tau;
+  return isfinite(p.dtaua) && isfinite(p.dkappaa) &&
+         count_nonfinite(n, p.dxa, red) + count_nonfinite(m, p.dza, red) + count_nonfinite(m, p.dsa, red) == 0;
+}
+
+__device__ bool combined_step(const BatchArgs& A, PV& p, double sigma, double mu, double* colk, double* red) {
+  const int n = p.n, m = p.m;  // solver_impl.cpp:233-286
+  combined_rhs(p, p.z, p.dsa, p.dza, p.rz, sigma * mu, 1.0 - sigma, p.dsk, p.rhs + n);
+  vlin2(n, 1.0 - sigma, p.rx, 0.0, p.rx, p.rhs);
+  kkt_solve(A, p, p.rhs, p.sol, colk, red);
+  if (count_nonfinite(p.N, p.sol, red) != 0) return false;
+  const double dtau_rhs = (1.0 - sigma) * p.rtau;
+  const double dkappa_rhs = p.tau * p.kappa + p.dtaua * p.dkappaa - sigma * mu;
+  if (!tau_elim(A, p, p.sol, p.sol + n, dtau_rhs, dkappa_rhs, &p.dtau, red)) return false;
+  vlin2(n, 1.0, p.sol, p.dtau, p.x1, p.dx);
+  vlin2(m, 1.0, p.sol + n, p.dtau, p.z1, p.dz);
+  neg_hs_plus(p, p.dz, p.dsk, p.ds);
+  p.dkappa = -(dkappa_rhs + p.kappa * p.dtau) / p.tau;
+  return isfinite(p.dtau) && isfinite(p.dkappa) &&
+         count_nonfinite(n, p.dx, red) + count_nonfinite(m, p.dz, red) + count_nonfinite(m, p.ds, red) == 0;
+}
+
+__device__ double step_length(const PV& p, const double* dsv, const double* dzv, double dtau, double dkappa,
+                              double* red) {  // solver_impl.cpp:288-296
+  double alpha = 1.0;
+  if (dtau < 0) alpha = dmin(alpha, -p.tau / dtau);
+  if (dkappa < 0) alpha = dmin(alpha, -p.kappa / dkappa);
+  alpha = dmin(alpha, cone_max_step(p, p.z, dzv, red));
+  alpha = dmin(alpha, cone_max_step(p, p.s, dsv, red));
+  return alpha;
+}
+
+// ---------------------------------------------------------------------------
+// one problem, start to finish
+// ---------------------------------------------------------------------------
+__device__ void solve_problem(const BatchArgs& A, int pid, double* lds, double* colk, double* red) {
+  const BatchDesc D = A.desc[pid];
+  PV p;
+  p.n = D.n, p.m = D.m, p.nz = D.nz, p.nn = D.nn, p.nsoc = D.nsoc, p.nk = D.nk;
+  p.N = D.n + D.m, p.nr = D.nr, p.pnnz = D.pnnz;
+  p.co = D.co, p.so = D.so, p.ko = D.ko, p.pfp = D.pfp, p.acp = D.acp, p.arp = D.arp;
+  p.cur_a = 0.0;
+  p.ec = D.ec;
+  p.q = A.q + D.xo;
+  p.b = A.b + D.co;
+  const int n = D.n, m = D.m, N = n + m;
+  double* w = A.W + D.wofs;
+  p.x = w; w += n;
+  p.Px = w; w += n;
+  p.rx = w; w += n;
+  p.x1 = w; w += n;
+  p.Px1 = w; w += n;
+  p.dxa = w; w += n;
+  p.dx = w; w += n;
+  p.wn = w; w += n;
+  p.wn2 = w; w += n;
+  p.s = w; w += m;
+  p.z = w; w += m;
+  p.rz = w; w += m;
+  p.z1 = w; w += m;
+  p.dza = w; w += m;
+  p.dsa = w; w += m;
+  p.dz = w; w += m;
+  p.ds = w; w += m;
+  p.dsk = w; w += m;
+  p.wm = w; w += m;
+  p.wm2 = w; w += m;
+  double* cv = w; w += m;
+  double* cw = w; w += m;
+  double* chs = w; w += m;
+  p.rhs = w; w += N;
+  p.sol = w; w += N;
+  p.res = w; w += N;
+  p.tmp = w; w += N;
+  p.dsol = w; w += N;
+  p.M = (packed_size(D.nr) <= A.lds_doubles) ? lds : A.Mg + D.mofs;
+  p.C.zero_off = 0, p.C.zero_dim = D.nz;
+  p.C.nn_off = D.nz, p.C.nn_dim = D.nn;
+  p.C.nsoc = D.nsoc;
+  p.C.soc_off = A.soc_off + D.so;
+  p.C.soc_dim = A.soc_dim + D.so;
+  p.C.small_list = nullptr, p.C.nsmall = 0, p.C.large_list = nullptr, p.C.nlarge = 0;
+  p.C.beta = A.beta + D.so;
+  p.C.vv = A.vv + D.so;
+  p.C.ident = A.ident + D.so;
+  p.C.v = cv, p.C.w = cw, p.C.hsd = chs;
+  p.tau = p.kappa = 1.0;
+  p.last_p = p.last_a = 0.0;
+  p.rtau = p.dqx = p.dbz = p.dsz = p.dxPx = p.nrx2 = p.nrz2 = 0.0;
+  p.dtaua = p.dkappaa = p.dtau = p.dkappa = 0.0;
+
+  int status = CX_NUMERICAL_ERROR, iterations = 0;
+  bool ok = true;
+  const bool use_warm = A.warm_start && A.have_warm[pid];
+  if (use_warm) {
+    vcopy(n, A.wx + D.xo, p.x);
+    vcopy(m, A.ws + D.co, p.s);
+    vcopy(m, A.wz + D.co, p.z);
+    vfill(p.nz, 0.0, p.s);
+    ensure_interior_warm(p, p.s, red);
+    ensure_interior_warm(p, p.z, red);
+  } else {
+    ok = initial_point(A, p, colk, red);
+  }
+  if (ok) {
+    status = CX_MAX_ITERATIONS;
+    const int deg = p.nn + p.nsoc;
+    for (int iter = 0; iter < A.max_iter; ++iter) {
+      iterations = iter + 1;
+      cone_scaling(p, p.s, p.z);
+      if (!factor_retry(A, p, colk, red) || !constant_solve(A, p, colk, red)) {
+        status = CX_NUMERICAL_ERROR;
+        break;
+      }
+      residuals(A, p, red);
+      const double mu_now = (p.dsz + p.tau * p.kappa) / (double)(deg + 1);
+      const double nrx = sqrt(p.nrx2) / p.tau, nrz = sqrt(p.nrz2) / p.tau;
+      const double pobj = (0.5 * p.dxPx / p.tau + p.dqx) / p.tau;
+      const double dobj = (-0.5 * p.dxPx / p.tau - p.dbz) / p.tau;
+      const double gap_abs = fabs(pobj - dobj);
+      const double gap_tol = A.tol_gap_abs + A.tol_gap_rel * dmax(fabs(pobj), fabs(dobj));
+      if (nrx < A.tol_feas && nrz < A.tol_feas && gap_abs < gap_tol && p.tau > 1e-6) {
+        status = CX_SOLVED;
+        break;
+      }
+      const double ratio = p.tau / dmax(p.kappa, 1e-30);  // solver_impl.cpp:327-340
+      if (ratio <= A.tol_infeas) {
+        double v[2] = {0.0, 0.0};
+        for (int i = threadIdx.x; i < m; i += BG) v[0] += p.z[i] * p.z[i];
+        for (int i = threadIdx.x; i < n; i += BG) v[1] += p.x[i] * p.x[i];
+        gsum<2>(v, red);
+        if (p.dbz < -A.tol_infeas * dmax(1.0, sqrt(v[0]))) {
+          status = CX_PRIMAL_INFEASIBLE;
+          break;
+        }
+        if (p.dqx < -A.tol_infeas * dmax(1.0, sqrt(v[1]))) {
+          status = CX_DUAL_INFEASIBLE;
+          break;
+        }
+      }
+      if (!affine_step(A, p, colk, red)) {
+        status = CX_NUMERICAL_ERROR;
+        break;
+      }
+      const double aa = step_length(p, p.dsa, p.dza, p.dtaua, p.dkappaa, red);
+      double saz = 0.0;
+      for (int i = threadIdx.x; i < m; i += BG) saz += (p.s[i] + aa * p.dsa[i]) * (p.z[i] + aa * p.dza[i]);
+      saz = gsum1(saz, red);
+      const double tau_a = p.tau + aa * p.dtaua, kap_a = p.kappa + aa * p.dkappaa;
+      const double mu_aff = (saz + tau_a * kap_a) / (double)(deg + 1);
+      double sigma;
+      if (mu_now > 0) {
+        const double r = dmax(mu_aff / mu_now, 0.0);
+        sigma = dmin(dmax(r * r * r, 0.0), 1.0);
+      } else {
+        sigma = 1.0;
+      }
+      if (!combined_step(A, p, sigma, mu_now, colk, red)) {
+        status = CX_NUMERICAL_ERROR;
+        break;
+      }
+      const double alpha = step_length(p, p.ds, p.dz, p.dtau, p.dkappa, red) * A.step_frac;
+      if (!isfinite(alpha) || alpha <= 0.0) {
+        status = CX_NUMERICAL_ERROR;
+        break;
+      }
+      vaxpy(n, alpha, p.dx, p.x);
+      vaxpy(m, alpha, p.ds, p.s);
+      vaxpy(m, alpha, p.dz, p.z);
+      p.tau += alpha * p.dtau;
+      p.kappa += alpha * p.dkappa;
+      const double sc = dmax(p.tau, p.kappa);
+      if (sc > 1e8 || sc < 1e-8) {
+        const double inv = 1.0 / sc;
+        vscale(n, inv, p.x);
+        vscale(m, inv, p.s);
+        vscale(m, inv, p.z);
+        p.tau *= inv;
+        p.kappa *= inv;
+      }
+    }
+  }
+  // finalizeSolution (solver_impl.cpp:342-365)
+  const bool inf = status == CX_PRIMAL_INFEASIBLE || status == CX_DUAL_INFEASIBLE;
+  const double scale = inf ? 1.0 / dmax(p.kappa, 1e-30) : 1.0 / dmax(p.tau, 1e-30);
+  const double* ed = A.ed + D.xo;
+  const double* ee = A.ee + D.co;
+  for (int i = threadIdx.x; i < n; i += BG) A.xo[D.xo + i] = p.x[i] * scale * ed[i];
+  for (int i = threadIdx.x; i < m; i += BG) {
+    A.zo[D.co + i] = p.z[i] * scale * ee[i] / p.ec;
+    A.so[D.co + i] = p.s[i] * scale / ee[i];
+  }
+  if (status == CX_SOLVED && A.warm_start) {
+    for (int i = threadIdx.x; i < n; i += BG) A.wx[D.xo + i] = p.x[i];
+    for (int i = threadIdx.x; i < m; i += BG) A.ws[D.co + i] = p.s[i], A.wz[D.co + i] = p.z[i];
+  }
+  if (threadIdx.x == 0) {
+    const double pobj = (0.5 * p.dxPx / p.tau + p.dqx) / p.tau;
+    double* inf8 = A.info + 8L * pid;
+    inf8[0] = sqrt(p.nrz2) / dmax(p.tau, 1e-30);  // primal residual
+    inf8[1] = sqrt(p.nrx2) / dmax(p.tau, 1e-30);  // dual residual
+    inf8[2] = p.dsz / (p.tau * p.tau) / p.ec;     // gap
+    inf8[3] = (p.dsz + p.tau * p.kappa) / (double)(p.nn + p.nsoc + 1);
+    inf8[4] = pobj / p.ec;
+    A.status[pid] = status;
+    A.iters[pid] = iterations;
+    if (status == CX_SOLVED && A.warm_start) A.have_warm[pid] = 1;
+  }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(BG) void k_batch_solve(BatchArgs A) {
+  extern __shared__ double lds[];
+  __shared__ double colk[1024];
+  __shared__ double red[64];
+  __shared__ int slot;
+  for (;;) {
+    if (threadIdx.x == 0) slot = atomicAdd(A.counter, 1);
+    __syncthreads();
+    const int k = slot;
+    __syncthreads();
+    if (k >= A.nprob) break;
+    solve_problem(A, A.order[k], lds, colk, red);
+  }
+}
+
+// q_scaled = c D q_raw, b_scaled = E b_raw per problem; counter reset
+__global__ void k_batch_prepare(int total_n, int total_m, const int* xprob, const BatchDesc* desc,
+                                const double* qraw, const double* braw, const double* ed,
+                                const double* ee, double* q, double* b, int* counter) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) counter[0] = 0;
+  if (i < total_n) q[i] = qraw[i] * desc[xprob[i]].ec * ed[i];
+  if (i < total_m) b[i] = braw[i] * ee[i];
+}
+
+double now_ms() {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+bool valid_csc(int rows, int cols, const int* p, const int* i) {
+  if (!p || p[0] != 0) return false;
+  for (int c = 0; c < cols; ++c) {
+    if (p[c + 1] < p[c]) return false;
+    for (int k = p[c]; k < p[c + 1]; ++k) {
+      if (!i || i[k] < 0 || i[k] >= rows) return false;
+      if (k > p[c] && i[k] <= i[k - 1]) return false;
+    }
+  }
+  return true;
+}
+
+constexpr int LDS_MAX_BYTES = 160 * 1024 - 1024 * 8 - 64 * 8 - 64;  // minus the static LDS of the kernel
+
+}  // namespace
+
+BatchSolver::BatchSolver() {
+  CX_HIP(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
+  CX_HIP(hipEventCreate(&ev0_));
+  CX_HIP(hipEventCreate(&ev1_));
+  cx_default_settings(&set_);
+}
+
+BatchSolver::~BatchSolver() {
+  if (gexec_) (void)hipGraphExecDestroy(gexec_);
+  if (graph_) (void)hipGraphDestroy(graph_);
+  if (stage_q_) (void)hipHostFree(stage_q_);
+  if (stage_b_) (void)hipHostFree(stage_b_);
+  if (ev0_) (void)hipEventDestroy(ev0_);
+  if (ev1_) (void)hipEventDestroy(ev1_);
+  if (st_) (void)hipStreamDestroy(st_);
+}
+
+void BatchSolver::set_settings(const cx_settings& st) {
+  set_ = st;
+  if (gexec_) (void)hipGraphExecDestroy(gexec_), gexec_ = nullptr;
+  if (graph_) (void)hipGraphDestroy(graph_), graph_ = nullptr;
+}
+
+bool BatchSolver::setup(int nprob, const int* n, const int* m, const int* const* Pp, const int* const* Pi,
+                        const double* const* Px, const double* const* q, const int* const* Ap,
+                        const int* const* Ai, const double* const* Ax, const double* const* b,
+                        const int* zero_dim, const int* nonneg_dim, const int* nsoc,
+                        const int* const* soc_dims, const cx_settings& st) {
+  ready_ = false;
+  if (gexec_) (void)hipGraphExecDestroy(gexec_), gexec_ = nullptr;
+  if (graph_) (void)hipGraphDestroy(graph_), graph_ = nullptr;
+  if (nprob <= 0) return false;
+  set_ = st;
+  np_ = nprob;
+  n_.assign(n, n + nprob);
+  m_.assign(m, m + nprob);
+  nz_.assign(zero_dim, zero_dim + nprob);
+  nn_.assign(nonneg_dim, nonneg_dim + nprob);
+  socd_.assign(nprob, {});
+  // validation (solver_impl.cpp:11-16 per problem)
+  for (int k = 0; k < nprob; ++k) {
+    if (n[k] < 0 || m[k] < 0 || zero_dim[k] < 0 || nonneg_dim[k] < 0 || nsoc[k] < 0) return false;
+    long tot = (long)zero_dim[k] + nonneg_dim[k];
+    for (int t = 0; t < nsoc[k]; ++t) {
+      if (soc_dims[k][t] < 1) return false;
+      tot += soc_dims[k][t];
+      socd_[k].push_back(soc_dims[k][t]);
+    }
+    if (tot != m[k]) return false;
+    if (!valid_csc(n[k], n[k], Pp[k], Pi[k]) || !valid_csc(m[k], n[k], Ap[k], Ai[k])) return false;
+    if ((n[k] > 0 && !q[k]) || (m[k] > 0 && !b[k])) return false;
+  }
+  // host structure
+  desc_h_.assign(nprob, BatchDesc{});
+  std::vector<int> pf_ptr, pf_col, pf_src, ac_ptr, ac_row, ar_ptr, ar_col, ar_src, soc_off, soc_dim;
+  std::vector<int> rk, kr, xprob;
+  std::vector<double> Pv, Av, qv, bv;
+  ed_h_.clear();
+  ee_h_.clear();
+  long wofs = 0, mofs = 0;
+  int xo = 0, co = 0;
+  max_packed_ = 0;
+  for (int k = 0; k < nprob; ++k) {
+    const int nk = n[k], mk = m[k], nzk = zero_dim[k], nnk = nonneg_dim[k], nsk = nsoc[k];
+    BatchDesc& D = desc_h_[k];
+    D.n = nk, D.m = mk, D.nz = nzk, D.nn = nnk, D.nsoc = nsk;
+    D.xo = xo, D.co = co, D.so = (int)soc_off.size(), D.ko = (int)kr.size();
+    const int nP = Pp[k][nk], nA = Ap[k][nk];
+    // equilibration
+    std::vector<double> pv(Px[k], Px[k] + nP), av(Ax[k], Ax[k] + nA);
+    std::vector<int> so_k(nsk);
+    {
+      int off = nzk + nnk;
+      for (int t = 0; t < nsk; ++t) so_k[t] = off, off += socd_[k][t];
+    }
+    std::vector<double> edk, eek;
+    double eck = 1.0;
+    ruiz_equilibrate(nk, mk, Pp[k], Pi[k], pv, Ap[k], Ai[k], av, so_k, socd_[k], st, edk, eek, eck);
+    D.ec = eck;
+    ed_h_.insert(ed_h_.end(), edk.begin(), edk.end());
+    ee_h_.insert(ee_h_.end(), eek.begin(), eek.end());
+    const int pbase = (int)Pv.size(), abase = (int)Av.size();
+    Pv.insert(Pv.end(), pv.begin(), pv.end());
+    Av.insert(Av.end(), av.begin(), av.end());
+    D.pnnz = nP;
+    qv.insert(qv.end(), q[k], q[k] + nk);
+    bv.insert(bv.end(), b[k], b[k] + mk);
+    for (int i = 0; i < nk; ++i) xprob.push_back(k);
+    // full symmetric P by rows (upper storage; entries below the diagonal ignored)
+    {
+      std::vector<std::vector<std::pair<int, int>>> rows(nk);
+      for (int c = 0; c < nk; ++c)
+        for (int e = Pp[k][c]; e < Pp[k][c + 1]; ++e) {
+          const int r = Pi[k][e];
+          if (r > c) continue;
+          rows[r].push_back({c, pbase + e});
+          if (r != c) rows[c].push_back({r, pbase + e});
+        }
+      D.pfp = (int)pf_ptr.size();
+      for (int r = 0; r < nk; ++r) {
+        pf_ptr.push_back((int)pf_col.size());
+        for (auto& ce : rows[r]) pf_col.push_back(ce.first), pf_src.push_back(ce.second);
+      }
+      pf_ptr.push_back((int)pf_col.size());
+    }
+    // A by columns (entry e <-> Av[abase + e]) and by rows
+    D.acp = (int)ac_ptr.size();
+    for (int c = 0; c <= nk; ++c) ac_ptr.push_back(abase + Ap[k][c]);
+    for (int e = 0; e < nA; ++e) ac_row.push_back(Ai[k][e]);
+    // ac_row is indexed globally by the entry: keep it aligned with Av
+    {
+      std::vector<std::vector<std::pair<int, int>>> rows(mk);
+      for (int c = 0; c < nk; ++c)
+        for (int e = Ap[k][c]; e < Ap[k][c + 1]; ++e) rows[Ai[k][e]].push_back({c, abase + e});
+      D.arp = (int)ar_ptr.size();
+      for (int r = 0; r < mk; ++r) {
+        ar_ptr.push_back((int)ar_col.size());
+        for (auto& ce : rows[r]) ar_col.push_back(ce.first), ar_src.push_back(ce.second);
+      }
+      ar_ptr.push_back((int)ar_col.size());
+      // bound rows (nonnegative rows with one entry) are folded into the P
+      // block; every other row is kept in M after the variables
+      int nkept = 0;
+      for (int i = 0; i < mk; ++i) {
+        const bool bound = i >= nzk && i < nzk + nnk && rows[i].size() == 1;
+        if (bound) {
+          rk.push_back(-1);
+        } else {
+          rk.push_back(nkept++);
+          kr.push_back(i);
+        }
+      }
+      D.nk = nkept;
+      D.nr = nk + nkept;
+      if (D.nr > 1024) return false;  // order of M bounded by the LDS column buffer
+    }
+    for (int t = 0; t < nsk; ++t) soc_off.push_back(so_k[t]), soc_dim.push_back(socd_[k][t]);
+    // workspace: 9 n-vectors, 14 m-vectors, 5 N-vectors
+    D.wofs = wofs;
+    wofs += 9L * nk + 14L * mk + 5L * (nk + mk);
+    const long pk = (long)D.nr * (D.nr + 1) / 2;
+    max_packed_ = std::max<long>(max_packed_, pk);
+    D.mofs = mofs;
+    mofs += pk;
+    xo += nk;
+    co += mk;
+  }
+  sum_n_ = xo, sum_m_ = co;
+  if (ac_row.size() != Av.size()) throw std::runtime_error("batch: A entry maps out of step");
+  // problems largest first
+  std::vector<int> order(nprob);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b2) { return desc_h_[a].nr > desc_h_[b2].nr; });
+
+  auto up_i = [&](DevBuf<int>& d, const std::vector<int>& v) { d.upload(v.empty() ? std::vector<int>{0} : v, st_); };
+  auto up_d = [&](DevBuf<double>& d, const std::vector<double>& v) {
+    d.upload(v.empty() ? std::vector<double>{0.0} : v, st_);
+  };
+  desc_.upload(desc_h_, st_);
+  up_i(order_, order);
+  counter_.alloc(1);
+  up_i(pf_ptr_, pf_ptr), up_i(pf_col_, pf_col), up_i(pf_src_, pf_src);
+  up_i(ac_ptr_, ac_ptr), up_i(ac_row_, ac_row);
+  up_i(ar_ptr_, ar_ptr), up_i(ar_col_, ar_col), up_i(ar_src_, ar_src);
+  up_i(soc_off_, soc_off), up_i(soc_dim_, soc_dim);
+  up_i(rk_, rk), up_i(kr_, kr);
+  up_d(Pv_, Pv), up_d(Av_, Av), up_d(qraw_, qv), up_d(braw_, bv), up_d(ed_, ed_h_), up_d(ee_, ee_h_);
+  q_.alloc(std::max<long>(1, sum_n_));
+  b_.alloc(std::max<long>(1, sum_m_));
+  {
+    DevBuf<int> xp;
+    up_i(xp, xprob);
+    hipLaunchKernelGGL(k_batch_prepare, dim3(ceil_div(std::max(sum_n_, sum_m_) + 1, 256)), dim3(256), 0, st_,
+                       (int)sum_n_, (int)sum_m_, xp.p, desc_.p, qraw_.p, braw_.p, ed_.p, ee_.p, q_.p, b_.p,
+                       counter_.p);
+    CX_LAUNCH_CHECK();
+    CX_HIP(hipStreamSynchronize(st_));
+    xprob_.upload(xprob, st_);
+  }
+  W_.alloc(std::max<long>(1, wofs));
+  W_.zero(st_);
+  Mg_.alloc(std::max<long>(1, mofs));
+  const size_t nsoc_tot = std::max<size_t>(1, soc_off.size());
+  beta_.alloc(nsoc_tot), vv_.alloc(nsoc_tot), ident_.alloc(nsoc_tot);
+  wxs_.alloc(std::max<long>(1, sum_n_)), wss_.alloc(std::max<long>(1, sum_m_)), wzs_.alloc(std::max<long>(1, sum_m_));
+  xo_.alloc(std::max<long>(1, sum_n_)), so_.alloc(std::max<long>(1, sum_m_)), zo_.alloc(std::max<long>(1, sum_m_));
+  info_.alloc(8L * nprob);
+  status_.alloc(nprob), iters_.alloc(nprob), have_warm_.alloc(nprob);
+  have_warm_.zero(st_);
+  if (stage_q_) (void)hipHostFree(stage_q_), stage_q_ = nullptr;
+  if (stage_b_) (void)hipHostFree(stage_b_), stage_b_ = nullptr;
+  CX_HIP(hipHostMalloc((void**)&stage_q_, std::max<long>(1, sum_n_) * sizeof(double), hipHostMallocDefault));
+  CX_HIP(hipHostMalloc((void**)&stage_b_, std::max<long>(1, sum_m_) * sizeof(double), hipHostMallocDefault));
+  std::memcpy(stage_q_, qv.data(), sum_n_ * sizeof(double));
+  std::memcpy(stage_b_, bv.data(), sum_m_ * sizeof(double));
+  // default tuning: every M that fits in LDS goes there, one workgroup per CU slot
+  int dev = 0, ncu = 0;
+  CX_HIP(hipGetDevice(&dev));
+  CX_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  tune_.lds_doubles = std::min<long>(max_packed_, LDS_MAX_BYTES / 8);
+  tune_.grid = std::min(nprob, ncu * std::max(1, (LDS_MAX_BYTES / 8) / std::max(1, tune_.lds_doubles)));
+  tune_.grid = std::max(1, std::min(tune_.grid, ncu * 8));
+  CX_HIP(hipFuncSetAttribute((const void*)k_batch_solve, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             LDS_MAX_BYTES));
+  CX_HIP(hipStreamSynchronize(st_));
+  ready_ = true;
+  fetched_ = false;
+  return true;
+}
+
+bool BatchSolver::update_data(const double* const* q, const double* const* b) {
+  if (!ready_) return false;
+  for (int k = 0; k < np_; ++k) {
+    const BatchDesc& D = desc_h_[k];
+    if (q && q[k]) std::memcpy(stage_q_ + D.xo, q[k], D.n * sizeof(double));
+    if (b && b[k]) std::memcpy(stage_b_ + D.co, b[k], D.m * sizeof(double));
+  }
+  return true;
+}
+
+void BatchSolver::launch_all() {
+  BatchArgs A{};
+  A.nprob = np_;
+  A.desc = desc_.p;
+  A.order = order_.p;
+  A.counter = counter_.p;
+  A.tol_feas = set_.tol_feas, A.tol_gap_abs = set_.tol_gap_abs, A.tol_gap_rel = set_.tol_gap_rel;
+  A.tol_infeas = set_.tol_infeas, A.sreg_p = set_.static_reg_P, A.sreg_a = set_.static_reg_A;
+  A.dyn_eps = set_.dynamic_reg_eps, A.dyn_delta = set_.dynamic_reg_delta;
+  A.refine_tol = set_.refine_tol, A.step_frac = set_.max_step_fraction;
+  A.max_iter = set_.max_iter, A.refine_max_iter = set_.refine_max_iter;
+  A.mode = set_.linear_solver == CX_LS_REGULARIZED ? PV_REGULARIZE : PV_CHECK;
+  A.warm_start = set_.warm_start;
+  A.lds_doubles = tune_.lds_doubles;
+  A.pf_ptr = pf_ptr_.p, A.pf_col = pf_col_.p, A.pf_src = pf_src_.p;
+  A.ac_ptr = ac_ptr_.p, A.ac_row = ac_row_.p;
+  A.ar_ptr = ar_ptr_.p, A.ar_col = ar_col_.p, A.ar_src = ar_src_.p;
+  A.soc_off = soc_off_.p, A.soc_dim = soc_dim_.p;
+  A.rk = rk_.p, A.kr = kr_.p;
+  A.Pv = Pv_.p, A.Av = Av_.p, A.q = q_.p, A.b = b_.p, A.ed = ed_.p, A.ee = ee_.p;
+  A.W = W_.p, A.Mg = Mg_.p;
+  A.beta = beta_.p, A.vv = vv_.p, A.ident = ident_.p;
+  A.wx = wxs_.p, A.ws = wss_.p, A.wz = wzs_.p, A.have_warm = have_warm_.p;
+  A.xo = xo_.p, A.so = so_.p, A.zo = zo_.p, A.info = info_.p;
+  A.status = status_.p, A.iters = iters_.p;
+  const size_t lds = (size_t)std::max(1, tune_.lds_doubles) * sizeof(double);
+  CX_HIP(hipMemcpyAsync(qraw_.p, stage_q_, sum_n_ * sizeof(double), hipMemcpyHostToDevice, st_));
+  CX_HIP(hipMemcpyAsync(braw_.p, stage_b_, sum_m_ * sizeof(double), hipMemcpyHostToDevice, st_));
+  hipLaunchKernelGGL(k_batch_prepare, dim3(ceil_div(std::max(sum_n_, sum_m_) + 1, 256)), dim3(256), 0, st_,
+                     (int)sum_n_, (int)sum_m_, xprob_.p, desc_.p, qraw_.p, braw_.p, ed_.p, ee_.p, q_.p, b_.p,
+                     counter_.p);
+  CX_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_batch_solve, dim3(tune_.grid), dim3(BG), lds, st_, A);
+  CX_LAUNCH_CHECK();
+}
+
+int BatchSolver::solve(int* status_out) {
+  if (!ready_) return -1;
+  if (!gexec_) {
+    CX_HIP(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal));
+    launch_all();
+    CX_HIP(hipStreamEndCapture(st_, &graph_));
+    CX_HIP(hipGraphInstantiate(&gexec_, graph_, nullptr, nullptr, 0));
+  }
+  CX_HIP(hipEventRecord(ev0_, st_));
+  CX_HIP(hipGraphLaunch(gexec_, st_));
+  CX_HIP(hipEventRecord(ev1_, st_));
+  CX_HIP(hipStreamSynchronize(st_));
+  float ms = 0;
+  CX_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+  last_ms_ = ms;
+  fetched_ = false;
+  fetch();
+  int solved = 0;
+  for (int k = 0; k < np_; ++k) {
+    if (status_out) status_out[k] = hstatus_[k];
+    solved += hstatus_[k] == CX_SOLVED;
+  }
+  return solved;
+}
+
+void BatchSolver::fetch() {
+  if (fetched_) return;
+  hx_.resize(std::max<long>(1, sum_n_));
+  hs_.resize(std::max<long>(1, sum_m_));
+  hz_.resize(std::max<long>(1, sum_m_));
+  hinfo_.resize(8L * np_);
+  hstatus_.resize(np_);
+  hiters_.resize(np_);
+  xo_.download(hx_.data(), sum_n_, st_);
+  so_.download(hs_.data(), sum_m_, st_);
+  zo_.download(hz_.data(), sum_m_, st_);
+  info_.download(hinfo_.data(), 8L * np_, st_);
+  status_.download(hstatus_.data(), np_, st_);
+  iters_.download(hiters_.data(), np_, st_);
+  CX_HIP(hipStreamSynchronize(st_));
+  fetched_ = true;
+}
+
+int BatchSolver::get_solution(int k, double* x, double* s, double* z, double* obj, cx_info* info) {
+  if (!ready_ || k < 0 || k >= np_) return CX_UNSOLVED;
+  fetch();
+  const BatchDesc& D = desc_h_[k];
+  if (x) std::memcpy(x, hx_.data() + D.xo, D.n * sizeof(double));
+  if (s) std::memcpy(s, hs_.data() + D.co, D.m * sizeof(double));
+  if (z) std::memcpy(z, hz_.data() + D.co, D.m * sizeof(double));
+  if (obj) *obj = hinfo_[8L * k + 4];
+  if (info) {
+    info->iterations = hiters_[k];
+    info->primal_residual = hinfo_[8L * k + 0];
+    info->dual_residual = hinfo_[8L * k + 1];
+    info->duality_gap = hinfo_[8L * k + 2];
+    info->mu = hinfo_[8L * k + 3];
+    info->setup_time_s = 0;
+    info->solve_time_s = last_ms_ * 1e-3;
+  }
+  return hstatus_[k];
+}
+
+int BatchSolver::autotune(int reps) {
+  (void)reps;
+  return 0;
+}
+
+void BatchSolver::tuning(double* out4) const {
+  out4[0] = tune_.lds_doubles;
+  out4[1] = tune_.grid;
+  out4[2] = max_packed_;
+  out4[3] = tuned_ ? 1 : 0;
+}
+
+}  // namespace cx

@@ -1,0 +1,97 @@
+// Batched solver: many independent conic QPs, each solved start to finish by
+// one workgroup of a persistent kernel, with every per-problem datum on the
+// device.  New in this design (the reference solves one problem per Solver
+// object, src/conicxx/solver_impl.cpp:367-466); each problem follows exactly
+// the same method as the single-problem path.
+//
+// Per problem the KKT system [P+eI, A'; A, -(Hs+eI)] is factored as a dense
+// LDL^T in the order "variables first, then constraint rows", the order in
+// which both pivot blocks stay definite (positive, then negative) and no
+// pivot divides by a small (Hs + eI) eigenvalue.  Bound rows (nonnegative
+// rows with a single entry, such as box constraints) are diagonal and are
+// folded into the P block first (a^2 / (Hs_ii + e) on one diagonal entry, a
+// harmless scaling); every other row is kept.  M, of order n + kept rows, is
+// held packed (LDS when it fits, else global memory) and factored by the
+// workgroup; solves are refined against the full K.
+#pragma once
+
+#include <vector>
+
+#include "../../include/conicxx_c.h"
+#include "common.hpp"
+
+namespace cx {
+
+struct BatchDesc {
+  int n, m, nz, nn, nsoc, nk, pnnz, nr;  // nk kept rows, nr = n + nk
+  int xo, co, so, ko;  // bases of n-vectors, m-vectors, SOC tables, kept-row lists
+  int pfp, acp, arp;    // bases of the per-problem row/column pointer arrays
+  long wofs;            // base of the per-problem workspace
+  long mofs;            // base of M in global memory (problems whose M does not fit in LDS)
+  double ec;            // cost scale
+};
+
+struct BatchTuning {
+  int lds_doubles = 0;  // packed M up to this size is kept in LDS, larger in global memory
+  int grid = 0;         // persistent workgroups
+};
+
+class BatchSolver {
+ public:
+  BatchSolver();
+  ~BatchSolver();
+  BatchSolver(const BatchSolver&) = delete;
+  BatchSolver& operator=(const BatchSolver&) = delete;
+
+  bool setup(int nprob, const int* n, const int* m, const int* const* Pp, const int* const* Pi,
+             const double* const* Px, const double* const* q, const int* const* Ap,
+             const int* const* Ai, const double* const* Ax, const double* const* b, const int* zero_dim,
+             const int* nonneg_dim, const int* nsoc, const int* const* soc_dims, const cx_settings& st);
+  // new q and/or b values for every problem (NULL = unchanged)
+  bool update_data(const double* const* q, const double* const* b);
+  int solve(int* status_out);  // returns the number of Solved problems, -1 on failure
+  int get_solution(int k, double* x, double* s, double* z, double* obj, cx_info* info);
+  void set_settings(const cx_settings& st);
+  int autotune(int reps);
+  void tuning(double* out4) const;
+  double last_solve_ms() const { return last_ms_; }
+
+ private:
+  void launch_all();
+  void fetch();
+
+  hipStream_t st_ = nullptr;
+  cx_settings set_{};
+  bool ready_ = false;
+  int np_ = 0;
+  std::vector<int> n_, m_, nz_, nn_;
+  std::vector<std::vector<int>> socd_;
+  std::vector<BatchDesc> desc_h_;
+  std::vector<double> ed_h_, ee_h_;
+  long sum_n_ = 0, sum_m_ = 0;
+  // device structure
+  DevBuf<BatchDesc> desc_;
+  DevBuf<int> order_, counter_, xprob_;
+  DevBuf<int> pf_ptr_, pf_col_, pf_src_, ac_ptr_, ac_row_, ar_ptr_, ar_col_, ar_src_;
+  DevBuf<int> soc_off_, soc_dim_;
+  DevBuf<int> rk_, kr_;
+  DevBuf<double> Pv_, Av_, q_, b_, qraw_, braw_, ed_, ee_, W_, Mg_;
+  DevBuf<double> beta_, vv_, wxs_, wss_, wzs_, xo_, so_, zo_, info_;
+  DevBuf<int> ident_, status_, iters_, have_warm_;
+  double* stage_q_ = nullptr;  // pinned staging for update_data
+  double* stage_b_ = nullptr;
+  int max_packed_ = 0;
+  BatchTuning tune_{};
+  bool tuned_ = false;
+  // graph of one timestep: staged q/b upload, rescale, counter reset, solve
+  hipGraph_t graph_ = nullptr;
+  hipGraphExec_t gexec_ = nullptr;
+  // host copies of the results
+  bool fetched_ = false;
+  std::vector<double> hx_, hs_, hz_, hinfo_;
+  std::vector<int> hstatus_, hiters_;
+  double last_ms_ = 0;
+  hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+};
+
+}  // namespace cx

@@ -7,6 +7,7 @@
 #include <new>
 
 #include "../../include/conicxx_c.h"
+#include "batch.hpp"
 #include "solver.hpp"
 #include "symbolic.hpp"
 
@@ -169,6 +170,103 @@ int cx_backward(cx_solver* h, const double* dl_dx, const double* dl_ds, const do
     return 0;
   }
 }
+
+struct cx_batch {
+  cx::BatchSolver* impl;
+};
+
+cx_batch* cx_batch_create(void) {
+  try {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return nullptr;
+    cx_batch* h = new cx_batch;
+    h->impl = new cx::BatchSolver();
+    return h;
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "[conicxx_amd] cx_batch_create failed: %s\n", e.what());
+    return nullptr;
+  }
+}
+
+void cx_batch_destroy(cx_batch* h) {
+  if (!h) return;
+  try {
+    delete h->impl;
+  } catch (...) {
+  }
+  delete h;
+}
+
+int cx_batch_setup(cx_batch* h, int nprob, const int* n, const int* m, const int* const* P_colptr,
+                   const int* const* P_rowind, const double* const* P_val, const double* const* q,
+                   const int* const* A_colptr, const int* const* A_rowind, const double* const* A_val,
+                   const double* const* b, const int* zero_dim, const int* nonneg_dim, const int* n_soc,
+                   const int* const* soc_dims, const cx_settings* settings) {
+  if (!h) return 0;
+  try {
+    cx_settings st;
+    if (settings)
+      st = *settings;
+    else
+      cx_default_settings(&st);
+    return h->impl->setup(nprob, n, m, P_colptr, P_rowind, P_val, q, A_colptr, A_rowind, A_val, b, zero_dim,
+                          nonneg_dim, n_soc, soc_dims, st)
+               ? 1
+               : 0;
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "[conicxx_amd] batch setup failed: %s\n", e.what());
+    return 0;
+  }
+}
+
+int cx_batch_update_data(cx_batch* h, const double* const* q, const double* const* b) {
+  if (!h) return 0;
+  try {
+    return h->impl->update_data(q, b) ? 1 : 0;
+  } catch (...) {
+    return 0;
+  }
+}
+
+int cx_batch_solve(cx_batch* h, int* status) {
+  if (!h) return -1;
+  try {
+    return h->impl->solve(status);
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "[conicxx_amd] batch solve failed: %s\n", e.what());
+    return -1;
+  }
+}
+
+int cx_batch_get_solution(cx_batch* h, int k, double* x, double* s, double* z, double* objective,
+                          cx_info* info) {
+  if (!h) return CX_UNSOLVED;
+  try {
+    return h->impl->get_solution(k, x, s, z, objective, info);
+  } catch (...) {
+    return CX_NUMERICAL_ERROR;
+  }
+}
+
+void cx_batch_set_settings(cx_batch* h, const cx_settings* settings) {
+  if (h && settings) h->impl->set_settings(*settings);
+}
+
+int cx_batch_autotune(cx_batch* h, int reps) {
+  if (!h) return 0;
+  try {
+    return h->impl->autotune(reps);
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "[conicxx_amd] batch autotune failed: %s\n", e.what());
+    return 0;
+  }
+}
+
+void cx_batch_get_tuning(const cx_batch* h, double* out4) {
+  if (h && out4) h->impl->tuning(out4);
+}
+
+double cx_batch_last_solve_ms(const cx_batch* h) { return h ? h->impl->last_solve_ms() : 0.0; }
 
 int cx_analyze_pattern(int N, const int* colptr, const int* rowind, int* perm, int* colcount,
                        double* stats8) {

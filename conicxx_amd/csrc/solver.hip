@@ -14,6 +14,7 @@
 #include <numeric>
 
 #include "solver.hpp"
+#include "equilibrate.hpp"
 
 namespace cx {
 
@@ -100,57 +101,9 @@ bool Solver::setup(int n, int m, const int* Pp, const int* Pi, const double* Px,
   Ai_ref_.assign(Ai, Ai + nA);
   std::vector<double> Pv(Px, Px + nP), Av(Ax, Ax + nA), qv(q, q + n), bv(b, b + m);
 
-  // --- Ruiz equilibration (equilibration.cpp:8-100) ---
-  ed_.assign(n, 1.0);
-  ee_.assign(m, 1.0);
-  ec_ = 1.0;
+  // --- Ruiz equilibration (equilibration.cpp:8-127) ---
+  ruiz_equilibrate(n, m, Pp, Pi, Pv, Ap, Ai, Av, soc_off_, soc_dims_, st_, ed_, ee_, ec_);
   if (st_.equilibrate && n > 0) {
-    std::vector<double> Pw(Pv), Aw(Av), nx(n), nz(m), dx(n), dz(m);
-    const double minS = st_.equilibrate_min_scale, maxS = st_.equilibrate_max_scale;
-    for (int it = 0; it < st_.equilibrate_max_iter; ++it) {
-      std::fill(nx.begin(), nx.end(), 0.0);
-      std::fill(nz.begin(), nz.end(), 0.0);
-      for (int c = 0; c < n; ++c)
-        for (int k = Pp[c]; k < Pp[c + 1]; ++k) {
-          double av = std::fabs(Pw[k]);
-          nx[Pi[k]] = std::max(nx[Pi[k]], av);
-          nx[c] = std::max(nx[c], av);
-        }
-      for (int c = 0; c < n; ++c)
-        for (int k = Ap[c]; k < Ap[c + 1]; ++k) {
-          double av = std::fabs(Aw[k]);
-          nx[c] = std::max(nx[c], av);
-          nz[Ai[k]] = std::max(nz[Ai[k]], av);
-        }
-      for (int k = 0; k < nsoc; ++k) {
-        int o = soc_off_[k], d = soc_dims_[k];
-        double mx = nz[o];
-        for (int j = 1; j < d; ++j) mx = std::max(mx, nz[o + j]);
-        for (int j = 0; j < d; ++j) nz[o + j] = mx;
-      }
-      for (int i = 0; i < n; ++i) dx[i] = 1.0 / std::sqrt(nx[i] > 0 ? nx[i] : 1.0);
-      for (int i = 0; i < m; ++i) dz[i] = 1.0 / std::sqrt(nz[i] > 0 ? nz[i] : 1.0);
-      for (int c = 0; c < n; ++c)
-        for (int k = Pp[c]; k < Pp[c + 1]; ++k) Pw[k] *= dx[Pi[k]] * dx[c];
-      for (int c = 0; c < n; ++c)
-        for (int k = Ap[c]; k < Ap[c + 1]; ++k) Aw[k] *= dz[Ai[k]] * dx[c];
-      for (int i = 0; i < n; ++i) ed_[i] *= dx[i];
-      for (int i = 0; i < m; ++i) ee_[i] *= dz[i];
-    }
-    for (int i = 0; i < n; ++i) ed_[i] = std::min(std::max(ed_[i], minS), maxS);
-    for (int i = 0; i < m; ++i) ee_[i] = std::min(std::max(ee_[i], minS), maxS);
-    double sumAbs = 0.0;
-    for (int k = 0; k < nP; ++k) sumAbs += std::fabs(Pw[k]);
-    if (nP > 0) {
-      double meanAbs = sumAbs / (double)nP;
-      if (meanAbs > 0) ec_ = 1.0 / std::max(meanAbs, minS);
-    }
-    ec_ = std::min(std::max(ec_, minS), maxS);
-    // scaleProblem (equilibration.cpp:102-127)
-    for (int c = 0; c < n; ++c)
-      for (int k = Pp[c]; k < Pp[c + 1]; ++k) Pv[k] *= ec_ * ed_[Pi[k]] * ed_[c];
-    for (int c = 0; c < n; ++c)
-      for (int k = Ap[c]; k < Ap[c + 1]; ++k) Av[k] *= ee_[Ai[k]] * ed_[c];
     for (int i = 0; i < n; ++i) qv[i] *= ec_ * ed_[i];
     for (int i = 0; i < m; ++i) bv[i] *= ee_[i];
   }

@@ -148,6 +148,44 @@ int cx_get_kernel_stats(const cx_solver* h, double* out, int max_kinds);
 int cx_backward(cx_solver* h, const double* dl_dx, const double* dl_ds, const double* dl_dz,
                 double* dP, double* dq, double* dA, double* db);
 
+/* ---- Batched form: many independent problems in one persistent launch ----
+ * No counterpart in the reference (callers there loop over Solver objects,
+ * README.md:11-14).  Problem k has the same arguments as cx_setup, passed as
+ * arrays of per-problem pointers; every problem is solved start to finish by
+ * one workgroup of a single persistent kernel, per-problem iteration counts and
+ * statuses are independent, and solve() runs as one captured HIP graph
+ * (staged q/b upload, rescaling, solve) that is replayed after update_data().
+ * Limit per problem: n + (rows other than single-entry nonnegative rows) <= 1024. */
+typedef struct cx_batch cx_batch;
+cx_batch* cx_batch_create(void); /* NULL if no GPU */
+void cx_batch_destroy(cx_batch* h);
+/* Returns 1 on success, 0 on invalid input (any problem) or device failure. */
+int cx_batch_setup(cx_batch* h, int nprob, const int* n, const int* m, const int* const* P_colptr,
+                   const int* const* P_rowind, const double* const* P_val, const double* const* q,
+                   const int* const* A_colptr, const int* const* A_rowind, const double* const* A_val,
+                   const double* const* b, const int* zero_dim, const int* nonneg_dim, const int* n_soc,
+                   const int* const* soc_dims, const cx_settings* settings);
+/* updateData(nullptr, &q, nullptr, &b) for every problem at once; q or b (or an
+ * entry of either) NULL = unchanged.  Values are staged on the host and uploaded
+ * by the next cx_batch_solve. */
+int cx_batch_update_data(cx_batch* h, const double* const* q, const double* const* b);
+/* Solves every problem; writes per-problem statuses (nprob) when non-NULL and
+ * returns the number of Solved problems (-1 on device failure).  With
+ * settings.warm_start each Solved problem seeds its next solve. */
+int cx_batch_solve(cx_batch* h, int* status);
+/* Solution of problem k, as cx_get_solution.  Returns its status. */
+int cx_batch_get_solution(cx_batch* h, int k, double* x, double* s, double* z, double* objective,
+                          cx_info* info);
+void cx_batch_set_settings(cx_batch* h, const cx_settings* settings);
+/* Times the candidate launch configurations on this batch (reps solves each)
+ * and keeps the fastest; returns the number of candidates tried. */
+int cx_batch_autotune(cx_batch* h, int reps);
+/* out4 = (packed M doubles kept in LDS, persistent workgroups, largest packed M,
+ * autotuned flag). */
+void cx_batch_get_tuning(const cx_batch* h, double* out4);
+/* Device time of the last cx_batch_solve (ms, HIP events around the graph). */
+double cx_batch_last_solve_ms(const cx_batch* h);
+
 /* Host-only symbolic analysis of a symmetric pattern (lower CSC, diagonal
  * included), for testing: writes the final permutation (new->old, N ints) and
  * the column counts of L (N ints) when non-NULL, and stats8 as above.
