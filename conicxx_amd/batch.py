@@ -113,5 +113,13 @@ class BatchSolver:
         _lib.lib().cx_batch_get_tuning(self._h, out.ctypes.data_as(_lib._f64p))
         return dict(lds_doubles=int(out[0]), grid=int(out[1]), max_packed=int(out[2]), autotuned=bool(out[3]))
 
+    def stats(self, k):
+        """Iterations, factorizations, reduced KKT solves (incl. refinement) and the
+        device time (us) of problem k in the last solve."""
+        out = np.zeros(8)
+        _lib.lib().cx_batch_get_stats(self._h, k, out.ctypes.data_as(_lib._f64p))
+        return dict(iterations=int(out[0]), factorizations=int(out[1]), solves=int(out[2]), us=float(out[3]),
+                    us_factor=float(out[4]), us_solve=float(out[5]), us_residual=float(out[6]))
+
     def last_solve_ms(self):
         return float(_lib.lib().cx_batch_last_solve_ms(self._h))

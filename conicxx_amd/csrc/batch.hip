@@ -103,6 +103,10 @@ struct PV {
   int n, m, nz, nn, nsoc, nk, N, nr, pnnz;
   int co, so, ko, pfp, acp, arp;
   double cur_a;  // (2,2) regularization of the current factorization
+  int nfact, nsolve;  // factorizations and reduced solves of this problem
+  long long t_fact, t_solve, t_res;  // device clock ticks in factorization, reduced solves, residuals
+  double* stage;      // LDS staging for panels when M is in global memory (else null)
+  long stage_cap;
   double ec;
   const double *q, *b;
   double *x, *Px, *rx, *x1, *Px1, *dxa, *dx, *wn, *wn2;
@@ -319,43 +323,141 @@ __device__ void assemble_M(const BatchArgs& A, const PV& p, double dp, double da
   __syncthreads();
 }
 
-// LDL^T of M in place (packed lower); pivot rule of the reference's backends.
-// Returns acceptance as in try_factor; colk: LDS scratch of nr doubles.
-__device__ bool factor_M(const BatchArgs& A, const PV& p, double* colk, double* red) {
+__device__ inline double bcast(double v, int l) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                          __builtin_amdgcn_readlane(__double2loint(v), l));
+}
+__device__ inline double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+// orders this wave's earlier memory writes before its later reads (M may be in
+// LDS or in global memory)
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+}
+
+constexpr int PB = 32;  // panel width of the blocked factorization
+constexpr int JC = 64;  // trailing columns staged per chunk
+
+// Blocked right-looking LDL^T of M (packed lower).  Within each 32-column
+// panel the columns are factored left-looking (per column: one dot product per
+// row over the panel's earlier columns, then the pivot), in LDS: when M itself
+// is in global memory the panel is first staged into the LDS region `stage`
+// (row-major, stride 33) and written back after.  The trailing matrix is then
+// updated in 64-column chunks: W = D * L(chunk rows, panel) is staged in LDS
+// and every thread owns a row, holding its 32 panel entries in registers and
+// updating 8 entries per batch of independent loads.  Pivot rule of the
+// reference's backends.  Returns acceptance as in try_factor.
+constexpr int PS = PB + 1;  // row stride of a staged panel
+
+__device__ bool factor_M(const BatchArgs& A, const PV& p, double* wbuf, double* pk, double* stage,
+                         long stage_cap, double* red) {
   const int nr = p.nr;
   double* M = p.M;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   int zero_piv = 0, nreg = 0;
-  for (int k = 0; k < nr; ++k) {
-    const long ck = colbase(k, nr);
-    if (tid == 0) {
-      double d = M[ck];
-      const double sg = k < p.n ? 1.0 : -1.0;
+  for (int k0 = 0; k0 < nr; k0 += PB) {
+    const int kb = min(PB, nr - k0);
+    const bool staged = stage != nullptr && (long)(nr - k0) * PS <= stage_cap;
+    // L(i, k0 + c) for i >= k0 + c (address arithmetic: an indexed local
+    // array would be placed in scratch memory)
+    auto L = [&](int i, int c) -> double& {
+      return staged ? stage[(long)(i - k0) * PS + c] : M[colbase(k0 + c, nr) + (i - k0 - c)];
+    };
+    if (staged) {
+      for (int c = 0; c < kb; ++c) {
+        const long cb = colbase(k0 + c, nr);
+        for (int i = k0 + c + tid; i < nr; i += BG) stage[(long)(i - k0) * PS + c] = M[cb + (i - k0 - c)];
+      }
+      __syncthreads();
+    }
+    // panel columns left-looking: column c takes the updates of panel columns
+    // 0..c-1 as one dot product per row (independent loads; pivots from
+    // pk[0..PB)), the updated diagonal goes through pk[PB]; then every thread
+    // applies the pivot rule to the same value and scales its rows.  Two
+    // barriers per column.
+    for (int c = 0; c < kb; ++c) {
+      const int col = k0 + c;
+      double d;
+      if (c > 0) {
+        for (int i = col + tid; i < nr; i += BG) {
+          double acc = 0.0;
+#pragma unroll 8
+          for (int q = 0; q < c; ++q) acc += L(i, q) * (L(col, q) * pk[q]);
+          const double v = L(i, c) - acc;
+          L(i, c) = v;
+          if (i == col) pk[PB] = v;
+        }
+        __syncthreads();
+        d = pk[PB];
+      } else {
+        d = L(col, 0);
+        __syncthreads();
+      }
+      const double sg = col < p.n ? 1.0 : -1.0;
+      bool reg = false;
       if (A.mode == PV_REGULARIZE) {
         if (!(d * sg >= A.dyn_eps)) {  // regularized_ldlt.cpp:132-139
           d += sg * A.dyn_delta;
           if (d * sg < A.dyn_eps) d = sg * A.dyn_delta;
-          ++nreg;
+          reg = true;
         }
-      } else if (d == 0.0) {
-        zero_piv = 1;
       }
-      M[ck] = d;
+      const double dinv = 1.0 / d;
+      for (int i = col + 1 + tid; i < nr; i += BG) L(i, c) *= dinv;
+      if (tid == 0) {
+        nreg += reg ? 1 : 0;
+        if (A.mode != PV_REGULARIZE && d == 0.0) zero_piv = 1;
+        L(col, c) = d;
+        pk[c] = d;
+      }
+      __syncthreads();
     }
-    __syncthreads();
-    const double dinv = 1.0 / M[ck];
-    for (int i = k + 1 + tid; i < nr; i += BG) {
-      const double c = M[ck + (i - k)];
-      colk[i] = c;
-      M[ck + (i - k)] = c * dinv;
+    if (staged) {
+      for (int c = 0; c < kb; ++c) {
+        const long cb = colbase(k0 + c, nr);
+        for (int i = k0 + c + tid; i < nr; i += BG) M[cb + (i - k0 - c)] = stage[(long)(i - k0) * PS + c];
+      }
+      __syncthreads();
     }
-    __syncthreads();
-    for (int j = k + 1 + wv; j < nr; j += 4) {
-      const long cj = colbase(j, nr);
-      const double cjv = colk[j];
-      for (int i = j + lane; i < nr; i += 64) M[cj + (i - j)] -= M[ck + (i - k)] * cjv;
+    for (int jc0 = k0 + kb; jc0 < nr; jc0 += JC) {
+      const int jn = min(JC, nr - jc0);
+      for (int e = tid; e < JC * PB; e += BG) {
+        const int jj = e / PB, c = e % PB;
+        double w = 0.0;
+        if (jj < jn && c < kb) w = L(jc0 + jj, c) * L(k0 + c, c);
+        wbuf[e] = w;
+      }
+      __syncthreads();
+      for (int i = jc0 + tid; i < nr; i += BG) {
+        double lr[PB];
+#pragma unroll
+        for (int c = 0; c < PB; ++c) lr[c] = L(i, c < kb ? c : 0) * (c < kb ? 1.0 : 0.0);
+        const int jmax = min(jn, i - jc0 + 1);
+        for (int j0 = 0; j0 < jmax; j0 += 8) {
+          long adr[8];
+          double old[8];
+#pragma unroll
+          for (int q = 0; q < 8; ++q) {
+            const int jj = j0 + q < jmax ? j0 + q : j0;
+            adr[q] = colbase(jc0 + jj, nr) + (i - jc0 - jj);
+            old[q] = M[adr[q]];
+          }
+#pragma unroll
+          for (int q = 0; q < 8; ++q) {
+            if (j0 + q >= jmax) continue;
+            const double* w = wbuf + (j0 + q) * PB;
+            double acc = 0.0;
+#pragma unroll
+            for (int c = 0; c < PB; ++c) acc += lr[c] * w[c];
+            M[adr[q]] = old[q] - acc;
+          }
+        }
+      }
+      __syncthreads();
     }
-    __syncthreads();
   }
   double mind = INFINITY, nanf = 0.0;
   for (int k = tid; k < nr; k += BG) {
@@ -375,8 +477,12 @@ __device__ bool factor_M(const BatchArgs& A, const PV& p, double* colk, double* 
 
 __device__ bool try_factor(const BatchArgs& A, PV& p, double ep, double ea, double* colk, double* red) {
   p.cur_a = A.sreg_a + ea;
+  ++p.nfact;
+  const long long t0 = wall_clock64();
   assemble_M(A, p, A.sreg_p + ep, p.cur_a);
-  return factor_M(A, p, colk, red);
+  const bool ok = factor_M(A, p, colk, colk + PB * JC, p.stage, p.stage_cap, red);
+  p.t_fact += wall_clock64() - t0;
+  return ok;
 }
 
 // factorizeWithRetry (kkt_system.cpp:242-264)
@@ -395,7 +501,14 @@ __device__ bool factor_retry(const BatchArgs& A, PV& p, double* colk, double* re
 
 // x = K^-1 rhs through the factored M (no refinement).  Bound rows:
 // y_i = (a_i x_j - r2_i) / (hs_i + da) after the solve of M.
-__device__ void reduced_solve(const BatchArgs& A, const PV& p, const double* rhs, double* x) {
+__device__ void reduced_solve_impl(const BatchArgs& A, PV& p, const double* rhs, double* x);
+__device__ void reduced_solve(const BatchArgs& A, PV& p, const double* rhs, double* x) {
+  ++p.nsolve;
+  const long long t0 = wall_clock64();
+  reduced_solve_impl(A, p, rhs, x);
+  p.t_solve += wall_clock64() - t0;
+}
+__device__ void reduced_solve_impl(const BatchArgs& A, PV& p, const double* rhs, double* x) {
   const int n = p.n, nr = p.nr, tid = threadIdx.x;
   const double* M = p.M;
   const double da = p.cur_a;
@@ -410,19 +523,76 @@ __device__ void reduced_solve(const BatchArgs& A, const PV& p, const double* rhs
   }
   for (int kk = tid; kk < p.nk; kk += BG) t[n + kk] = rhs[n + A.kr[p.ko + kk]];
   __syncthreads();
-  // L y = t
-  for (int k = 0; k < nr - 1; ++k) {
-    const long ck = colbase(k, nr);
-    const double yk = t[k];
-    for (int i = k + 1 + tid; i < nr; i += BG) t[i] -= M[ck + (i - k)] * yk;
+  const int lane = tid & 63, wv = tid >> 6;
+  // L y = t in 64-row blocks.  Wave 0 loads its diagonal block into registers
+  // (one row per lane, independent loads) and runs the dependent 64-step
+  // recurrence on registers only; then every thread updates one row below the
+  // block with a fully unrolled, clamped-address dot product.
+  for (int k0 = 0; k0 < nr; k0 += 64) {
+    const int kb = min(64, nr - k0);
+    if (wv == 0) {
+      double Lr[64];
+#pragma unroll
+      for (int c = 0; c < 64; ++c) {
+        const bool ok = c < lane && lane < kb;
+        const double v = M[ok ? colbase(k0 + c, nr) + (lane - c) : 0];
+        Lr[c] = ok ? v : 0.0;
+      }
+      double tl = lane < kb ? t[k0 + lane] : 0.0;
+#pragma unroll
+      for (int c = 0; c < 64; ++c) tl -= Lr[c] * bcast(tl, c);
+      if (lane < kb) t[k0 + lane] = tl;
+    }
+    __syncthreads();
+    for (int i = k0 + kb + tid; i < nr; i += BG) {
+      double acc = 0.0;
+#pragma unroll 16
+      for (int c = 0; c < 64; ++c) {
+        const bool ok = c < kb;
+        const double lv = M[ok ? colbase(k0 + c, nr) + (i - k0 - c) : 0];
+        const double yv = t[ok ? k0 + c : 0];
+        acc += ok ? lv * yv : 0.0;
+      }
+      t[i] -= acc;
+    }
     __syncthreads();
   }
   for (int k = tid; k < nr; k += BG) t[k] /= M[colbase(k, nr)];
   __syncthreads();
-  // L' x = y
-  for (int k = nr - 1; k > 0; --k) {
-    const double xk = t[k];
-    for (int j = tid; j < k; j += BG) t[j] -= M[colbase(j, nr) + (k - j)] * xk;
+  // L' x = y, blocks from the last: column dot products with the rows below
+  // (one wave per column), then wave 0 solves the diagonal block on registers
+  for (int k0 = ((nr - 1) / 64) * 64; k0 >= 0; k0 -= 64) {
+    const int kb = min(64, nr - k0);
+    for (int c = wv; c < kb; c += 4) {
+      const long cb = colbase(k0 + c, nr);
+      double acc = 0.0;
+      for (int i0 = k0 + kb; i0 < nr; i0 += 256) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int i = i0 + 64 * u + lane;
+          const bool ok = i < nr;
+          const double lv = M[ok ? cb + (i - k0 - c) : cb];
+          const double xv = t[ok ? i : 0];
+          acc += ok ? lv * xv : 0.0;
+        }
+      }
+      acc = wave_sum(acc);
+      if (lane == 0) t[k0 + c] -= acc;
+    }
+    __syncthreads();
+    if (wv == 0) {
+      double Lc[64];  // Lc[c] = L(k0 + c, k0 + lane) for c > lane
+#pragma unroll
+      for (int c = 0; c < 64; ++c) {
+        const bool ok = c > lane && c < kb;
+        const double v = M[ok ? colbase(k0 + lane, nr) + (c - lane) : 0];
+        Lc[c] = ok ? v : 0.0;
+      }
+      double tl = lane < kb ? t[k0 + lane] : 0.0;
+#pragma unroll
+      for (int c = 63; c > 0; --c) tl -= Lc[c] * bcast(tl, c);
+      if (lane < kb) t[k0 + lane] = tl;
+    }
     __syncthreads();
   }
   for (int j = tid; j < n; j += BG) x[j] = t[j];
@@ -436,7 +606,8 @@ __device__ void reduced_solve(const BatchArgs& A, const PV& p, const double* rhs
 }
 
 // res = rhs - K x with K the statically regularized KKT matrix (the refinement target)
-__device__ void kkt_residual(const BatchArgs& A, const PV& p, const double* rhs, const double* x, double* res) {
+__device__ void kkt_residual(const BatchArgs& A, PV& p, const double* rhs, const double* x, double* res) {
+  const long long t0 = wall_clock64();
   const int n = p.n;
   pmul(A, p, x, p.wn);
   atmul(A, p, x + n, p.wn2);
@@ -446,6 +617,7 @@ __device__ void kkt_residual(const BatchArgs& A, const PV& p, const double* rhs,
   for (int j = threadIdx.x; j < n; j += BG) res[j] = rhs[j] - p.wn[j] - A.sreg_p * x[j] - p.wn2[j];
   for (int i = threadIdx.x; i < p.m; i += BG) res[n + i] = rhs[n + i] - p.wm[i] - p.wm2[i] + A.sreg_a * x[n + i];
   __syncthreads();
+  p.t_res += wall_clock64() - t0;
 }
 
 // escalateAndResolve (kkt_system.cpp:266-294)
@@ -664,17 +836,43 @@ __device__ double step_length(const PV& p, const double* dsv, const double* dzv,
 // one problem, start to finish
 // ---------------------------------------------------------------------------
 __device__ void solve_problem(const BatchArgs& A, int pid, double* lds, double* colk, double* red) {
+  const long long t_start = wall_clock64();
   const BatchDesc D = A.desc[pid];
   PV p;
   p.n = D.n, p.m = D.m, p.nz = D.nz, p.nn = D.nn, p.nsoc = D.nsoc, p.nk = D.nk;
   p.N = D.n + D.m, p.nr = D.nr, p.pnnz = D.pnnz;
   p.co = D.co, p.so = D.so, p.ko = D.ko, p.pfp = D.pfp, p.acp = D.acp, p.arp = D.arp;
   p.cur_a = 0.0;
+  p.nfact = p.nsolve = 0;
+  p.t_fact = p.t_solve = p.t_res = 0;
   p.ec = D.ec;
   p.q = A.q + D.xo;
   p.b = A.b + D.co;
   const int n = D.n, m = D.m, N = n + m;
-  double* w = A.W + D.wofs;
+  // LDS placement: the vectors (touched by every step) before the factor M
+  // (touched once per factorization and triangular solve); what does not fit
+  // lives in this problem's global workspace
+  const long V = 14L * n + 19L * m, PM = packed_size(D.nr), cap = A.lds_doubles;
+  double* w;
+  p.stage = nullptr;
+  p.stage_cap = 0;
+  if (V + PM <= cap) {
+    p.M = lds;
+    w = lds + PM;
+  } else if (V + (long)D.nr * PS <= cap) {  // vectors in LDS and room to stage every panel
+    w = lds;
+    p.M = A.Mg + D.mofs;
+    p.stage = lds + V;
+    p.stage_cap = cap - V;
+  } else if (PM <= cap) {
+    p.M = lds;
+    w = A.W + D.wofs;
+  } else {
+    p.M = A.Mg + D.mofs;
+    w = A.W + D.wofs;
+    p.stage = lds;
+    p.stage_cap = cap;
+  }
   p.x = w; w += n;
   p.Px = w; w += n;
   p.rx = w; w += n;
@@ -703,7 +901,6 @@ __device__ void solve_problem(const BatchArgs& A, int pid, double* lds, double* 
   p.res = w; w += N;
   p.tmp = w; w += N;
   p.dsol = w; w += N;
-  p.M = (packed_size(D.nr) <= A.lds_doubles) ? lds : A.Mg + D.mofs;
   p.C.zero_off = 0, p.C.zero_dim = D.nz;
   p.C.nn_off = D.nz, p.C.nn_dim = D.nn;
   p.C.nsoc = D.nsoc;
@@ -826,12 +1023,18 @@ __device__ void solve_problem(const BatchArgs& A, int pid, double* lds, double* 
   }
   if (threadIdx.x == 0) {
     const double pobj = (0.5 * p.dxPx / p.tau + p.dqx) / p.tau;
-    double* inf8 = A.info + 8L * pid;
+    double* inf8 = A.info + 16L * pid;
     inf8[0] = sqrt(p.nrz2) / dmax(p.tau, 1e-30);  // primal residual
     inf8[1] = sqrt(p.nrx2) / dmax(p.tau, 1e-30);  // dual residual
     inf8[2] = p.dsz / (p.tau * p.tau) / p.ec;     // gap
     inf8[3] = (p.dsz + p.tau * p.kappa) / (double)(p.nn + p.nsoc + 1);
     inf8[4] = pobj / p.ec;
+    inf8[5] = p.nfact;
+    inf8[6] = p.nsolve;
+    inf8[7] = (double)(wall_clock64() - t_start);  // constant-rate device clock ticks
+    inf8[8] = (double)p.t_fact;
+    inf8[9] = (double)p.t_solve;
+    inf8[10] = (double)p.t_res;
     A.status[pid] = status;
     A.iters[pid] = iterations;
     if (status == CX_SOLVED && A.warm_start) A.have_warm[pid] = 1;
@@ -841,7 +1044,7 @@ __device__ void solve_problem(const BatchArgs& A, int pid, double* lds, double* 
 
 __global__ __launch_bounds__(BG) void k_batch_solve(BatchArgs A) {
   extern __shared__ double lds[];
-  __shared__ double colk[1024];
+  __shared__ double colk[PB * JC + PB + 8];  // staged W of the trailing update + panel pivots
   __shared__ double red[64];
   __shared__ int slot;
   for (;;) {
@@ -880,7 +1083,7 @@ bool valid_csc(int rows, int cols, const int* p, const int* i) {
   return true;
 }
 
-constexpr int LDS_MAX_BYTES = 160 * 1024 - 1024 * 8 - 64 * 8 - 64;  // minus the static LDS of the kernel
+constexpr int LDS_MAX_BYTES = 160 * 1024 - (PB * JC + PB + 8) * 8 - 64 * 8 - 64;  // minus the static LDS of the kernel
 
 }  // namespace
 
@@ -946,6 +1149,7 @@ bool BatchSolver::setup(int nprob, const int* n, const int* m, const int* const*
   long wofs = 0, mofs = 0;
   int xo = 0, co = 0;
   max_packed_ = 0;
+  max_lds_need_ = 0;
   for (int k = 0; k < nprob; ++k) {
     const int nk = n[k], mk = m[k], nzk = zero_dim[k], nnk = nonneg_dim[k], nsk = nsoc[k];
     BatchDesc& D = desc_h_[k];
@@ -1018,7 +1222,6 @@ bool BatchSolver::setup(int nprob, const int* n, const int* m, const int* const*
       }
       D.nk = nkept;
       D.nr = nk + nkept;
-      if (D.nr > 1024) return false;  // order of M bounded by the LDS column buffer
     }
     for (int t = 0; t < nsk; ++t) soc_off.push_back(so_k[t]), soc_dim.push_back(socd_[k][t]);
     // workspace: 9 n-vectors, 14 m-vectors, 5 N-vectors
@@ -1026,6 +1229,7 @@ bool BatchSolver::setup(int nprob, const int* n, const int* m, const int* const*
     wofs += 9L * nk + 14L * mk + 5L * (nk + mk);
     const long pk = (long)D.nr * (D.nr + 1) / 2;
     max_packed_ = std::max<long>(max_packed_, pk);
+    max_lds_need_ = std::max<long>(max_lds_need_, pk + 14L * nk + 19L * mk);
     D.mofs = mofs;
     mofs += pk;
     xo += nk;
@@ -1070,7 +1274,7 @@ bool BatchSolver::setup(int nprob, const int* n, const int* m, const int* const*
   beta_.alloc(nsoc_tot), vv_.alloc(nsoc_tot), ident_.alloc(nsoc_tot);
   wxs_.alloc(std::max<long>(1, sum_n_)), wss_.alloc(std::max<long>(1, sum_m_)), wzs_.alloc(std::max<long>(1, sum_m_));
   xo_.alloc(std::max<long>(1, sum_n_)), so_.alloc(std::max<long>(1, sum_m_)), zo_.alloc(std::max<long>(1, sum_m_));
-  info_.alloc(8L * nprob);
+  info_.alloc(16L * nprob);
   status_.alloc(nprob), iters_.alloc(nprob), have_warm_.alloc(nprob);
   have_warm_.zero(st_);
   if (stage_q_) (void)hipHostFree(stage_q_), stage_q_ = nullptr;
@@ -1083,9 +1287,14 @@ bool BatchSolver::setup(int nprob, const int* n, const int* m, const int* const*
   int dev = 0, ncu = 0;
   CX_HIP(hipGetDevice(&dev));
   CX_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-  tune_.lds_doubles = std::min<long>(max_packed_, LDS_MAX_BYTES / 8);
+  tune_.lds_doubles = std::min<long>(max_lds_need_, LDS_MAX_BYTES / 8);
   tune_.grid = std::min(nprob, ncu * std::max(1, (LDS_MAX_BYTES / 8) / std::max(1, tune_.lds_doubles)));
   tune_.grid = std::max(1, std::min(tune_.grid, ncu * 8));
+  {
+    int khz = 0;
+    CX_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+    clock_khz_ = khz > 0 ? khz : 100000;
+  }
   CX_HIP(hipFuncSetAttribute((const void*)k_batch_solve, hipFuncAttributeMaxDynamicSharedMemorySize,
                              LDS_MAX_BYTES));
   CX_HIP(hipStreamSynchronize(st_));
@@ -1170,13 +1379,13 @@ void BatchSolver::fetch() {
   hx_.resize(std::max<long>(1, sum_n_));
   hs_.resize(std::max<long>(1, sum_m_));
   hz_.resize(std::max<long>(1, sum_m_));
-  hinfo_.resize(8L * np_);
+  hinfo_.resize(16L * np_);
   hstatus_.resize(np_);
   hiters_.resize(np_);
   xo_.download(hx_.data(), sum_n_, st_);
   so_.download(hs_.data(), sum_m_, st_);
   zo_.download(hz_.data(), sum_m_, st_);
-  info_.download(hinfo_.data(), 8L * np_, st_);
+  info_.download(hinfo_.data(), 16L * np_, st_);
   status_.download(hstatus_.data(), np_, st_);
   iters_.download(hiters_.data(), np_, st_);
   CX_HIP(hipStreamSynchronize(st_));
@@ -1190,13 +1399,13 @@ int BatchSolver::get_solution(int k, double* x, double* s, double* z, double* ob
   if (x) std::memcpy(x, hx_.data() + D.xo, D.n * sizeof(double));
   if (s) std::memcpy(s, hs_.data() + D.co, D.m * sizeof(double));
   if (z) std::memcpy(z, hz_.data() + D.co, D.m * sizeof(double));
-  if (obj) *obj = hinfo_[8L * k + 4];
+  if (obj) *obj = hinfo_[16L * k + 4];
   if (info) {
     info->iterations = hiters_[k];
-    info->primal_residual = hinfo_[8L * k + 0];
-    info->dual_residual = hinfo_[8L * k + 1];
-    info->duality_gap = hinfo_[8L * k + 2];
-    info->mu = hinfo_[8L * k + 3];
+    info->primal_residual = hinfo_[16L * k + 0];
+    info->dual_residual = hinfo_[16L * k + 1];
+    info->duality_gap = hinfo_[16L * k + 2];
+    info->mu = hinfo_[16L * k + 3];
     info->setup_time_s = 0;
     info->solve_time_s = last_ms_ * 1e-3;
   }
@@ -1206,6 +1415,19 @@ int BatchSolver::get_solution(int k, double* x, double* s, double* z, double* ob
 int BatchSolver::autotune(int reps) {
   (void)reps;
   return 0;
+}
+
+void BatchSolver::stats(int k, double* out4) {
+  for (int i = 0; i < 8; ++i) out4[i] = 0.0;
+  if (!ready_ || k < 0 || k >= np_) return;
+  fetch();
+  out4[0] = hiters_[k];
+  out4[1] = hinfo_[16L * k + 5];
+  out4[2] = hinfo_[16L * k + 6];
+  out4[3] = hinfo_[16L * k + 7] / (clock_khz_ * 1e-3);  // microseconds
+  out4[4] = hinfo_[16L * k + 8] / (clock_khz_ * 1e-3);
+  out4[5] = hinfo_[16L * k + 9] / (clock_khz_ * 1e-3);
+  out4[6] = hinfo_[16L * k + 10] / (clock_khz_ * 1e-3);
 }
 
 void BatchSolver::tuning(double* out4) const {

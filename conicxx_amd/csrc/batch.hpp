@@ -54,6 +54,7 @@ class BatchSolver {
   void set_settings(const cx_settings& st);
   int autotune(int reps);
   void tuning(double* out4) const;
+  void stats(int k, double* out4);
   double last_solve_ms() const { return last_ms_; }
 
  private:
@@ -81,6 +82,7 @@ class BatchSolver {
   double* stage_q_ = nullptr;  // pinned staging for update_data
   double* stage_b_ = nullptr;
   int max_packed_ = 0;
+  long max_lds_need_ = 0;
   BatchTuning tune_{};
   bool tuned_ = false;
   // graph of one timestep: staged q/b upload, rescale, counter reset, solve
@@ -91,6 +93,7 @@ class BatchSolver {
   std::vector<double> hx_, hs_, hz_, hinfo_;
   std::vector<int> hstatus_, hiters_;
   double last_ms_ = 0;
+  double clock_khz_ = 100000;
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
 };
 

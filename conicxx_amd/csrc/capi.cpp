@@ -266,6 +266,14 @@ void cx_batch_get_tuning(const cx_batch* h, double* out4) {
   if (h && out4) h->impl->tuning(out4);
 }
 
+void cx_batch_get_stats(cx_batch* h, int k, double* out4) {
+  if (!h || !out4) return;
+  try {
+    h->impl->stats(k, out4);
+  } catch (...) {
+  }
+}
+
 double cx_batch_last_solve_ms(const cx_batch* h) { return h ? h->impl->last_solve_ms() : 0.0; }
 
 int cx_analyze_pattern(int N, const int* colptr, const int* rowind, int* perm, int* colcount,

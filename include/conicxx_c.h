@@ -155,7 +155,7 @@ int cx_backward(cx_solver* h, const double* dl_dx, const double* dl_ds, const do
  * one workgroup of a single persistent kernel, per-problem iteration counts and
  * statuses are independent, and solve() runs as one captured HIP graph
  * (staged q/b upload, rescaling, solve) that is replayed after update_data().
- * Limit per problem: n + (rows other than single-entry nonnegative rows) <= 1024. */
+ * The factor of a problem lives in LDS when it fits, otherwise in global memory. */
 typedef struct cx_batch cx_batch;
 cx_batch* cx_batch_create(void); /* NULL if no GPU */
 void cx_batch_destroy(cx_batch* h);
@@ -183,6 +183,11 @@ int cx_batch_autotune(cx_batch* h, int reps);
 /* out4 = (packed M doubles kept in LDS, persistent workgroups, largest packed M,
  * autotuned flag). */
 void cx_batch_get_tuning(const cx_batch* h, double* out4);
+/* Work counters of problem k in the last solve: out8 = (iterations,
+ * factorizations, reduced KKT solves incl. refinement, device microseconds its
+ * workgroup spent on it, of which in factorizations, in triangular solves, in
+ * refinement residuals, 0). */
+void cx_batch_get_stats(cx_batch* h, int k, double* out8);
 /* Device time of the last cx_batch_solve (ms, HIP events around the graph). */
 double cx_batch_last_solve_ms(const cx_batch* h);
 
