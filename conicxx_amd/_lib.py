@@ -89,6 +89,9 @@ SIGNATURES = [
     ("cx_batch_autotune", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("cx_batch_get_tuning", None, [ctypes.c_void_p, _f64p]),
     ("cx_batch_get_stats", None, [ctypes.c_void_p, ctypes.c_int, _f64p]),
+    ("cx_batch_set_differentiable", None, [ctypes.c_void_p, ctypes.c_int]),
+    ("cx_batch_backward", ctypes.c_int, [ctypes.c_void_p] + [ctypes.POINTER(_f64p)] * 7),
+    ("cx_batch_last_backward_ms", ctypes.c_double, [ctypes.c_void_p]),
     ("cx_batch_last_solve_ms", ctypes.c_double, [ctypes.c_void_p]),
     ("cx_analyze_pattern", ctypes.c_int, [ctypes.c_int, _i32p, _i32p, _i32p, _i32p, _f64p]),
 ]

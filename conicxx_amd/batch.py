@@ -23,6 +23,47 @@ class BatchSolver:
             _lib.lib().cx_batch_destroy(h)
             self._h = None
 
+    def set_differentiable(self, on=True):
+        """Make every Solved problem's forward pass also factor its adjoint matrix."""
+        _lib.lib().cx_batch_set_differentiable(self._h, 1 if on else 0)
+
+    def backward(self, dl_dx=None, dl_ds=None, dl_dz=None):
+        """Gradients for every problem (lists of per-problem arrays, None = zero).
+        Returns a list of dict(P=values on P's pattern, q, A=values on A's pattern, b)."""
+        K = len(self._dims)
+        keep = []
+
+        def inp(vals, idx):
+            if vals is None:
+                return None
+            out = (_lib._f64p * K)()
+            for k, v in enumerate(vals):
+                if v is None:
+                    out[k] = _lib.NULL_F
+                    continue
+                a, p = _lib.f64(np.asarray(v, dtype=np.float64).reshape(-1))
+                if a.size != self._dims[k][idx]:
+                    raise ValueError(f"problem {k}: wrong length")
+                keep.append(a)
+                out[k] = p
+            return out
+        res = [dict(P=np.zeros(self._nnz[k][0]), q=np.zeros(self._dims[k][0]), A=np.zeros(self._nnz[k][1]),
+                    b=np.zeros(self._dims[k][1])) for k in range(K)]
+
+        def outp(key):
+            arr = (_lib._f64p * K)()
+            for k in range(K):
+                arr[k] = res[k][key].ctypes.data_as(_lib._f64p) if res[k][key].size else _lib.NULL_F
+            return arr
+        r = _lib.lib().cx_batch_backward(self._h, inp(dl_dx, 0), inp(dl_ds, 1), inp(dl_dz, 1), outp("P"),
+                                         outp("q"), outp("A"), outp("b"))
+        if r < 0:
+            raise RuntimeError("batch backward failed (differentiation off, or device failure)")
+        return res
+
+    def last_backward_ms(self):
+        return float(_lib.lib().cx_batch_last_backward_ms(self._h))
+
     def setup(self, problems, settings=None):
         """problems: sequence of (P, q, A, b, ConeSpec)."""
         settings = settings or Settings()
@@ -31,6 +72,7 @@ class BatchSolver:
         arrs = {k: [] for k in ("Pp", "Pi", "Px", "q", "Ap", "Ai", "Ax", "b", "sd")}
         n, m, zd, nd, ns = (np.zeros(K, np.int32) for _ in range(5))
         self._dims = []
+        self._nnz = []
         for k, (P, q, A, b, cone) in enumerate(problems):
             if not isinstance(cone, ConeSpec):
                 cone = ConeSpec(*cone)
@@ -41,6 +83,7 @@ class BatchSolver:
             n[k], m[k] = Pc.shape[0], Ac.shape[0]
             zd[k], nd[k], ns[k] = cone.zero_dim, cone.nonneg_dim, len(cone.soc_dims)
             self._dims.append((int(n[k]), int(m[k])))
+            self._nnz.append((int(Pc.nnz), int(Ac.nnz)))
             for key, a, conv in (("Pp", Pc.indptr, _lib.i32), ("Pi", Pc.indices, _lib.i32),
                                  ("Px", Pc.data, _lib.f64), ("q", q, _lib.f64),
                                  ("Ap", Ac.indptr, _lib.i32), ("Ai", Ac.indices, _lib.i32),

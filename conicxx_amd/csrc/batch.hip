@@ -60,6 +60,14 @@ struct BatchArgs {
   int* have_warm;
   double *xo, *so, *zo, *info;
   int *status, *iters;
+  // differentiation: prepare the adjoint factorization after a Solved problem
+  int diff;
+  double *tcorr, *tdir, *xh, *sh, *zh, *diff_a;
+  int* diff_ok;
+  // backward pass inputs and outputs (concatenated like the data)
+  const double *gwx, *gws, *gwz;
+  double *gP, *gq, *gA, *gb;
+  int* counter2;
 };
 
 // ---------------------------------------------------------------------------
@@ -107,6 +115,8 @@ struct PV {
   long long t_fact, t_solve, t_res;  // device clock ticks in factorization, reduced solves, residuals
   double* stage;      // LDS staging for panels when M is in global memory (else null)
   long stage_cap;
+  const double* tcorr;  // SOC tangent correction of the adjoint matrix (null in the IPM)
+  const double* tdir;
   double ec;
   const double *q, *b;
   double *x, *Px, *rx, *x1, *Px1, *dxa, *dx, *wn, *wn2;
@@ -316,6 +326,10 @@ __device__ void assemble_M(const BatchArgs& A, const PV& p, double dp, double da
         else
           h = be * be * (4.0 * vvk * v[a] * v[c] - 2.0 * v[a] * (jsign(c) * v[c]) -
                          2.0 * (jsign(a) * v[a]) * v[c] + (a == c ? 1.0 : 0.0));
+        if (p.tcorr && !id && a > 0 && c > 0) {
+          const double* u = p.tdir + off;
+          h += p.tcorr[k] * ((a == c ? 1.0 : 0.0) - u[a] * u[c]);
+        }
         M[cb + (a - c)] = -(h + (a == c ? da : 0.0));
       }
     }
@@ -605,6 +619,37 @@ __device__ void reduced_solve_impl(const BatchArgs& A, PV& p, const double* rhs,
   __syncthreads();
 }
 
+// SOC tangent correction of the adjoint system (the limit of the exact
+// linearization arw(z) ds + arw(s) dz = 0 at a complementary pair): on the
+// directions orthogonal to z1 the scaling eigenvalue beta^2 becomes s0/z0.
+// tcorr = s0/z0 - beta^2 and tdir = (0, z1/|z1|) per block (0 where |z1| = 0
+// or the scaling fell back to the identity).
+__device__ void soc_tangent(const PV& p, const double* sv, const double* zv, double* tc, double* td) {
+  for (int k = threadIdx.x; k < p.nsoc; k += BG) {
+    const int o = p.C.soc_off[k], d = p.C.soc_dim[k];
+    double zn = 0.0;
+    for (int j = 1; j < d; ++j) zn += zv[o + j] * zv[o + j];
+    zn = sqrt(zn);
+    const bool ok = zn > 0.0 && zv[o] > 0.0 && !p.C.ident[k];
+    td[o] = 0.0;
+    for (int j = 1; j < d; ++j) td[o + j] = ok ? zv[o + j] / zn : 0.0;
+    tc[k] = ok ? sv[o] / zv[o] - p.C.beta[k] * p.C.beta[k] : 0.0;
+  }
+  __syncthreads();
+}
+// out -= tcorr_k P_T x on every SOC block
+__device__ void soc_tangent_apply(const PV& p, const double* x, double* out) {
+  for (int k = threadIdx.x; k < p.nsoc; k += BG) {
+    const int o = p.C.soc_off[k], d = p.C.soc_dim[k];
+    const double c = p.tcorr[k];
+    if (c == 0.0) continue;
+    double ux = 0.0;
+    for (int j = 1; j < d; ++j) ux += p.tdir[o + j] * x[o + j];
+    for (int j = 1; j < d; ++j) out[o + j] -= c * (x[o + j] - p.tdir[o + j] * ux);
+  }
+  __syncthreads();
+}
+
 // res = rhs - K x with K the statically regularized KKT matrix (the refinement target)
 __device__ void kkt_residual(const BatchArgs& A, PV& p, const double* rhs, const double* x, double* res) {
   const long long t0 = wall_clock64();
@@ -614,6 +659,7 @@ __device__ void kkt_residual(const BatchArgs& A, PV& p, const double* rhs, const
   amul(A, p, x, p.wm);
   vfill(p.m, 0.0, p.wm2);
   neg_hs_plus(p, x + n, p.wm2, p.wm2);  // -Hs y (each entry read and written by one thread)
+  if (p.tcorr) soc_tangent_apply(p, x + n, p.wm2);
   for (int j = threadIdx.x; j < n; j += BG) res[j] = rhs[j] - p.wn[j] - A.sreg_p * x[j] - p.wn2[j];
   for (int i = threadIdx.x; i < p.m; i += BG) res[n + i] = rhs[n + i] - p.wm[i] - p.wm2[i] + A.sreg_a * x[n + i];
   __syncthreads();
@@ -835,10 +881,10 @@ __device__ double step_length(const PV& p, const double* dsv, const double* dzv,
 // ---------------------------------------------------------------------------
 // one problem, start to finish
 // ---------------------------------------------------------------------------
-__device__ void solve_problem(const BatchArgs& A, int pid, double* lds, double* colk, double* red) {
-  const long long t_start = wall_clock64();
-  const BatchDesc D = A.desc[pid];
-  PV p;
+// Per-problem view: sizes, bases, LDS/global placement of the factor and of
+// the vectors.  global_M forces the factor into this problem's global slot
+// (the backward pass reads the factor its forward pass left there).
+__device__ void setup_view(const BatchArgs& A, const BatchDesc& D, PV& p, double* lds, bool global_M) {
   p.n = D.n, p.m = D.m, p.nz = D.nz, p.nn = D.nn, p.nsoc = D.nsoc, p.nk = D.nk;
   p.N = D.n + D.m, p.nr = D.nr, p.pnnz = D.pnnz;
   p.co = D.co, p.so = D.so, p.ko = D.ko, p.pfp = D.pfp, p.acp = D.acp, p.arp = D.arp;
@@ -856,7 +902,7 @@ __device__ void solve_problem(const BatchArgs& A, int pid, double* lds, double* 
   double* w;
   p.stage = nullptr;
   p.stage_cap = 0;
-  if (V + PM <= cap) {
+  if (V + PM <= cap && !global_M) {
     p.M = lds;
     w = lds + PM;
   } else if (V + (long)D.nr * PS <= cap) {  // vectors in LDS and room to stage every panel
@@ -864,7 +910,7 @@ __device__ void solve_problem(const BatchArgs& A, int pid, double* lds, double* 
     p.M = A.Mg + D.mofs;
     p.stage = lds + V;
     p.stage_cap = cap - V;
-  } else if (PM <= cap) {
+  } else if (PM <= cap && !global_M) {
     p.M = lds;
     w = A.W + D.wofs;
   } else {
@@ -915,7 +961,17 @@ __device__ void solve_problem(const BatchArgs& A, int pid, double* lds, double* 
   p.last_p = p.last_a = 0.0;
   p.rtau = p.dqx = p.dbz = p.dsz = p.dxPx = p.nrx2 = p.nrz2 = 0.0;
   p.dtaua = p.dkappaa = p.dtau = p.dkappa = 0.0;
+  p.tcorr = nullptr;
+  p.tdir = nullptr;
+}
 
+
+__device__ void solve_problem(const BatchArgs& A, int pid, double* lds, double* colk, double* red) {
+  const long long t_start = wall_clock64();
+  const BatchDesc D = A.desc[pid];
+  PV p;
+  setup_view(A, D, p, lds, false);
+  const int n = D.n, m = D.m;
   int status = CX_NUMERICAL_ERROR, iterations = 0;
   bool ok = true;
   const bool use_warm = A.warm_start && A.have_warm[pid];
@@ -1021,7 +1077,32 @@ __device__ void solve_problem(const BatchArgs& A, int pid, double* lds, double* 
     for (int i = threadIdx.x; i < n; i += BG) A.wx[D.xo + i] = p.x[i];
     for (int i = threadIdx.x; i < m; i += BG) A.ws[D.co + i] = p.s[i], A.wz[D.co + i] = p.z[i];
   }
+  // differentiation: the adjoint matrix (final NT scaling, SOC tangent
+  // correction) is factored here, at the end of the forward pass, and left in
+  // the problem's global slot with the normalized optimum, so that the
+  // backward pass is one KKT solve
+  int dok = 0;
+  if (A.diff && status == CX_SOLVED) {
+    double* tc = A.tcorr + D.so;
+    double* td = A.tdir + D.co;
+    soc_tangent(p, p.s, p.z, tc, td);
+    p.tcorr = tc;
+    p.tdir = td;
+    dok = factor_retry(A, p, colk, red) ? 1 : 0;
+    double* Mg = A.Mg + D.mofs;
+    if (p.M != Mg) {
+      const long tot = packed_size(p.nr);
+      for (long e = threadIdx.x; e < tot; e += BG) Mg[e] = p.M[e];
+    }
+    const double it = 1.0 / dmax(p.tau, 1e-30);
+    for (int i = threadIdx.x; i < n; i += BG) A.xh[D.xo + i] = p.x[i] * it;
+    for (int i = threadIdx.x; i < m; i += BG) A.sh[D.co + i] = p.s[i] * it, A.zh[D.co + i] = p.z[i] * it;
+  }
   if (threadIdx.x == 0) {
+    if (A.diff) {
+      A.diff_ok[pid] = dok;
+      A.diff_a[pid] = p.cur_a;
+    }
     const double pobj = (0.5 * p.dxPx / p.tau + p.dqx) / p.tau;
     double* inf8 = A.info + 16L * pid;
     inf8[0] = sqrt(p.nrz2) / dmax(p.tau, 1e-30);  // primal residual
@@ -1054,6 +1135,76 @@ __global__ __launch_bounds__(BG) void k_batch_solve(BatchArgs A) {
     __syncthreads();
     if (k >= A.nprob) break;
     solve_problem(A, A.order[k], lds, colk, red);
+  }
+}
+
+// Backward pass of one problem whose forward pass left the adjoint factor in
+// its global slot: the cone scaling is recomputed from the stored optimum (it
+// is invariant under the common 1/tau scaling), then
+//   K [u; v] = [D wx; E wz / c - H (ws / E)]     (refined against the same K)
+//   dL/dq = -c D u,  dL/db = E v,  dL/dA_ij = -(z_i u_j + v_i x_j) E_i D_j,
+//   dL/dP_ij (upper) = -(u_i x_j + u_j x_i) c D_i D_j, diagonal -u_i x_i c D_i^2,
+// with x, z the normalized optimum in the equilibrated space.
+__device__ void backward_problem(const BatchArgs& A, int pid, double* lds, double* colk, double* red) {
+  const BatchDesc D = A.desc[pid];
+  PV p;
+  setup_view(A, D, p, lds, true);
+  const int n = D.n, m = D.m;
+  if (!A.diff_ok[pid]) {
+    for (int j = threadIdx.x; j < n; j += BG) A.gq[D.xo + j] = 0.0;
+    for (int i = threadIdx.x; i < m; i += BG) A.gb[D.co + i] = 0.0;
+    __syncthreads();
+    return;
+  }
+  const double* ed = A.ed + D.xo;
+  const double* ee = A.ee + D.co;
+  vcopy(n, A.xh + D.xo, p.x);
+  vcopy(m, A.sh + D.co, p.s);
+  vcopy(m, A.zh + D.co, p.z);
+  cone_scaling(p, p.s, p.z);
+  p.tcorr = A.tcorr + D.so;
+  p.tdir = A.tdir + D.co;
+  p.cur_a = A.diff_a[pid];
+  for (int j = threadIdx.x; j < n; j += BG) p.rhs[j] = A.gwx[D.xo + j] * ed[j];
+  for (int i = threadIdx.x; i < m; i += BG) {
+    p.wm[i] = A.gws[D.co + i] / ee[i];
+    p.wm2[i] = -A.gwz[D.co + i] * ee[i] / p.ec;
+  }
+  __syncthreads();
+  neg_hs_plus(p, p.wm, p.wm2, p.rhs + n);
+  soc_tangent_apply(p, p.wm, p.rhs + n);
+  kkt_solve(A, p, p.rhs, p.sol, colk, red);
+  const double* u = p.sol;
+  const double* v = p.sol + n;
+  for (int j = threadIdx.x; j < n; j += BG) {
+    A.gq[D.xo + j] = -p.ec * ed[j] * u[j];
+    for (int e = A.ac_ptr[p.acp + j]; e < A.ac_ptr[p.acp + j + 1]; ++e) {
+      const int i = A.ac_row[e];
+      A.gA[e] = -(p.z[i] * u[j] + v[i] * p.x[j]) * ee[i] * ed[j];
+    }
+    for (int e = A.pf_ptr[p.pfp + j]; e < A.pf_ptr[p.pfp + j + 1]; ++e) {
+      const int c = A.pf_col[e];
+      if (c < j) continue;  // each stored upper entry once, from its row
+      A.gP[A.pf_src[e]] = (c == j) ? -u[j] * p.x[j] * p.ec * ed[j] * ed[j]
+                                   : -(u[j] * p.x[c] + u[c] * p.x[j]) * p.ec * ed[j] * ed[c];
+    }
+  }
+  for (int i = threadIdx.x; i < m; i += BG) A.gb[D.co + i] = ee[i] * v[i];
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(BG) void k_batch_backward(BatchArgs A) {
+  extern __shared__ double lds[];
+  __shared__ double colk[PB * JC + PB + 8];
+  __shared__ double red[64];
+  __shared__ int slot;
+  for (;;) {
+    if (threadIdx.x == 0) slot = atomicAdd(A.counter2, 1);
+    __syncthreads();
+    const int k = slot;
+    __syncthreads();
+    if (k >= A.nprob) break;
+    backward_problem(A, A.order[k], lds, colk, red);
   }
 }
 
@@ -1150,6 +1301,9 @@ bool BatchSolver::setup(int nprob, const int* n, const int* m, const int* const*
   int xo = 0, co = 0;
   max_packed_ = 0;
   max_lds_need_ = 0;
+  pv_base_.clear();
+  av_base_.clear();
+  av_len_.clear();
   for (int k = 0; k < nprob; ++k) {
     const int nk = n[k], mk = m[k], nzk = zero_dim[k], nnk = nonneg_dim[k], nsk = nsoc[k];
     BatchDesc& D = desc_h_[k];
@@ -1170,6 +1324,9 @@ bool BatchSolver::setup(int nprob, const int* n, const int* m, const int* const*
     ed_h_.insert(ed_h_.end(), edk.begin(), edk.end());
     ee_h_.insert(ee_h_.end(), eek.begin(), eek.end());
     const int pbase = (int)Pv.size(), abase = (int)Av.size();
+    pv_base_.push_back(pbase);
+    av_base_.push_back(abase);
+    av_len_.push_back(nA);
     Pv.insert(Pv.end(), pv.begin(), pv.end());
     Av.insert(Av.end(), av.begin(), av.end());
     D.pnnz = nP;
@@ -1236,6 +1393,7 @@ bool BatchSolver::setup(int nprob, const int* n, const int* m, const int* const*
     co += mk;
   }
   sum_n_ = xo, sum_m_ = co;
+  nnzP_total_ = (long)Pv.size(), nnzA_total_ = (long)Av.size(), nsoc_total_ = soc_off.size();
   if (ac_row.size() != Av.size()) throw std::runtime_error("batch: A entry maps out of step");
   // problems largest first
   std::vector<int> order(nprob);
@@ -1248,7 +1406,7 @@ bool BatchSolver::setup(int nprob, const int* n, const int* m, const int* const*
   };
   desc_.upload(desc_h_, st_);
   up_i(order_, order);
-  counter_.alloc(1);
+  counter_.alloc(2);  // [0] forward, [1] backward work counters
   up_i(pf_ptr_, pf_ptr), up_i(pf_col_, pf_col), up_i(pf_src_, pf_src);
   up_i(ac_ptr_, ac_ptr), up_i(ac_row_, ac_row);
   up_i(ar_ptr_, ar_ptr), up_i(ar_col_, ar_col), up_i(ar_src_, ar_src);
@@ -1300,6 +1458,10 @@ bool BatchSolver::setup(int nprob, const int* n, const int* m, const int* const*
   CX_HIP(hipStreamSynchronize(st_));
   ready_ = true;
   fetched_ = false;
+  if (diff_) {
+    diff_ = false;
+    set_differentiable(true);
+  }
   return true;
 }
 
@@ -1313,8 +1475,8 @@ bool BatchSolver::update_data(const double* const* q, const double* const* b) {
   return true;
 }
 
-void BatchSolver::launch_all() {
-  BatchArgs A{};
+void BatchSolver::build_args(void* argp) {
+  BatchArgs& A = *static_cast<BatchArgs*>(argp);
   A.nprob = np_;
   A.desc = desc_.p;
   A.order = order_.p;
@@ -1338,6 +1500,12 @@ void BatchSolver::launch_all() {
   A.wx = wxs_.p, A.ws = wss_.p, A.wz = wzs_.p, A.have_warm = have_warm_.p;
   A.xo = xo_.p, A.so = so_.p, A.zo = zo_.p, A.info = info_.p;
   A.status = status_.p, A.iters = iters_.p;
+  fill_diff_args(&A);
+}
+
+void BatchSolver::launch_all() {
+  BatchArgs A{};
+  build_args(&A);
   const size_t lds = (size_t)std::max(1, tune_.lds_doubles) * sizeof(double);
   CX_HIP(hipMemcpyAsync(qraw_.p, stage_q_, sum_n_ * sizeof(double), hipMemcpyHostToDevice, st_));
   CX_HIP(hipMemcpyAsync(braw_.p, stage_b_, sum_m_ * sizeof(double), hipMemcpyHostToDevice, st_));
@@ -1347,6 +1515,88 @@ void BatchSolver::launch_all() {
   CX_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_batch_solve, dim3(tune_.grid), dim3(BG), lds, st_, A);
   CX_LAUNCH_CHECK();
+}
+
+void BatchSolver::fill_diff_args(void* argp) {
+  BatchArgs& A = *static_cast<BatchArgs*>(argp);
+  A.diff = diff_ ? 1 : 0;
+  A.tcorr = tcorr_.p, A.tdir = tdir_.p, A.xh = xh_.p, A.sh = sh_.p, A.zh = zh_.p;
+  A.diff_a = diff_a_.p, A.diff_ok = diff_ok_.p;
+  A.gwx = gwx_.p, A.gws = gws_.p, A.gwz = gwz_.p;
+  A.gP = gP_.p, A.gq = gq_.p, A.gA = gA_.p, A.gb = gb_.p;
+  A.counter2 = counter_.p + 1;
+}
+
+void BatchSolver::set_differentiable(bool on) {
+  if (on == diff_) return;
+  diff_ = on;
+  if (on && ready_) {
+    tcorr_.alloc(std::max<size_t>(1, nsoc_total_));
+    tdir_.alloc(std::max<long>(1, sum_m_));
+    xh_.alloc(std::max<long>(1, sum_n_));
+    sh_.alloc(std::max<long>(1, sum_m_));
+    zh_.alloc(std::max<long>(1, sum_m_));
+    diff_a_.alloc(np_);
+    diff_ok_.alloc(np_);
+    diff_ok_.zero(st_);
+    gwx_.alloc(std::max<long>(1, sum_n_));
+    gws_.alloc(std::max<long>(1, sum_m_));
+    gwz_.alloc(std::max<long>(1, sum_m_));
+    gP_.alloc(std::max<long>(1, nnzP_total_));
+    gA_.alloc(std::max<long>(1, nnzA_total_));
+    gq_.alloc(std::max<long>(1, sum_n_));
+    gb_.alloc(std::max<long>(1, sum_m_));
+    CX_HIP(hipStreamSynchronize(st_));
+  }
+  if (gexec_) (void)hipGraphExecDestroy(gexec_), gexec_ = nullptr;
+  if (graph_) (void)hipGraphDestroy(graph_), graph_ = nullptr;
+}
+
+int BatchSolver::backward(const double* const* wx, const double* const* ws, const double* const* wz,
+                          double* const* gP, double* const* gq, double* const* gA, double* const* gb) {
+  if (!ready_ || !diff_) return -1;
+  std::vector<double> hx(std::max<long>(1, sum_n_), 0.0), hs(std::max<long>(1, sum_m_), 0.0),
+      hz(std::max<long>(1, sum_m_), 0.0);
+  for (int k = 0; k < np_; ++k) {
+    const BatchDesc& D = desc_h_[k];
+    if (wx && wx[k]) std::memcpy(hx.data() + D.xo, wx[k], D.n * sizeof(double));
+    if (ws && ws[k]) std::memcpy(hs.data() + D.co, ws[k], D.m * sizeof(double));
+    if (wz && wz[k]) std::memcpy(hz.data() + D.co, wz[k], D.m * sizeof(double));
+  }
+  gwx_.upload(hx, st_);
+  gws_.upload(hs, st_);
+  gwz_.upload(hz, st_);
+  CX_HIP(hipMemsetAsync(gP_.p, 0, gP_.n * sizeof(double), st_));
+  CX_HIP(hipMemsetAsync(gA_.p, 0, gA_.n * sizeof(double), st_));
+  CX_HIP(hipMemsetAsync(counter_.p + 1, 0, sizeof(int), st_));
+  BatchArgs A{};
+  build_args(&A);
+  const size_t lds = (size_t)std::max(1, tune_.lds_doubles) * sizeof(double);
+  CX_HIP(hipEventRecord(ev0_, st_));
+  hipLaunchKernelGGL(k_batch_backward, dim3(tune_.grid), dim3(BG), lds, st_, A);
+  CX_LAUNCH_CHECK();
+  CX_HIP(hipEventRecord(ev1_, st_));
+  std::vector<double> hgP(gP_.n), hgA(gA_.n), hgq(gq_.n), hgb(gb_.n);
+  std::vector<int> ok(np_);
+  gP_.download(hgP.data(), gP_.n, st_);
+  gA_.download(hgA.data(), gA_.n, st_);
+  gq_.download(hgq.data(), gq_.n, st_);
+  gb_.download(hgb.data(), gb_.n, st_);
+  diff_ok_.download(ok.data(), np_, st_);
+  CX_HIP(hipStreamSynchronize(st_));
+  float ms = 0;
+  CX_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+  last_bw_ms_ = ms;
+  int nok = 0;
+  for (int k = 0; k < np_; ++k) {
+    const BatchDesc& D = desc_h_[k];
+    nok += ok[k] ? 1 : 0;
+    if (gP && gP[k]) std::memcpy(gP[k], hgP.data() + pv_base_[k], D.pnnz * sizeof(double));
+    if (gA && gA[k]) std::memcpy(gA[k], hgA.data() + av_base_[k], av_len_[k] * sizeof(double));
+    if (gq && gq[k]) std::memcpy(gq[k], hgq.data() + D.xo, D.n * sizeof(double));
+    if (gb && gb[k]) std::memcpy(gb[k], hgb.data() + D.co, D.m * sizeof(double));
+  }
+  return nok;
 }
 
 int BatchSolver::solve(int* status_out) {

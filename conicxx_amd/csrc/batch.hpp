@@ -53,12 +53,20 @@ class BatchSolver {
   int get_solution(int k, double* x, double* s, double* z, double* obj, cx_info* info);
   void set_settings(const cx_settings& st);
   int autotune(int reps);
+  // differentiation: when on, every Solved problem's forward pass also factors
+  // its adjoint KKT matrix; backward() is then one KKT solve per problem
+  void set_differentiable(bool on);
+  int backward(const double* const* wx, const double* const* ws, const double* const* wz,
+               double* const* gP, double* const* gq, double* const* gA, double* const* gb);
+  double last_backward_ms() const { return last_bw_ms_; }
   void tuning(double* out4) const;
   void stats(int k, double* out4);
   double last_solve_ms() const { return last_ms_; }
 
  private:
   void launch_all();
+  void build_args(void* args);
+  void fill_diff_args(void* args);
   void fetch();
 
   hipStream_t st_ = nullptr;
@@ -92,7 +100,14 @@ class BatchSolver {
   bool fetched_ = false;
   std::vector<double> hx_, hs_, hz_, hinfo_;
   std::vector<int> hstatus_, hiters_;
-  double last_ms_ = 0;
+  double last_ms_ = 0, last_bw_ms_ = 0;
+  // differentiation state
+  bool diff_ = false;
+  std::vector<long> pv_base_, av_base_, av_len_;
+  long nnzP_total_ = 0, nnzA_total_ = 0;
+  size_t nsoc_total_ = 0;
+  DevBuf<double> tcorr_, tdir_, xh_, sh_, zh_, diff_a_, gwx_, gws_, gwz_, gP_, gA_, gq_, gb_;
+  DevBuf<int> diff_ok_;
   double clock_khz_ = 100000;
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
 };

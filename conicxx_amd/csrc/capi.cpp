@@ -274,6 +274,29 @@ void cx_batch_get_stats(cx_batch* h, int k, double* out4) {
   }
 }
 
+void cx_batch_set_differentiable(cx_batch* h, int on) {
+  if (!h) return;
+  try {
+    h->impl->set_differentiable(on != 0);
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "[conicxx_amd] set_differentiable failed: %s\n", e.what());
+  }
+}
+
+int cx_batch_backward(cx_batch* h, const double* const* dl_dx, const double* const* dl_ds,
+                      const double* const* dl_dz, double* const* dP, double* const* dq, double* const* dA,
+                      double* const* db) {
+  if (!h) return -1;
+  try {
+    return h->impl->backward(dl_dx, dl_ds, dl_dz, dP, dq, dA, db);
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "[conicxx_amd] batch backward failed: %s\n", e.what());
+    return -1;
+  }
+}
+
+double cx_batch_last_backward_ms(const cx_batch* h) { return h ? h->impl->last_backward_ms() : 0.0; }
+
 double cx_batch_last_solve_ms(const cx_batch* h) { return h ? h->impl->last_solve_ms() : 0.0; }
 
 int cx_analyze_pattern(int N, const int* colptr, const int* rowind, int* perm, int* colcount,

@@ -188,6 +188,19 @@ void cx_batch_get_tuning(const cx_batch* h, double* out4);
  * workgroup spent on it, of which in factorizations, in triangular solves, in
  * refinement residuals, 0). */
 void cx_batch_get_stats(cx_batch* h, int k, double* out8);
+/* Differentiation of the batch (no counterpart in the reference).  With it on,
+ * every Solved problem's forward pass also factors its adjoint KKT matrix (the
+ * final scaling with the SOC tangent correction, as cx_backward uses), so that
+ * cx_batch_backward is one KKT solve plus the gradient assembly per problem.
+ * Gradient layout per problem as in cx_backward; any array (or entry) may be
+ * NULL.  Returns the number of problems with gradients (the Solved ones; the
+ * others get zeros), -1 if differentiation is off or on device failure. */
+void cx_batch_set_differentiable(cx_batch* h, int on);
+int cx_batch_backward(cx_batch* h, const double* const* dl_dx, const double* const* dl_ds,
+                      const double* const* dl_dz, double* const* dP, double* const* dq, double* const* dA,
+                      double* const* db);
+/* Device time of the last cx_batch_backward (ms, HIP events around the launch). */
+double cx_batch_last_backward_ms(const cx_batch* h);
 /* Device time of the last cx_batch_solve (ms, HIP events around the graph). */
 double cx_batch_last_solve_ms(const cx_batch* h);
 

@@ -132,3 +132,34 @@ def test_autograd_wrapper_batch():
         assert relerr(b.grad[k].numpy(), ref["b"]) < 5e-3
         assert relerr(Av.grad[k].numpy(), ref["A"]) < 5e-3
         assert relerr(Pv.grad[k].numpy(), ref["P"]) < 5e-3
+
+
+def test_batched_backward_matches_reference_and_single_problem_path():
+    """cx_batch_backward over a batch of differentiation problems: each problem's
+    gradients against the dense implicit-differentiation reference (at the
+    batch's own solution) and against the single-problem cx_backward."""
+    from conicxx_amd import BatchSolver
+    idx = [0, 4, 5, 7]
+    probs = [PR.config3_problem(i) for i in idx]
+    b = BatchSolver()
+    assert b.setup([(p.P, p.q, p.A, p.b, ConeSpec(p.zero_dim, p.nonneg_dim, p.soc_dims)) for p in probs], TIGHT)
+    b.set_differentiable(True)
+    st = b.solve()
+    grads = b.backward([p.meta["w"] for p in probs])
+    print("forward ms", b.last_solve_ms(), "backward ms", b.last_backward_ms())
+    for k, p in enumerate(probs):
+        sol = b.solution(k)
+        if sol.status != Status.Solved:
+            assert not grads[k]["q"].any()
+            continue
+        ref = implicit_gradients(p, sol.x, sol.s, sol.z, p.meta["w"])
+        errs = {key: relerr(grads[k][key], ref[key]) for key in ("P", "q", "A", "b")}
+        print(idx[k], {key: f"{v:.2e}" for key, v in errs.items()})
+        for key, e in errs.items():
+            assert e < 1e-3, (idx[k], key, e)
+        s1, sol1 = solved(p)
+        if sol1.status == Status.Solved:
+            g1 = s1.backward(p.meta["w"])
+            assert relerr(grads[k]["q"], g1["q"]) < 2e-3
+            assert relerr(grads[k]["b"], g1["b"]) < 2e-3
+    assert (st == int(Status.Solved)).sum() >= 3
