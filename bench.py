@@ -66,12 +66,105 @@ def cpu_baseline(full_flops, scale=0.1):
     }
 
 
+def batched_form(K=4096, timesteps=3, sample=32):
+    """The 4096-problem contact batch: update_data + solve per timestep on the
+    batched engine, against a per-problem loop of the plain GPU solve timed on a
+    sample of the same problems (solve() only, setup untimed)."""
+    from conicxx_amd import BatchSolver, ConeSpec, Settings, Solver, Status
+    from conicxx_amd import problems as PR
+
+    t0 = time.perf_counter()
+    probs = [PR.config2_problem(i) for i in range(K)]
+    gen_s = time.perf_counter() - t0
+    st = Settings(warm_start=False)
+    b = BatchSolver()
+    assert b.setup([(p.P, p.q, p.A, p.b, ConeSpec(p.zero_dim, p.nonneg_dim, p.soc_dims)) for p in probs], st)
+    b.solve()  # warm-up: graph capture
+    solved, wall, dev_ms = 0, 0.0, []
+    for t in range(1, timesteps + 1):
+        qs, bs = [], []
+        for i, p in enumerate(probs):
+            dq, db = PR.config2_perturbation(i, t, p.n, p.m)
+            qs.append(p.q + dq)
+            bs.append(p.b + db)
+        b.update_data(qs, bs)
+        t1 = time.perf_counter()
+        stt = b.solve()
+        wall += time.perf_counter() - t1
+        dev_ms.append(b.last_solve_ms())
+        solved += int((stt == int(Status.Solved)).sum())
+        print(f"[bench] batch timestep {t}: {int((stt == 1).sum())}/{K} Solved, {dev_ms[-1]:.0f} ms",
+              file=sys.stderr, flush=True)
+    qps = solved / wall
+    # per-problem loop of the plain GPU solver on a sample
+    loop_s, loop_solved = 0.0, 0
+    for i in range(sample):
+        p = probs[i]
+        s = Solver()
+        assert s.setup(p.P, p.q, p.A, p.b, ConeSpec(p.zero_dim, p.nonneg_dim, p.soc_dims), st)
+        t1 = time.perf_counter()
+        r = s.solve()
+        loop_s += time.perf_counter() - t1
+        loop_solved += r.status == Status.Solved
+    loop_qps = loop_solved / loop_s
+    return {
+        "workload": f"config2: {K} friction-contact QPs (c~U{{4..64}}, n=3c, 2n box rows + c SOC dim 3), "
+                    f"{timesteps} update_data()+solve() timesteps, cold-started",
+        "qps": qps, "unit": "QPs/s", "ms_per_timestep": 1e3 * wall / timesteps,
+        "device_ms_per_timestep": sum(dev_ms) / len(dev_ms),
+        "solved_fraction": solved / (K * timesteps),
+        "plain_loop_qps": loop_qps,
+        "plain_loop_sample": f"{sample} problems of the same batch, one plain Solver each, solve() timed",
+        "speedup_vs_plain_loop": qps / loop_qps, "tuning": b.tuning(), "generate_s": gen_s,
+    }
+
+
+def differentiation_form(K=1024):
+    """The 1024-problem differentiation batch: batched forward (with the adjoint
+    factorization) + batched backward of L = w'x; microseconds per problem."""
+    from conicxx_amd import BatchSolver, ConeSpec, Settings, Status
+    from conicxx_amd import problems as PR
+
+    probs = [PR.config3_problem(i) for i in range(K)]
+    st = Settings(warm_start=False)
+    b = BatchSolver()
+    assert b.setup([(p.P, p.q, p.A, p.b, ConeSpec(p.zero_dim, p.nonneg_dim, p.soc_dims)) for p in probs], st)
+    b.set_differentiable(True)
+    b.solve()  # warm-up
+    ws = [p.meta["w"] for p in probs]
+    b.backward(ws)
+    t0 = time.perf_counter()
+    stt = b.solve()
+    fwd_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    b.backward(ws)
+    bwd_s = time.perf_counter() - t0
+    # per problem, on its workgroup: the backward pass against one forward KKT
+    # solve (triangular solves + refinement residuals / KKT solves)
+    stats = [b.stats(k) for k in range(K)]
+    ok = [s for s in stats if s["us_backward"] > 0 and s["kkt_solves"] > 0]
+    bwd_us = sum(s["us_backward"] for s in ok) / max(1, len(ok))
+    kkt_us = sum((s["us_solve"] + s["us_residual"]) / s["kkt_solves"] for s in ok) / max(1, len(ok))
+    return {
+        "workload": f"config3: {K} QPs n=64, m=112 (zero 16, nonneg 64, 8 SOC dim 4), loss w'x",
+        "fwd_bwd_us_per_problem": 1e6 * (fwd_s + bwd_s) / K,
+        "fwd_us_per_problem": 1e6 * fwd_s / K, "bwd_us_per_problem": 1e6 * bwd_s / K,
+        "device_fwd_ms": b.last_solve_ms(), "device_bwd_ms": b.last_backward_ms(),
+        "solved_fraction": float((stt == int(Status.Solved)).mean()),
+        "workgroup_backward_us": bwd_us, "workgroup_forward_kkt_solve_us": kkt_us,
+        "backward_over_forward_kkt_solve": bwd_us / max(1e-9, kkt_us),
+        "note": "backward = one adjoint KKT solve (with refinement) per problem on the factor its forward "
+                "pass left, plus the gradient assembly; per-problem times from device clocks",
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-forms", action="store_true", help="skip the batched and differentiation forms")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the plain solve of one QP runs on a single device "
@@ -138,7 +231,7 @@ def main():
     trail_tflops = (trail["work"] / (trail["ms"] / 1e3) / 1e12) if trail and trail["ms"] > 0 else None
 
     out = {
-        "metric": "QPs solved/sec at 1e-8 rel gap (plain FP64 GPU solve)",
+        "metric": "QPs solved/sec at 1e-8 rel gap, batched vs plain GPU; fwd+bwd us per problem",
         "value": value if valid else 0.0,
         "unit": "QPs/s",
         "n_gpus": 1,
@@ -150,8 +243,10 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded generator, feasible by construction)",
-        "config": {"workload": "config1: one sparse conic QP, n=20000, m=28500 (zero 2000, nonneg 20000, "
-                               "2000 SOC dim 3, 10 SOC dim 50), cold-started solve per step",
+        "config": {"workload": "value: plain FP64 solve of config1, one sparse conic QP, n=20000, m=28500 "
+                               "(zero 2000, nonneg 20000, 2000 SOC dim 3, 10 SOC dim 50), cold-started solve "
+                               "per step; the batched (config2) and fwd+bwd (config3) figures are in "
+                               "'batched' and 'differentiation'",
                    "psd_repair": p.meta.get("psd_repair"), "kkt_N": kkt["N"], "nnz_L": kkt["nnz_L"],
                    "factor_flops": kkt["flops"], "largest_front": kkt["max_front"],
                    "iterations": iters, "all_solved": valid},
@@ -159,6 +254,9 @@ def main():
         "kernel_ms_per_step": kernel_ms,
         "trail_tflops": trail_tflops,
     }
+    if not args.no_forms:
+        out["batched"] = batched_form()
+        out["differentiation"] = differentiation_form()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(kkt["flops"])
     print(json.dumps(out))

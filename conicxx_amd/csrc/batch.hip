@@ -111,7 +111,7 @@ struct PV {
   int n, m, nz, nn, nsoc, nk, N, nr, pnnz;
   int co, so, ko, pfp, acp, arp;
   double cur_a;  // (2,2) regularization of the current factorization
-  int nfact, nsolve;  // factorizations and reduced solves of this problem
+  int nfact, nsolve, nkkt;  // factorizations, reduced solves and KKT solves (with refinement)
   long long t_fact, t_solve, t_res;  // device clock ticks in factorization, reduced solves, residuals
   double* stage;      // LDS staging for panels when M is in global memory (else null)
   long stage_cap;
@@ -684,6 +684,7 @@ __device__ bool escalate(const BatchArgs& A, PV& p, const double* rhs, double* x
 
 // KktSystem::solve (kkt_system.cpp:296-311)
 __device__ double kkt_solve(const BatchArgs& A, PV& p, const double* rhs, double* x, double* colk, double* red) {
+  ++p.nkkt;
   reduced_solve(A, p, rhs, x);
   const double nonfin = count_nonfinite(p.N, x, red);
   kkt_residual(A, p, rhs, x, p.res);
@@ -889,7 +890,7 @@ __device__ void setup_view(const BatchArgs& A, const BatchDesc& D, PV& p, double
   p.N = D.n + D.m, p.nr = D.nr, p.pnnz = D.pnnz;
   p.co = D.co, p.so = D.so, p.ko = D.ko, p.pfp = D.pfp, p.acp = D.acp, p.arp = D.arp;
   p.cur_a = 0.0;
-  p.nfact = p.nsolve = 0;
+  p.nfact = p.nsolve = p.nkkt = 0;
   p.t_fact = p.t_solve = p.t_res = 0;
   p.ec = D.ec;
   p.q = A.q + D.xo;
@@ -1116,6 +1117,7 @@ __device__ void solve_problem(const BatchArgs& A, int pid, double* lds, double* 
     inf8[8] = (double)p.t_fact;
     inf8[9] = (double)p.t_solve;
     inf8[10] = (double)p.t_res;
+    inf8[11] = p.nkkt;
     A.status[pid] = status;
     A.iters[pid] = iterations;
     if (status == CX_SOLVED && A.warm_start) A.have_warm[pid] = 1;
@@ -1146,10 +1148,19 @@ __global__ __launch_bounds__(BG) void k_batch_solve(BatchArgs A) {
 //   dL/dP_ij (upper) = -(u_i x_j + u_j x_i) c D_i D_j, diagonal -u_i x_i c D_i^2,
 // with x, z the normalized optimum in the equilibrated space.
 __device__ void backward_problem(const BatchArgs& A, int pid, double* lds, double* colk, double* red) {
+  const long long t_start = wall_clock64();
   const BatchDesc D = A.desc[pid];
   PV p;
-  setup_view(A, D, p, lds, true);
+  setup_view(A, D, p, lds, false);
   const int n = D.n, m = D.m;
+  // the forward pass left the adjoint factor in the global slot: bring it into
+  // LDS when this problem's placement keeps M there
+  double* Mg = A.Mg + D.mofs;
+  if (p.M != Mg && A.diff_ok[pid]) {
+    const long tot = packed_size(p.nr);
+    for (long e = threadIdx.x; e < tot; e += BG) p.M[e] = Mg[e];
+    __syncthreads();
+  }
   if (!A.diff_ok[pid]) {
     for (int j = threadIdx.x; j < n; j += BG) A.gq[D.xo + j] = 0.0;
     for (int i = threadIdx.x; i < m; i += BG) A.gb[D.co + i] = 0.0;
@@ -1190,6 +1201,11 @@ __device__ void backward_problem(const BatchArgs& A, int pid, double* lds, doubl
     }
   }
   for (int i = threadIdx.x; i < m; i += BG) A.gb[D.co + i] = ee[i] * v[i];
+  if (threadIdx.x == 0) {
+    A.info[16L * pid + 12] = (double)(wall_clock64() - t_start);
+    A.info[16L * pid + 13] = (double)(p.t_solve + p.t_res);
+    A.info[16L * pid + 14] = p.nsolve;
+  }
   __syncthreads();
 }
 
@@ -1587,6 +1603,7 @@ int BatchSolver::backward(const double* const* wx, const double* const* ws, cons
   float ms = 0;
   CX_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
   last_bw_ms_ = ms;
+  fetched_ = false;
   int nok = 0;
   for (int k = 0; k < np_; ++k) {
     const BatchDesc& D = desc_h_[k];
@@ -1668,7 +1685,7 @@ int BatchSolver::autotune(int reps) {
 }
 
 void BatchSolver::stats(int k, double* out4) {
-  for (int i = 0; i < 8; ++i) out4[i] = 0.0;
+  for (int i = 0; i < 11; ++i) out4[i] = 0.0;
   if (!ready_ || k < 0 || k >= np_) return;
   fetch();
   out4[0] = hiters_[k];
@@ -1678,6 +1695,10 @@ void BatchSolver::stats(int k, double* out4) {
   out4[4] = hinfo_[16L * k + 8] / (clock_khz_ * 1e-3);
   out4[5] = hinfo_[16L * k + 9] / (clock_khz_ * 1e-3);
   out4[6] = hinfo_[16L * k + 10] / (clock_khz_ * 1e-3);
+  out4[7] = hinfo_[16L * k + 11];
+  out4[8] = hinfo_[16L * k + 12] / (clock_khz_ * 1e-3);
+  out4[9] = hinfo_[16L * k + 13] / (clock_khz_ * 1e-3);
+  out4[10] = hinfo_[16L * k + 14];
 }
 
 void BatchSolver::tuning(double* out4) const {

@@ -204,6 +204,13 @@ def config2_problem(index, timestep=0, c=None):
                    meta={"contacts": c, "box_rows": "2n"})
 
 
+def config2_perturbation(index, timestep, n, m):
+    """The (q, b) perturbation config2_problem(index, timestep) applies to the
+    base problem's q and b (N(0, 0.01), seeded per problem and timestep)."""
+    pr = _rng(2, index, timestep)
+    return pr.normal(0.0, 0.01, n), pr.normal(0.0, 0.01, m)
+
+
 # --------------------------------------------------------------------------
 # Config 3: differentiation batch member: n=64, m = 16 zero + 64 nonneg + 8 Q4,
 # P = M'M + 1e-2 I with M 32x64 dense, A dense 112x64, loss weights w ~ N(0,1).
