@@ -37,6 +37,7 @@ class CxSettings(ctypes.Structure):
         ("validate_inputs", ctypes.c_int),
         ("linear_solver", ctypes.c_int),
         ("factor_precision", ctypes.c_int),
+        ("fp32_reg", ctypes.c_double),
     ]
 
 

@@ -41,6 +41,7 @@ void cx_default_settings(cx_settings* s) {
   s->validate_inputs = 1;
   s->linear_solver = CX_LS_QDLDL;
   s->factor_precision = CX_FACTOR_FP64;
+  s->fp32_reg = 1e-4;
 }
 
 cx_solver* cx_create(void) {
@@ -151,8 +152,8 @@ void cx_get_kkt_stats(const cx_solver* h, double* stats8) {
   if (h && stats8) h->impl->kkt_stats(stats8);
 }
 
-void cx_get_profile(const cx_solver* h, double* out8) {
-  if (h && out8) h->impl->profile(out8);
+void cx_get_profile(const cx_solver* h, double* out10) {
+  if (h && out10) h->impl->profile(out10);
 }
 
 int cx_get_kernel_stats(const cx_solver* h, double* out, int max_kinds) {

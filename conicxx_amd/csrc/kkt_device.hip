@@ -111,22 +111,23 @@ __device__ inline bool wait_count(int* f, int want, int* err) {
 // Right-looking LDL^T of an n x n block held row-major in LDS (stride LDSP),
 // nc pivot columns; all 256 threads participate.  Pivots at their diagonal
 // positions afterwards, L strictly below.
-__device__ void ldl_lds(double* A, int n, int nc, const double* sgn, int mode, double eps,
-                        double delta, int* stats, bool count, double* colk) {
+template <typename T>
+__device__ void ldl_lds(T* A, int n, int nc, const double* sgn, int mode, double eps,
+                        double delta, int* stats, bool count, T* colk) {
   const int tid = threadIdx.x;
   for (int k = 0; k < nc; ++k) {
-    if (tid == 0) A[k * LDSP + k] = pivot_fix(A[k * LDSP + k], sgn[k], mode, eps, delta, stats, count);
+    if (tid == 0) A[k * LDSP + k] = (T)pivot_fix((double)A[k * LDSP + k], sgn[k], mode, eps, delta, stats, count);
     __syncthreads();
-    const double dinv = 1.0 / A[k * LDSP + k];
+    const T dinv = T(1) / A[k * LDSP + k];
     for (int i = k + 1 + tid; i < n; i += blockDim.x) {
-      double c = A[i * LDSP + k];
+      T c = A[i * LDSP + k];
       colk[i] = c;
       A[i * LDSP + k] = c * dinv;
     }
     __syncthreads();
     // trailing update, thread (row i, column group g): columns k+1+g, k+5+g, ...
     for (int i = k + 1 + (tid & 63); i < n; i += 64) {
-      const double lik = A[i * LDSP + k];
+      const T lik = A[i * LDSP + k];
       for (int j = k + 1 + (tid >> 6); j <= i; j += 4) A[i * LDSP + j] -= lik * colk[j];
     }
     __syncthreads();
@@ -136,46 +137,47 @@ __device__ void ldl_lds(double* A, int n, int nc, const double* sgn, int mode, d
 // ---------------------------------------------------------------------------
 // assembly: zero-fill + K scatter + extend-add of children update matrices
 // ---------------------------------------------------------------------------
+template <typename T>
 __global__ __launch_bounds__(256) void k_assemble(
     const AsmTask* __restrict__ tasks, const AsmSeg* __restrict__ segs, const long* __restrict__ fofs,
     const int* __restrict__ nrv, const int* __restrict__ ncv, const int* __restrict__ firstv,
     const int* __restrict__ kcolptr, const int* __restrict__ ksrc, const long* __restrict__ kdst,
     const signed char* __restrict__ kdiag, const double* __restrict__ Kx, double extra_p,
-    double extra_a, const int* __restrict__ rmap_ofs, const int* __restrict__ rmap, double* F) {
-  const AsmTask T = tasks[blockIdx.x];
-  const int s = T.s, nr = nrv[s], nc = ncv[s];
+    double extra_a, const int* __restrict__ rmap_ofs, const int* __restrict__ rmap, T* F) {
+  const AsmTask Tk = tasks[blockIdx.x];
+  const int s = Tk.s, nr = nrv[s], nc = ncv[s];
   const long base = fofs[s];
   const int tid = threadIdx.x;
   // 1. zero the task's columns
-  const long ncolw = (long)(T.j1 - T.j0) * nr;
-  double* Fc = F + base + (long)T.j0 * nr;
-  for (long idx = tid; idx < ncolw; idx += blockDim.x) Fc[idx] = 0.0;
+  const long ncolw = (long)(Tk.j1 - Tk.j0) * nr;
+  T* Fc = F + base + (long)Tk.j0 * nr;
+  for (long idx = tid; idx < ncolw; idx += blockDim.x) Fc[idx] = T(0);
   __syncthreads();
   // 2. original K entries of the pivot columns in range (distinct destinations)
-  const int jend = min(T.j1, nc);
-  if (T.j0 < jend) {
-    const int e0 = kcolptr[firstv[s] + T.j0], e1 = kcolptr[firstv[s] + jend];
+  const int jend = min(Tk.j1, nc);
+  if (Tk.j0 < jend) {
+    const int e0 = kcolptr[firstv[s] + Tk.j0], e1 = kcolptr[firstv[s] + jend];
     for (int e = e0 + tid; e < e1; e += blockDim.x) {
       double v = Kx[ksrc[e]];
       signed char dg = kdiag[e];
       if (dg > 0) v += extra_p;
       if (dg < 0) v -= extra_a;
-      F[kdst[e]] += v;
+      F[kdst[e]] += (T)v;
     }
   }
   __syncthreads();
   // 3. children update matrices, one child at a time
   const int lane = tid & 63, wv = tid >> 6;
-  for (int g = T.seg_beg; g < T.seg_end; ++g) {
+  for (int g = Tk.seg_beg; g < Tk.seg_end; ++g) {
     const AsmSeg S = segs[g];
     const int c = S.child, nrc = nrv[c], ncc = ncv[c];
     const int nu = nrc - ncc;
     const int* rm = rmap + rmap_ofs[c];
-    const double* Uc = F + fofs[c] + (long)ncc * nrc + ncc;  // U(i', i) = Uc[i*nrc + i']
+    const T* Uc = F + fofs[c] + (long)ncc * nrc + ncc;  // U(i', i) = Uc[i*nrc + i']
     for (int i = S.i0 + wv; i < S.i1; i += 4) {
       const int pc = rm[i];
-      double* Fp = F + base + (long)pc * nr;
-      const double* Ucol = Uc + (long)i * nrc;
+      T* Fp = F + base + (long)pc * nr;
+      const T* Ucol = Uc + (long)i * nrc;
       for (int ip = i + lane; ip < nu; ip += 64) Fp[rm[ip]] += Ucol[ip];
     }
     __syncthreads();
@@ -185,12 +187,13 @@ __global__ __launch_bounds__(256) void k_assemble(
 // ---------------------------------------------------------------------------
 // small fronts: whole front in LDS
 // ---------------------------------------------------------------------------
+template <typename T>
 __global__ __launch_bounds__(256) void k_factor_small(
     const int* __restrict__ list, const long* __restrict__ fofs, const int* __restrict__ nrv,
     const int* __restrict__ ncv, const int* __restrict__ firstv, const double* __restrict__ sgn,
-    double* F, double* Dg, int mode, double eps, double delta, int* stats) {
-  __shared__ double A[SMALL_NR * LDSP];
-  __shared__ double colk[SMALL_NR];
+    T* F, double* Dg, int mode, double eps, double delta, int* stats) {
+  __shared__ T A[SMALL_NR * LDSP];
+  __shared__ T colk[SMALL_NR];
   const int s = list[blockIdx.x];
   const int nr = nrv[s], nc = ncv[s], first = firstv[s];
   const long base = fofs[s];
@@ -213,14 +216,15 @@ __global__ __launch_bounds__(256) void k_factor_small(
 // ---------------------------------------------------------------------------
 constexpr int PANEL_ROWS = 64;
 
+template <typename T>
 __global__ __launch_bounds__(256) void k_diag(
     const PanelTask* __restrict__ tasks, const long* __restrict__ fofs, const int* __restrict__ nrv,
-    const int* __restrict__ firstv, const double* __restrict__ sgn, double* F, double* Dg, int mode,
+    const int* __restrict__ firstv, const double* __restrict__ sgn, T* F, double* Dg, int mode,
     double eps, double delta, int* stats) {
-  __shared__ double L11[NB * LDSP];
-  __shared__ double colk[NB];
-  const PanelTask T = tasks[blockIdx.x];
-  const int s = T.s, nr = nrv[s], first = firstv[s], a = T.a, w = T.w;
+  __shared__ T L11[NB * LDSP];
+  __shared__ T colk[NB];
+  const PanelTask Tk = tasks[blockIdx.x];
+  const int s = Tk.s, nr = nrv[s], first = firstv[s], a = Tk.a, w = Tk.w;
   const long base = fofs[s];
   const int tid = threadIdx.x;
   for (int idx = tid; idx < w * w; idx += blockDim.x) {
@@ -238,32 +242,33 @@ __global__ __launch_bounds__(256) void k_diag(
 
 // sub-panel, step 2: rows below the diagonal block, L21 = R L11^-T D^-1, one
 // 64-row chunk per workgroup (the factored block is read back from F)
+template <typename T>
 __global__ __launch_bounds__(256) void k_trsm(const PanelTask* __restrict__ tasks,
                                               const long* __restrict__ fofs,
-                                              const int* __restrict__ nrv, double* F) {
-  __shared__ double L11[NB * LDSP];
-  __shared__ double X[PANEL_ROWS * LDSP];
-  __shared__ double dinv[NB];
-  const PanelTask T = tasks[blockIdx.x];
-  const int s = T.s, nr = nrv[s], a = T.a, w = T.w;
+                                              const int* __restrict__ nrv, T* F) {
+  __shared__ T L11[NB * LDSP];
+  __shared__ T X[PANEL_ROWS * LDSP];
+  __shared__ T dinv[NB];
+  const PanelTask Tk = tasks[blockIdx.x];
+  const int s = Tk.s, nr = nrv[s], a = Tk.a, w = Tk.w;
   const long base = fofs[s];
   const int tid = threadIdx.x;
   for (int idx = tid; idx < w * w; idx += blockDim.x) {
     int j = idx / w, i = idx % w;
     if (i >= j) L11[i * LDSP + j] = F[base + (long)(a + j) * nr + (a + i)];
   }
-  const int rows = T.r1 - T.r0;
+  const int rows = Tk.r1 - Tk.r0;
   for (int idx = tid; idx < rows * w; idx += blockDim.x) {
     int k = idx / rows, rr = idx % rows;
-    X[rr * LDSP + k] = F[base + (long)(a + k) * nr + T.r0 + rr];
+    X[rr * LDSP + k] = F[base + (long)(a + k) * nr + Tk.r0 + rr];
   }
   __syncthreads();
-  for (int k = tid; k < w; k += blockDim.x) dinv[k] = 1.0 / L11[k * LDSP + k];
+  for (int k = tid; k < w; k += blockDim.x) dinv[k] = T(1) / L11[k * LDSP + k];
   // X L11^T = R (unit lower L11): four lanes per row share the inner sums
   const int rr = tid >> 2, q = tid & 3;
   if (rr < rows) {
     for (int k = 1; k < w; ++k) {
-      double part = 0.0;
+      T part = T(0);
       for (int t = q; t < k; t += 4) part += X[rr * LDSP + t] * L11[k * LDSP + t];
       part += __shfl_xor(part, 1, 64);
       part += __shfl_xor(part, 2, 64);
@@ -274,49 +279,76 @@ __global__ __launch_bounds__(256) void k_trsm(const PanelTask* __restrict__ task
   __syncthreads();
   for (int idx = tid; idx < rows * w; idx += blockDim.x) {
     int k = idx / rows, r2 = idx % rows;
-    F[base + (long)(a + k) * nr + T.r0 + r2] = X[r2 * LDSP + k] * dinv[k];
+    F[base + (long)(a + k) * nr + Tk.r0 + r2] = X[r2 * LDSP + k] * dinv[k];
   }
 }
 
 // ---------------------------------------------------------------------------
-// trailing update on FP64 MFMA:  C[i][j] -= sum_k L[i][k] D[k] L[j][k]
+// trailing update on MFMA:  C[i][j] -= sum_k L[i][k] D[k] L[j][k]
+// FP64 fronts use v_mfma_f64_16x16x4f64, FP32 fronts v_mfma_f32_16x16x4f32;
+// both take a-lane l -> A[l%16][l/16] and b-lane l -> B[l/16][l%16]; their
+// accumulator layouts differ (row of register r: l/16 + 4r for f64,
+// 4(l/16) + r for f32).
 // ---------------------------------------------------------------------------
+typedef float v4f __attribute__((ext_vector_type(4)));
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<double> {
+  typedef v4d acc;
+  static __device__ inline acc op(double a, double b, acc c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ inline int row(int lane, int r) { return (lane >> 4) + 4 * r; }
+};
+template <>
+struct Mfma<float> {
+  typedef v4f acc;
+  static __device__ inline acc op(float a, float b, acc c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ inline int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+
+template <typename T>
 __global__ __launch_bounds__(256) void k_trail(const TrailTask* __restrict__ tasks,
                                                const long* __restrict__ fofs,
                                                const int* __restrict__ nrv,
                                                const int* __restrict__ firstv,
-                                               const double* __restrict__ Dg, double* F) {
+                                               const double* __restrict__ Dg, T* F) {
+  typedef typename Mfma<T>::acc acc_t;
   constexpr int LDA = TM + 4;
   constexpr int PER = KC * TM / 256;  // staged elements per thread and operand
-  __shared__ double As[KC * LDA];
-  __shared__ double Bs[KC * LDA];
-  const TrailTask T = tasks[blockIdx.x];
-  const int s = T.s, nr = nrv[s], first = firstv[s];
+  __shared__ T As[KC * LDA];
+  __shared__ T Bs[KC * LDA];
+  const TrailTask Tk = tasks[blockIdx.x];
+  const int s = Tk.s, nr = nrv[s], first = firstv[s];
   const long base = fofs[s];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int wr = (wv >> 1) * 64, wc = (wv & 1) * 64;  // wave tile origin within the block tile
-  v4d acc[4][4];
+  acc_t acc[4][4];
 #pragma unroll
   for (int mi = 0; mi < 4; ++mi)
 #pragma unroll
-    for (int ni = 0; ni < 4; ++ni) acc[mi][ni] = (v4d){0.0, 0.0, 0.0, 0.0};
+    for (int ni = 0; ni < 4; ++ni) acc[mi][ni] = (acc_t){0, 0, 0, 0};
   const int lrow = tid & (TM - 1);  // staged row (0..127)
   const int lk = tid >> 7;          // staged column parity (0..1)
-  const int gi = T.ti + lrow, gj = T.tj + lrow;
+  const int gi = Tk.ti + lrow, gj = Tk.tj + lrow;
   const bool iok = gi < nr, jok = gj < nr;
-  const double* Fa = F + base + (iok ? gi : 0);
-  const double* Fb = F + base + (jok ? gj : 0);
+  const T* Fa = F + base + (iok ? gi : 0);
+  const T* Fb = F + base + (jok ? gj : 0);
   // register prefetch of the first chunk
-  double ra[PER], rb[PER];
+  T ra[PER], rb[PER];
   // Branch-free staging: every load goes to a clamped, valid address and the
   // out-of-range values are zeroed by a select afterwards, so all PER loads of
   // a chunk are in flight together (a guarded load would make the compiler
   // wait for each one at the join of its branch).
   auto load_chunk = [&](int k0) {
-    double va[PER], vb[PER], vd[PER];
+    T va[PER], vb[PER];
+    double vd[PER];
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
-      const int kq = min(k0 + lk + 2 * q, T.k1 - 1);
+      const int kq = min(k0 + lk + 2 * q, Tk.k1 - 1);
       const long col = (long)kq * nr;
       va[q] = Fa[col];
       vb[q] = Fb[col];
@@ -324,13 +356,13 @@ __global__ __launch_bounds__(256) void k_trail(const TrailTask* __restrict__ tas
     }
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
-      const bool kin = k0 + lk + 2 * q < T.k1;
-      ra[q] = (kin && iok) ? va[q] : 0.0;
-      rb[q] = (kin && jok) ? vb[q] * vd[q] : 0.0;
+      const bool kin = k0 + lk + 2 * q < Tk.k1;
+      ra[q] = (kin && iok) ? va[q] : T(0);
+      rb[q] = (kin && jok) ? vb[q] * (T)vd[q] : T(0);
     }
   };
-  load_chunk(T.k0);
-  for (int k0 = T.k0; k0 < T.k1; k0 += KC) {
+  load_chunk(Tk.k0);
+  for (int k0 = Tk.k0; k0 < Tk.k1; k0 += KC) {
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
       const int kk = lk + 2 * q;
@@ -338,10 +370,10 @@ __global__ __launch_bounds__(256) void k_trail(const TrailTask* __restrict__ tas
       Bs[kk * LDA + lrow] = rb[q];
     }
     __syncthreads();
-    if (k0 + KC < T.k1) load_chunk(k0 + KC);  // in flight while the MFMAs run
+    if (k0 + KC < Tk.k1) load_chunk(k0 + KC);  // in flight while the MFMAs run
 #pragma unroll
     for (int kk = 0; kk < KC; kk += 4) {
-      double af[4], bf[4];
+      T af[4], bf[4];
       const int kr = kk + (lane >> 4);
 #pragma unroll
       for (int mi = 0; mi < 4; ++mi) af[mi] = As[kr * LDA + wr + mi * 16 + (lane & 15)];
@@ -350,26 +382,24 @@ __global__ __launch_bounds__(256) void k_trail(const TrailTask* __restrict__ tas
 #pragma unroll
       for (int mi = 0; mi < 4; ++mi)
 #pragma unroll
-        for (int ni = 0; ni < 4; ++ni)
-          acc[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[mi], bf[ni], acc[mi][ni], 0, 0, 0);
+        for (int ni = 0; ni < 4; ++ni) acc[mi][ni] = Mfma<T>::op(af[mi], bf[ni], acc[mi][ni]);
     }
     __syncthreads();
   }
-  // write back: v_mfma_f64_16x16x4f64 leaves D[lane/16 + 4r][lane%16] in
-  // register r.  Per row block mi, the 16 old values are loaded together from
-  // clamped addresses, then the valid ones are stored.
+  // write back: per row block mi, the 16 old values are loaded together from
+  // clamped addresses, then the valid ones are stored
 #pragma unroll
   for (int mi = 0; mi < 4; ++mi) {
-    double old[4][4];
+    T old[4][4];
     long adr[4][4];
     bool ok[4][4];
 #pragma unroll
     for (int ni = 0; ni < 4; ++ni) {
-      const int j = T.tj + wc + ni * 16 + (lane & 15);
+      const int j = Tk.tj + wc + ni * 16 + (lane & 15);
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int i = T.ti + wr + mi * 16 + (lane >> 4) + 4 * r;
-        ok[ni][r] = j < T.chi && i < nr && i >= j && i >= T.rlo;
+        const int i = Tk.ti + wr + mi * 16 + Mfma<T>::row(lane, r);
+        ok[ni][r] = j < Tk.chi && i < nr && i >= j && i >= Tk.rlo;
         adr[ni][r] = ok[ni][r] ? base + (long)j * nr + i : base;
         old[ni][r] = F[adr[ni][r]];
       }
@@ -435,21 +465,22 @@ __device__ inline double bcast(double v, int l) {
 // it loads the 64x64 block of L into registers *before* polling the block's
 // flag, so once y_cb is published only a broadcast-and-FMA pass remains.  The
 // diagonal block is staged in LDS up front.
+template <typename T>
 __global__ __launch_bounds__(256) void k_fwd(
     const SolveTask* __restrict__ tasks, int* ticket, const long* __restrict__ fofs,
     const int* __restrict__ nrv, const int* __restrict__ ncv, const int* __restrict__ firstv,
     const int* __restrict__ rowptr, const int* __restrict__ gptr, const int* __restrict__ gidx,
     const long* __restrict__ uofs, const int* __restrict__ child_ptr, const int* __restrict__ child,
     const int* __restrict__ nblk, const int* __restrict__ flag_ofs, int* flags, int* done, int gen,
-    const double* __restrict__ F, double* xp, double* U, int* err) {
+    const T* __restrict__ F, double* xp, double* U, int* err) {
   __shared__ int tk;
   __shared__ double part[4][SB];
   __shared__ double Ld[SB * LDSP];  // Ld[row][col] of the diagonal block
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   if (tid == 0) tk = atomicAdd(ticket, 1);
   __syncthreads();
-  const SolveTask T = tasks[tk];
-  const int s = T.s, rb = T.blk, nr = nrv[s], nc = ncv[s], first = firstv[s];
+  const SolveTask Tk = tasks[tk];
+  const int s = Tk.s, rb = Tk.blk, nr = nrv[s], nc = ncv[s], first = firstv[s];
   const long base = fofs[s];
   const int r0 = rb * SB;
   const int ncb = (nc + SB - 1) / SB;
@@ -458,7 +489,7 @@ __global__ __launch_bounds__(256) void k_fwd(
   const int rc = rv ? r : nr - 1;  // clamped, always in bounds
   const bool diag = rb < ncb;
   const int kc = diag ? min(SB, nc - r0) : 0;
-  for (int c = wv; c < kc; c += 4) Ld[lane * LDSP + c] = F[base + (long)(r0 + c) * nr + rc];
+  for (int c = wv; c < kc; c += 4) Ld[lane * LDSP + c] = (double)F[base + (long)(r0 + c) * nr + rc];
   // Children's update vectors first: they do not depend on this front's own
   // blocks, so gathering them here keeps them off the front's dependency
   // chain.  Every thread waits for a share of the children, every wave sums a
@@ -480,10 +511,10 @@ __global__ __launch_bounds__(256) void k_fwd(
   for (int cb = wv; cb < ncbk; cb += 4) {
     const int cbase = cb * SB;
     const int cend = min(SB, nc - cbase);
-    const double* Fc = F + base + (long)cbase * nr + rc;
+    const T* Fc = F + base + (long)cbase * nr + rc;
     double Lb[SB];
 #pragma unroll
-    for (int c = 0; c < SB; ++c) Lb[c] = (c < cend) ? Fc[(long)c * nr] : 0.0;
+    for (int c = 0; c < SB; ++c) Lb[c] = (c < cend) ? (double)Fc[(long)c * nr] : 0.0;
     if (cb > ready) {
       if (lane == 0) {
         int* fl = &flags[(flag_ofs[s] + ncbk - 1) * FSTRIDE];  // newest block this task needs
@@ -555,24 +586,25 @@ __device__ inline double wave_reduce_scatter64(double (&a)[SB], int lane) {
 // parent front is done, then later pivot blocks in descending order) with
 // lane = row: every lane keeps 64 column partials (reduce-scattered once at
 // the end); the L block is loaded into registers before the flag is polled.
+template <typename T>
 __global__ __launch_bounds__(256) void k_bwd(
     const SolveTask* __restrict__ tasks, int* ticket, const long* __restrict__ fofs,
     const int* __restrict__ nrv, const int* __restrict__ ncv, const int* __restrict__ firstv,
     const int* __restrict__ rowptr, const int* __restrict__ rows, const int* __restrict__ parent,
     const int* __restrict__ nblkb, const int* __restrict__ flag_ofs, int* flags, int* done, int gen,
-    const double* __restrict__ F, const double* __restrict__ Dg, double* xp, int* err) {
+    const T* __restrict__ F, const double* __restrict__ Dg, double* xp, int* err) {
   __shared__ int tk;
   __shared__ double part[4][SB];
   __shared__ double Ld[SB * LDSP];  // Ld[row][col] of the diagonal block
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   if (tid == 0) tk = atomicAdd(ticket, 1);
   __syncthreads();
-  const SolveTask T = tasks[tk];
-  const int s = T.s, cb = T.blk, nr = nrv[s], nc = ncv[s], first = firstv[s];
+  const SolveTask Tk = tasks[tk];
+  const int s = Tk.s, cb = Tk.blk, nr = nrv[s], nc = ncv[s], first = firstv[s];
   const long base = fofs[s];
   const int c0 = cb * SB, kc = min(SB, nc - c0);
   const int ncb = (nc + SB - 1) / SB;
-  for (int c = wv; c < kc; c += 4) Ld[lane * LDSP + c] = (lane < kc) ? F[base + (long)(c0 + c) * nr + c0 + lane] : 0.0;
+  for (int c = wv; c < kc; c += 4) Ld[lane * LDSP + c] = (lane < kc) ? (double)F[base + (long)(c0 + c) * nr + c0 + lane] : 0.0;
   if (tid == 0 && parent[s] >= 0) wait_count(&done[parent[s] * FSTRIDE], nblkb[parent[s]], err);
   __syncthreads();
   double acc[SB];
@@ -584,11 +616,11 @@ __global__ __launch_bounds__(256) void k_bwd(
     const int R = rstart + q * SB;
     const int rr = R + lane;
     const bool rv = rr < nr;
-    const double* Lr = F + base + (long)c0 * nr + (rv ? rr : nr - 1);
+    const T* Lr = F + base + (long)c0 * nr + (rv ? rr : nr - 1);
     constexpr int H = SB / 2;
     double Lb[H];  // first half of the block, loaded before the flag is polled
 #pragma unroll
-    for (int c = 0; c < H; ++c) Lb[c] = (c < kc) ? Lr[(long)c * nr] : 0.0;
+    for (int c = 0; c < H; ++c) Lb[c] = (c < kc) ? (double)Lr[(long)c * nr] : 0.0;
     if (lane == 0 && R < nc) {
       // pivot rows [R, min(R+SB, nc)) belong to blocks b > cb of this front
       const int b0 = R / SB, b1 = (min(R + SB, nc) - 1) / SB;
@@ -604,7 +636,7 @@ __global__ __launch_bounds__(256) void k_bwd(
 #pragma unroll
     for (int c = 0; c < H; ++c) acc[c] += Lb[c] * xv;
 #pragma unroll
-    for (int c = 0; c < H; ++c) Lb[c] = (H + c < kc) ? Lr[(long)(H + c) * nr] : 0.0;
+    for (int c = 0; c < H; ++c) Lb[c] = (H + c < kc) ? (double)Lr[(long)(H + c) * nr] : 0.0;
 #pragma unroll
     for (int c = 0; c < H; ++c) acc[H + c] += Lb[c] * xv;
   }
@@ -1011,9 +1043,9 @@ void DeviceKKT::reset_timing() {
   for (auto& k : kstats_) k = KindStats{};
 }
 
-void DeviceKKT::factorize_async(const double* Kx, double extra_p, double extra_a, int mode,
-                                double eps, double delta) {
-  CX_HIP(hipMemsetAsync(fstat_.p, 0, 2 * sizeof(int), st_));
+template <typename T>
+void DeviceKKT::launch_factor(T* F, const double* Kx, double extra_p, double extra_a, int mode, double eps,
+                              double delta) {
   size_t tl = 0;
   for (const Launch& L : launches_) {
     const int n = L.end - L.beg;
@@ -1021,38 +1053,54 @@ void DeviceKKT::factorize_async(const double* Kx, double extra_p, double extra_a
     switch (L.kind) {
       case L_ASSEMBLE:
         tbegin();
-        hipLaunchKernelGGL(k_assemble, dim3(n), dim3(256), 0, st_, asm_.p + L.beg, seg_.p, fofs_.p,
+        hipLaunchKernelGGL(k_assemble<T>, dim3(n), dim3(256), 0, st_, asm_.p + L.beg, seg_.p, fofs_.p,
                            nr_.p, nc_.p, first_.p, kcolptr_.p, ksrc_.p, kdst_.p, kdiag_.p, Kx,
-                           extra_p, extra_a, rmap_ofs_.p, rmap_.p, F_.p);
+                           extra_p, extra_a, rmap_ofs_.p, rmap_.p, F);
         tend(T_ASM, 0.0);
         break;
       case L_SMALL:
         tbegin();
-        hipLaunchKernelGGL(k_factor_small, dim3(n), dim3(256), 0, st_, small_.p + L.beg, fofs_.p,
-                           nr_.p, nc_.p, first_.p, sgn_.p, F_.p, D_.p, mode, eps, delta, fstat_.p);
+        hipLaunchKernelGGL(k_factor_small<T>, dim3(n), dim3(256), 0, st_, small_.p + L.beg, fofs_.p,
+                           nr_.p, nc_.p, first_.p, sgn_.p, F, D_.p, mode, eps, delta, fstat_.p);
         tend(T_SMALL, 0.0);
         break;
       case L_DIAG:
         tbegin();
-        hipLaunchKernelGGL(k_diag, dim3(n), dim3(256), 0, st_, panel_.p + L.beg, fofs_.p, nr_.p,
-                           first_.p, sgn_.p, F_.p, D_.p, mode, eps, delta, fstat_.p);
+        hipLaunchKernelGGL(k_diag<T>, dim3(n), dim3(256), 0, st_, panel_.p + L.beg, fofs_.p, nr_.p,
+                           first_.p, sgn_.p, F, D_.p, mode, eps, delta, fstat_.p);
         tend(T_DIAG, 0.0);
         break;
       case L_PANEL:
         tbegin();
-        hipLaunchKernelGGL(k_trsm, dim3(n), dim3(256), 0, st_, panel_.p + L.beg, fofs_.p, nr_.p, F_.p);
+        hipLaunchKernelGGL(k_trsm<T>, dim3(n), dim3(256), 0, st_, panel_.p + L.beg, fofs_.p, nr_.p, F);
         tend(T_TRSM, 0.0);
         break;
       case L_TRAIL:
         tbegin();
-        hipLaunchKernelGGL(k_trail, dim3(n), dim3(256), 0, st_, trail_.p + L.beg, fofs_.p, nr_.p,
-                           first_.p, D_.p, F_.p);
+        hipLaunchKernelGGL(k_trail<T>, dim3(n), dim3(256), 0, st_, trail_.p + L.beg, fofs_.p, nr_.p,
+                           first_.p, D_.p, F);
         tend(T_TRAIL, trail_launch_flops_[tl]);
         ++tl;
         break;
     }
     CX_LAUNCH_CHECK();
   }
+}
+
+void DeviceKKT::enable_fp32(bool on) {
+  if (on && !Ff_.p) Ff_.alloc(F_.n);
+}
+
+void DeviceKKT::factorize_async(const double* Kx, double extra_p, double extra_a, int mode,
+                                double eps, double delta, int prec) {
+  CX_HIP(hipMemsetAsync(fstat_.p, 0, 2 * sizeof(int), st_));
+  if (prec == 1) {
+    enable_fp32(true);
+    launch_factor<float>(Ff_.p, Kx, extra_p, extra_a, mode, eps, delta);
+  } else {
+    launch_factor<double>(F_.p, Kx, extra_p, extra_a, mode, eps, delta);
+  }
+  prec_ = prec;
   hipLaunchKernelGGL(k_dstats, dim3(1), dim3(256), 0, st_, D_.p, P_.N, dstat_.p);
   CX_LAUNCH_CHECK();
 }
@@ -1080,15 +1128,25 @@ void DeviceKKT::solve_async(const double* rhs, double* x) {
   int* tick = done_.p + (size_t)2 * ns * FSTRIDE;
   ++gen_;
   tbegin();
-  hipLaunchKernelGGL(k_fwd, dim3(nfwd_), dim3(256), 0, st_, fwd_.p, tick, fofs_.p, nr_.p, nc_.p,
-                     first_.p, rowptr_.p, gptr_.p, gidx_.p, uofs_.p, child_ptr_.p, child_.p, nblk_.p,
-                     flag_ofs_.p, flags_.p, done_.p, gen_, F_.p, xp_.p, U_.p, serr_.p);
+  if (prec_ == 1)
+    hipLaunchKernelGGL(k_fwd<float>, dim3(nfwd_), dim3(256), 0, st_, fwd_.p, tick, fofs_.p, nr_.p, nc_.p,
+                       first_.p, rowptr_.p, gptr_.p, gidx_.p, uofs_.p, child_ptr_.p, child_.p, nblk_.p,
+                       flag_ofs_.p, flags_.p, done_.p, gen_, (const float*)Ff_.p, xp_.p, U_.p, serr_.p);
+  else
+    hipLaunchKernelGGL(k_fwd<double>, dim3(nfwd_), dim3(256), 0, st_, fwd_.p, tick, fofs_.p, nr_.p, nc_.p,
+                       first_.p, rowptr_.p, gptr_.p, gidx_.p, uofs_.p, child_ptr_.p, child_.p, nblk_.p,
+                       flag_ofs_.p, flags_.p, done_.p, gen_, (const double*)F_.p, xp_.p, U_.p, serr_.p);
   tend(T_FWD, sweep_bytes_);
   ++gen_;
   tbegin();
-  hipLaunchKernelGGL(k_bwd, dim3(nbwd_), dim3(256), 0, st_, bwd_.p, tick + FSTRIDE, fofs_.p, nr_.p,
+  if (prec_ == 1)
+    hipLaunchKernelGGL(k_bwd<float>, dim3(nbwd_), dim3(256), 0, st_, bwd_.p, tick + FSTRIDE, fofs_.p, nr_.p,
                      nc_.p, first_.p, rowptr_.p, rows_.p, parent_.p, nblkb_.p, flag_ofs_.p, flags_.p,
-                     done_.p + (size_t)ns * FSTRIDE, gen_, F_.p, D_.p, xp_.p, serr_.p);
+                     done_.p + (size_t)ns * FSTRIDE, gen_, (const float*)Ff_.p, D_.p, xp_.p, serr_.p);
+  else
+    hipLaunchKernelGGL(k_bwd<double>, dim3(nbwd_), dim3(256), 0, st_, bwd_.p, tick + FSTRIDE, fofs_.p, nr_.p,
+                     nc_.p, first_.p, rowptr_.p, rows_.p, parent_.p, nblkb_.p, flag_ofs_.p, flags_.p,
+                     done_.p + (size_t)ns * FSTRIDE, gen_, (const double*)F_.p, D_.p, xp_.p, serr_.p);
   tend(T_BWD, sweep_bytes_);
   hipLaunchKernelGGL(k_perm_out, dim3(ceil_div(N, 256)), dim3(256), 0, st_, perm_.p, xp_.p, x, N);
   CX_LAUNCH_CHECK();

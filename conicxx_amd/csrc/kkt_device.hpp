@@ -44,8 +44,12 @@ class DeviceKKT {
 
   // Numeric factorization of the K lower values Kx (device, CSC order) with the
   // diagonal bumped by +extra_p (rows < nx) and -extra_a (rows >= nx).
+  // prec 1: the fronts are factored in FP32 (separate FP32 front storage,
+  // FP32 MFMA trailing updates); pivots and the solve sweeps stay FP64.
   void factorize_async(const double* Kx, double extra_p, double extra_a, int mode, double eps,
-                       double delta);
+                       double delta, int prec = 0);
+  void enable_fp32(bool on);
+  int factor_precision() const { return prec_; }
   FactorStats factor_stats();  // synchronizes the stream
 
   // x = K^-1 rhs with the current factorization (device vectors of length N).
@@ -75,6 +79,9 @@ class DeviceKKT {
 
  private:
   void build_schedule();
+  template <typename T>
+  void launch_factor(T* F, const double* Kx, double extra_p, double extra_a, int mode, double eps,
+                     double delta);
 
   hipStream_t st_ = nullptr;
   SupernodalPlan P_;
@@ -102,7 +109,9 @@ class DeviceKKT {
   DevBuf<int> small_;
   DevBuf<PanelTask> panel_;
   DevBuf<TrailTask> trail_;
-  DevBuf<double> F_;   // fronts
+  DevBuf<double> F_;   // fronts (FP64)
+  DevBuf<float> Ff_;   // fronts (FP32 factorization), allocated on first use
+  int prec_ = 0;       // precision of the current factorization
   DevBuf<double> D_;   // pivots, permuted order
   DevBuf<int> fstat_;  // [0] zero pivot, [1] nreg
   DevBuf<double> dstat_;

@@ -310,8 +310,22 @@ bool Solver::kkt_update_scaling() {
 
 bool Solver::try_factor(double ep, double ea) {
   const double t0 = now_s();
+  if (fp32_) {
+    // FP32 factorization of K + r*S (S = +1 on variable pivots, -1 on cone
+    // pivots), every pivot floored at magnitude r.  Single precision cannot
+    // carry the O(1e-8) static shifts: a pivot that small grows the update
+    // by 1e8 and wipes out the O(1) entries it is added to.  The shift keeps
+    // the growth near 1/r, and FP64 refinement against K removes it.
+    const double r = st_.fp32_reg;
+    kkt_.factorize_async(Kx_.p, ep + r, ea + r, PIVOT_REGULARIZE, 0.5 * r, r, 1);
+    FactorStats fs = kkt_.factor_stats();
+    prof_nfact32_++;
+    prof_fact_ms_ += (now_s() - t0) * 1e3;
+    prof_nfact_++;
+    return std::isfinite(fs.min_abs_d) && fs.min_abs_d > 0.0;
+  }
   const int mode = st_.linear_solver == CX_LS_REGULARIZED ? PIVOT_REGULARIZE : PIVOT_CHECK;
-  kkt_.factorize_async(Kx_.p, ep, ea, mode, st_.dynamic_reg_eps, st_.dynamic_reg_delta);
+  kkt_.factorize_async(Kx_.p, ep, ea, mode, st_.dynamic_reg_eps, st_.dynamic_reg_delta, 0);
   FactorStats fs = kkt_.factor_stats();
   prof_fact_ms_ += (now_s() - t0) * 1e3;
   prof_nfact_++;
@@ -390,7 +404,11 @@ double Solver::kkt_solve(const double* rhs, double* x) {
     rr = reduce(B2)[0];
   }
   double rel = std::sqrt(rr) / rn;
-  for (int it = 0; it < st_.refine_max_iter && rel > st_.refine_tol; ++it) {
+  // An FP32 factorization is a preconditioner for FP64 refinement: it gets
+  // more steps, and when the residual still misses refine_tol the solve falls
+  // back to an FP64 factorization for the rest of this solve().
+  const int rmax = fp32_ ? std::max(st_.refine_max_iter, FP32_REFINE_MAX) : st_.refine_max_iter;
+  for (int it = 0; it < rmax && rel > st_.refine_tol; ++it) {
     kkt_.solve_async(r_.p, dsol_.p);
     prof_nsolve_++;
     launch_axpby(N_, 1.0, dsol_.p, 1.0, x, stream_);
@@ -398,9 +416,17 @@ double Solver::kkt_solve(const double* rhs, double* x) {
     launch_lin2(N_, 1.0, rhs, -1.0, t_.p, r_.p, stream_);
     RedBatch B3{};
     B3.op[B3.nops++] = dot_op(r_.p, r_.p, N_);
+    const double prev = rel;
     rel = std::sqrt(reduce(B3)[0]) / rn;
+    if (fp32_ && !(rel < 0.5 * prev)) break;  // FP32 refinement stalled
   }
   prof_solve_ms_ += (now_s() - t0) * 1e3;
+  if (fp32_ && !(rel <= st_.refine_tol)) {
+    fp32_ = false;
+    prof_fp64_fallback_++;
+    if (!factor_retry()) return NAN;
+    return kkt_solve(rhs, x);
+  }
   return rel;
 }
 
@@ -505,6 +531,7 @@ int Solver::backward(const double* wx, const double* ws, const double* wz, doubl
     us[i] = ws ? ws[i] / ee_[i] : 0.0;
     ntz[i] = wz ? -wz[i] * ee_[i] / ec_ : 0.0;
   }
+  fp32_ = false;  // the adjoint factorization is FP64
   if (!bw_factored_) {
     if (!soc_dims_.empty()) {
       bw_tcorr_.alloc(soc_dims_.size());
@@ -735,6 +762,8 @@ int Solver::solve() {  // solver_impl.cpp:367-466
   const double t0 = now_s();
   status_ = CX_UNSOLVED;
   bw_valid_ = false;
+  fp32_ = st_.factor_precision == CX_FACTOR_FP32_REFINED;
+  prof_nfact32_ = prof_fp64_fallback_ = 0;
   iterations_ = 0;
   objective_ = pres_ = dres_ = gap_ = mu_out_ = 0;
   prof_fact_ms_ = prof_solve_ms_ = 0;
@@ -895,7 +924,8 @@ void Solver::kkt_stats(double* out8) const {
   out8[7] = P.nnzL_exact;
 }
 
-void Solver::profile(double* out8) const {
+void Solver::profile(double* out10) const {
+  double* out8 = out10;
   const DeviceKKT::KindStats& t = kkt_.kind_stats(DeviceKKT::T_TRAIL);
   out8[0] = prof_fact_ms_;
   out8[1] = prof_solve_ms_;
@@ -905,6 +935,8 @@ void Solver::profile(double* out8) const {
   out8[5] = t.launches ? t.ms / t.launches : 0.0;
   out8[6] = (double)t.launches;
   out8[7] = t.work;
+  out10[8] = prof_nfact32_;
+  out10[9] = prof_fp64_fallback_;
 }
 
 int Solver::kernel_stats(double* out, int max_kinds) const {

@@ -34,7 +34,7 @@ class Solver {
   int get_solution(double* x, double* s, double* z, double* obj, cx_info* info) const;
   void dims(int* n, int* m) const { *n = n_; *m = m_; }
   void kkt_stats(double* out8) const;
-  void profile(double* out8) const;
+  void profile(double* out10) const;
   int kernel_stats(double* out, int max_kinds) const;
   int backward(const double* wx, const double* ws, const double* wz, double* gP, double* gq,
                double* gA, double* gb);
@@ -125,6 +125,11 @@ class Solver {
   // profile
   double prof_fact_ms_ = 0, prof_solve_ms_ = 0, prof_total_ms_ = 0;
   int prof_nfact_ = 0, prof_nsolve_ = 0;
+  // FP32 factorization with FP64 refinement (factor_precision); falls back to
+  // FP64 for the rest of a solve when refinement misses refine_tol
+  bool fp32_ = false;
+  int prof_nfact32_ = 0, prof_fp64_fallback_ = 0;
+  static constexpr int FP32_REFINE_MAX = 10;
 };
 
 }  // namespace cx

@@ -68,6 +68,7 @@ class Settings:  # include/conicxx/settings.h:21-62 (defaults identical)
     validate_inputs: bool = True
     linear_solver: LinearSolverBackend = LinearSolverBackend.Qdldl
     factor_precision: FactorPrecision = FactorPrecision.FP64
+    fp32_reg: float = 1e-4  # FP32 factorization: diagonal shift and pivot floor
 
     def to_c(self):
         c = _lib.CxSettings()
@@ -276,10 +277,10 @@ class Solver:
         return dict(zip(keys, out.tolist()))
 
     def profile(self):
-        out = np.zeros(8)
+        out = np.zeros(10)
         _lib.lib().cx_get_profile(self._h, out.ctypes.data_as(_lib._f64p))
         keys = ["factor_ms", "solve_ms", "other_ms", "factorizations", "kkt_solves",
-                "trail_avg_ms", "trail_launches", "trail_flops"]
+                "trail_avg_ms", "trail_launches", "trail_flops", "fp32_factorizations", "fp64_fallbacks"]
         return dict(zip(keys, out.tolist()))
 
     KERNEL_CLASSES = ["trail", "diag", "trsm", "assemble", "small", "fwd", "bwd"]

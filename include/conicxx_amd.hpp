@@ -72,6 +72,7 @@ struct Settings {
   bool validate_inputs = true;
   LinearSolverBackend linear_solver = LinearSolverBackend::Qdldl;
   FactorPrecision factor_precision = FactorPrecision::FP64;
+  Scalar fp32_reg = 1e-4;
 
   cx_settings to_c() const {
     cx_settings c;
@@ -97,6 +98,7 @@ struct Settings {
     c.validate_inputs = validate_inputs;
     c.linear_solver = static_cast<int>(linear_solver);
     c.factor_precision = static_cast<int>(factor_precision);
+    c.fp32_reg = fp32_reg;
     return c;
   }
 };

@@ -57,6 +57,7 @@ typedef struct cx_settings {
   int validate_inputs; /* accepted, unused */
   int linear_solver;   /* enum cx_linear_solver */
   int factor_precision; /* enum cx_factor_precision */
+  double fp32_reg;      /* FP32 factorization: diagonal shift and pivot floor (precision form) */
 } cx_settings;
 
 /* Info (include/conicxx/solution.h:20-28). */
@@ -124,9 +125,11 @@ void cx_get_kkt_stats(const cx_solver* h, double* stats8);
  * the pivot check), [1] KKT solves incl. refinement (ms), [2] everything else (ms),
  * [3] number of factorizations, [4] number of KKT solves (incl. refinement steps),
  * [5] average device duration of the dense trailing-update kernel (ms), [6] its
- * launches, [7] its algorithmic flops.  [5..7] need CONICXX_KERNEL_TIMING=1 in the
- * environment at setup().  No counterpart in the reference. */
-void cx_get_profile(const cx_solver* h, double* out8);
+ * launches, [7] its algorithmic flops, [8] FP32 factorizations, [9] FP32 -> FP64
+ * fallbacks (factor_precision = CX_FACTOR_FP32_REFINED).  [5..7] need
+ * CONICXX_KERNEL_TIMING=1 in the environment at setup().  No counterpart in the
+ * reference. */
+void cx_get_profile(const cx_solver* h, double* out10);
 
 /* Device time per kernel class of the last solve, measured with HIP events on the
  * solver's stream (CONICXX_KERNEL_TIMING=1 at setup()): for class k, out[3k..3k+2]
