@@ -41,6 +41,14 @@ constexpr long SPIN_LIMIT = 1L << 21;  // ~0.2 s of polling
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
+// Fronts are stored lower-packed by columns: column j of an nr-row front holds
+// rows j..nr-1 and starts colofs(nr, j) elements after the front's base; a
+// front takes nr(nr+1)/2 elements.  FP32 and FP64 factors share one buffer
+// (the FP32 factor uses its first half).
+__host__ __device__ inline long colofs(int nr, int j) { return (long)j * nr - ((long)j * (j - 1)) / 2; }
+// element (i, j), i >= j
+__host__ __device__ inline long fpos(int nr, int i, int j) { return colofs(nr, j) + (i - j); }
+
 __device__ inline double pivot_fix(double d, double sgn, int mode, double eps, double delta,
                                    int* stats, bool count) {
   if (mode == PIVOT_REGULARIZE) {
@@ -149,8 +157,8 @@ __global__ __launch_bounds__(256) void k_assemble(
   const long base = fofs[s];
   const int tid = threadIdx.x;
   // 1. zero the task's columns
-  const long ncolw = (long)(Tk.j1 - Tk.j0) * nr;
-  T* Fc = F + base + (long)Tk.j0 * nr;
+  const long ncolw = colofs(nr, Tk.j1) - colofs(nr, Tk.j0);
+  T* Fc = F + base + colofs(nr, Tk.j0);
   for (long idx = tid; idx < ncolw; idx += blockDim.x) Fc[idx] = T(0);
   __syncthreads();
   // 2. original K entries of the pivot columns in range (distinct destinations)
@@ -173,11 +181,11 @@ __global__ __launch_bounds__(256) void k_assemble(
     const int c = S.child, nrc = nrv[c], ncc = ncv[c];
     const int nu = nrc - ncc;
     const int* rm = rmap + rmap_ofs[c];
-    const T* Uc = F + fofs[c] + (long)ncc * nrc + ncc;  // U(i', i) = Uc[i*nrc + i']
     for (int i = S.i0 + wv; i < S.i1; i += 4) {
       const int pc = rm[i];
-      T* Fp = F + base + (long)pc * nr;
-      const T* Ucol = Uc + (long)i * nrc;
+      T* Fp = F + base + colofs(nr, pc) - pc;  // Fp[row] = element (row, pc)
+      // Ucol[i'] = element (ncc + i', ncc + i) of the child, i' >= i
+      const T* Ucol = F + fofs[c] + colofs(nrc, ncc + i) - i;
       for (int ip = i + lane; ip < nu; ip += 64) Fp[rm[ip]] += Ucol[ip];
     }
     __syncthreads();
@@ -200,13 +208,13 @@ __global__ __launch_bounds__(256) void k_factor_small(
   const int tid = threadIdx.x;
   for (int idx = tid; idx < nr * nr; idx += blockDim.x) {
     int col = idx / nr, row = idx % nr;
-    if (row >= col) A[row * LDSP + col] = F[base + idx];
+    if (row >= col) A[row * LDSP + col] = F[base + fpos(nr, row, col)];
   }
   __syncthreads();
   ldl_lds(A, nr, nc, sgn + first, mode, eps, delta, stats, true, colk);
   for (int idx = tid; idx < nr * nr; idx += blockDim.x) {
     int col = idx / nr, row = idx % nr;
-    if (row >= col) F[base + idx] = A[row * LDSP + col];
+    if (row >= col) F[base + fpos(nr, row, col)] = A[row * LDSP + col];
   }
   for (int k = tid; k < nc; k += blockDim.x) Dg[first + k] = A[k * LDSP + k];
 }
@@ -229,13 +237,13 @@ __global__ __launch_bounds__(256) void k_diag(
   const int tid = threadIdx.x;
   for (int idx = tid; idx < w * w; idx += blockDim.x) {
     int j = idx / w, i = idx % w;
-    if (i >= j) L11[i * LDSP + j] = F[base + (long)(a + j) * nr + (a + i)];
+    if (i >= j) L11[i * LDSP + j] = F[base + fpos(nr, a + i, a + j)];
   }
   __syncthreads();
   ldl_lds(L11, w, w, sgn + first + a, mode, eps, delta, stats, true, colk);
   for (int idx = tid; idx < w * w; idx += blockDim.x) {
     int j = idx / w, i = idx % w;
-    if (i >= j) F[base + (long)(a + j) * nr + (a + i)] = L11[i * LDSP + j];
+    if (i >= j) F[base + fpos(nr, a + i, a + j)] = L11[i * LDSP + j];
   }
   for (int k = tid; k < w; k += blockDim.x) Dg[first + a + k] = L11[k * LDSP + k];
 }
@@ -255,12 +263,12 @@ __global__ __launch_bounds__(256) void k_trsm(const PanelTask* __restrict__ task
   const int tid = threadIdx.x;
   for (int idx = tid; idx < w * w; idx += blockDim.x) {
     int j = idx / w, i = idx % w;
-    if (i >= j) L11[i * LDSP + j] = F[base + (long)(a + j) * nr + (a + i)];
+    if (i >= j) L11[i * LDSP + j] = F[base + fpos(nr, a + i, a + j)];
   }
   const int rows = Tk.r1 - Tk.r0;
   for (int idx = tid; idx < rows * w; idx += blockDim.x) {
     int k = idx / rows, rr = idx % rows;
-    X[rr * LDSP + k] = F[base + (long)(a + k) * nr + Tk.r0 + rr];
+    X[rr * LDSP + k] = F[base + fpos(nr, Tk.r0 + rr, a + k)];
   }
   __syncthreads();
   for (int k = tid; k < w; k += blockDim.x) dinv[k] = T(1) / L11[k * LDSP + k];
@@ -279,7 +287,7 @@ __global__ __launch_bounds__(256) void k_trsm(const PanelTask* __restrict__ task
   __syncthreads();
   for (int idx = tid; idx < rows * w; idx += blockDim.x) {
     int k = idx / rows, r2 = idx % rows;
-    F[base + (long)(a + k) * nr + Tk.r0 + r2] = X[r2 * LDSP + k] * dinv[k];
+    F[base + fpos(nr, Tk.r0 + r2, a + k)] = X[r2 * LDSP + k] * dinv[k];
   }
 }
 
@@ -335,8 +343,10 @@ __global__ __launch_bounds__(256) void k_trail(const TrailTask* __restrict__ tas
   const int lk = tid >> 7;          // staged column parity (0..1)
   const int gi = Tk.ti + lrow, gj = Tk.tj + lrow;
   const bool iok = gi < nr, jok = gj < nr;
-  const T* Fa = F + base + (iok ? gi : 0);
-  const T* Fb = F + base + (jok ? gj : 0);
+  // rows of the staged operands are below the staged columns (clamped rows to
+  // the last row of the front, also below them)
+  const T* Fa = F + base + (iok ? gi : nr - 1);
+  const T* Fb = F + base + (jok ? gj : nr - 1);
   // register prefetch of the first chunk
   T ra[PER], rb[PER];
   // Branch-free staging: every load goes to a clamped, valid address and the
@@ -349,7 +359,7 @@ __global__ __launch_bounds__(256) void k_trail(const TrailTask* __restrict__ tas
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
       const int kq = min(k0 + lk + 2 * q, Tk.k1 - 1);
-      const long col = (long)kq * nr;
+      const long col = colofs(nr, kq) - kq;
       va[q] = Fa[col];
       vb[q] = Fb[col];
       vd[q] = Dg[first + kq];
@@ -400,7 +410,7 @@ __global__ __launch_bounds__(256) void k_trail(const TrailTask* __restrict__ tas
       for (int r = 0; r < 4; ++r) {
         const int i = Tk.ti + wr + mi * 16 + Mfma<T>::row(lane, r);
         ok[ni][r] = j < Tk.chi && i < nr && i >= j && i >= Tk.rlo;
-        adr[ni][r] = ok[ni][r] ? base + (long)j * nr + i : base;
+        adr[ni][r] = ok[ni][r] ? base + fpos(nr, i, j) : base;
         old[ni][r] = F[adr[ni][r]];
       }
     }
@@ -489,7 +499,8 @@ __global__ __launch_bounds__(256) void k_fwd(
   const int rc = rv ? r : nr - 1;  // clamped, always in bounds
   const bool diag = rb < ncb;
   const int kc = diag ? min(SB, nc - r0) : 0;
-  for (int c = wv; c < kc; c += 4) Ld[lane * LDSP + c] = (double)F[base + (long)(r0 + c) * nr + rc];
+  // (entries above the diagonal are never used: read the diagonal instead)
+  for (int c = wv; c < kc; c += 4) Ld[lane * LDSP + c] = (double)F[base + fpos(nr, max(rc, r0 + c), r0 + c)];
   // Children's update vectors first: they do not depend on this front's own
   // blocks, so gathering them here keeps them off the front's dependency
   // chain.  Every thread waits for a share of the children, every wave sums a
@@ -511,10 +522,9 @@ __global__ __launch_bounds__(256) void k_fwd(
   for (int cb = wv; cb < ncbk; cb += 4) {
     const int cbase = cb * SB;
     const int cend = min(SB, nc - cbase);
-    const T* Fc = F + base + (long)cbase * nr + rc;
     double Lb[SB];
 #pragma unroll
-    for (int c = 0; c < SB; ++c) Lb[c] = (c < cend) ? (double)Fc[(long)c * nr] : 0.0;
+    for (int c = 0; c < SB; ++c) Lb[c] = (c < cend) ? (double)F[base + fpos(nr, rc, cbase + c)] : 0.0;
     if (cb > ready) {
       if (lane == 0) {
         int* fl = &flags[(flag_ofs[s] + ncbk - 1) * FSTRIDE];  // newest block this task needs
@@ -604,7 +614,8 @@ __global__ __launch_bounds__(256) void k_bwd(
   const long base = fofs[s];
   const int c0 = cb * SB, kc = min(SB, nc - c0);
   const int ncb = (nc + SB - 1) / SB;
-  for (int c = wv; c < kc; c += 4) Ld[lane * LDSP + c] = (lane < kc) ? (double)F[base + (long)(c0 + c) * nr + c0 + lane] : 0.0;
+  for (int c = wv; c < kc; c += 4)
+    Ld[lane * LDSP + c] = (lane < kc) ? (double)F[base + fpos(nr, c0 + max(lane, c), c0 + c)] : 0.0;
   if (tid == 0 && parent[s] >= 0) wait_count(&done[parent[s] * FSTRIDE], nblkb[parent[s]], err);
   __syncthreads();
   double acc[SB];
@@ -616,11 +627,11 @@ __global__ __launch_bounds__(256) void k_bwd(
     const int R = rstart + q * SB;
     const int rr = R + lane;
     const bool rv = rr < nr;
-    const T* Lr = F + base + (long)c0 * nr + (rv ? rr : nr - 1);
+    const int rrc = rv ? rr : nr - 1;
     constexpr int H = SB / 2;
     double Lb[H];  // first half of the block, loaded before the flag is polled
 #pragma unroll
-    for (int c = 0; c < H; ++c) Lb[c] = (c < kc) ? (double)Lr[(long)c * nr] : 0.0;
+    for (int c = 0; c < H; ++c) Lb[c] = (c < kc) ? (double)F[base + fpos(nr, rrc, c0 + c)] : 0.0;
     if (lane == 0 && R < nc) {
       // pivot rows [R, min(R+SB, nc)) belong to blocks b > cb of this front
       const int b0 = R / SB, b1 = (min(R + SB, nc) - 1) / SB;
@@ -636,7 +647,7 @@ __global__ __launch_bounds__(256) void k_bwd(
 #pragma unroll
     for (int c = 0; c < H; ++c) acc[c] += Lb[c] * xv;
 #pragma unroll
-    for (int c = 0; c < H; ++c) Lb[c] = (H + c < kc) ? (double)Lr[(long)(H + c) * nr] : 0.0;
+    for (int c = 0; c < H; ++c) Lb[c] = (H + c < kc) ? (double)F[base + fpos(nr, rrc, c0 + H + c)] : 0.0;
 #pragma unroll
     for (int c = 0; c < H; ++c) acc[H + c] += Lb[c] * xv;
   }
@@ -696,7 +707,7 @@ void DeviceKKT::setup(const LowerPattern& K, int nx, hipStream_t stream) {
     nr[s] = P.nr(s);
     nc[s] = P.sn_ncol[s];
     first[s] = P.sn_first[s];
-    fofs[s + 1] = fofs[s] + (long)nr[s] * nr[s];
+    fofs[s + 1] = fofs[s] + colofs(nr[s], nr[s]);
   }
   // K entries grouped by permuted column (= column of the lower permuted entry)
   std::vector<int> kcount(N + 1, 0);
@@ -728,7 +739,7 @@ void DeviceKKT::setup(const LowerPattern& K, int nx, hipStream_t stream) {
         int rloc = (int)(it - rb), cloc = lo - P.sn_first[s];
         int dst = fill[lo]++;
         ksrc[dst] = k;
-        kdst[dst] = fofs[s] + (long)cloc * nr[s] + rloc;
+        kdst[dst] = fofs[s] + fpos(nr[s], rloc, cloc);
         kdiag[dst] = (K.rowind[k] == c) ? (c < nx ? 1 : -1) : 0;
       }
   }
@@ -916,18 +927,26 @@ void DeviceKKT::build_schedule() {
     for (int t = P.level_ptr[l]; t < P.level_ptr[l + 1]; ++t) {
       int s = P.level_sn[t];
       int nr = P.nr(s);
-      int cb = std::max(1, std::min(nr, 16384 / std::max(1, nr)));
-      for (int j0 = 0; j0 < nr; j0 += cb) {
-        int j1 = std::min(nr, j0 + cb);
-        AsmTask T{s, j0, j1, (int)seg_h_.size(), 0};
-        for (int q = P.child_ptr[s]; q < P.child_ptr[s + 1]; ++q) {
-          int c = P.child[q];
-          const int* rb = rmap.data() + rmap_ofs[c];
-          const int* re = rmap.data() + rmap_ofs[c + 1];
-          int i0 = (int)(std::lower_bound(rb, re, j0) - rb);
-          int i1 = (int)(std::lower_bound(rb, re, j1) - rb);
-          if (i1 > i0) seg_h_.push_back({c, i0, i1});
+      const int cb = std::max(1, std::min(nr, 16384 / std::max(1, nr)));
+      const int ntask = (nr + cb - 1) / cb;
+      // split every child's row map (sorted parent positions) into the
+      // column chunks of the tasks: O(sum of child update sizes + tasks)
+      std::vector<std::vector<AsmSeg>> per(ntask);
+      for (int q = P.child_ptr[s]; q < P.child_ptr[s + 1]; ++q) {
+        const int c = P.child[q];
+        const int* rb = rmap.data() + rmap_ofs[c];
+        const int* re = rmap.data() + rmap_ofs[c + 1];
+        const int nu = (int)(re - rb);
+        for (int i = 0; i < nu;) {
+          const int t = rb[i] / cb;
+          const int iend = (int)(std::lower_bound(rb + i, re, (t + 1) * cb) - rb);
+          per[t].push_back({c, i, iend});
+          i = iend;
         }
+      }
+      for (int t = 0; t < ntask; ++t) {
+        AsmTask T{s, t * cb, std::min(nr, (t + 1) * cb), (int)seg_h_.size(), 0};
+        seg_h_.insert(seg_h_.end(), per[t].begin(), per[t].end());
         T.seg_end = (int)seg_h_.size();
         asm_h_.push_back(T);
       }
@@ -1087,16 +1106,11 @@ void DeviceKKT::launch_factor(T* F, const double* Kx, double extra_p, double ext
   }
 }
 
-void DeviceKKT::enable_fp32(bool on) {
-  if (on && !Ff_.p) Ff_.alloc(F_.n);
-}
-
 void DeviceKKT::factorize_async(const double* Kx, double extra_p, double extra_a, int mode,
                                 double eps, double delta, int prec) {
   CX_HIP(hipMemsetAsync(fstat_.p, 0, 2 * sizeof(int), st_));
   if (prec == 1) {
-    enable_fp32(true);
-    launch_factor<float>(Ff_.p, Kx, extra_p, extra_a, mode, eps, delta);
+    launch_factor<float>(reinterpret_cast<float*>(F_.p), Kx, extra_p, extra_a, mode, eps, delta);
   } else {
     launch_factor<double>(F_.p, Kx, extra_p, extra_a, mode, eps, delta);
   }
@@ -1131,7 +1145,7 @@ void DeviceKKT::solve_async(const double* rhs, double* x) {
   if (prec_ == 1)
     hipLaunchKernelGGL(k_fwd<float>, dim3(nfwd_), dim3(256), 0, st_, fwd_.p, tick, fofs_.p, nr_.p, nc_.p,
                        first_.p, rowptr_.p, gptr_.p, gidx_.p, uofs_.p, child_ptr_.p, child_.p, nblk_.p,
-                       flag_ofs_.p, flags_.p, done_.p, gen_, (const float*)Ff_.p, xp_.p, U_.p, serr_.p);
+                       flag_ofs_.p, flags_.p, done_.p, gen_, reinterpret_cast<const float*>(F_.p), xp_.p, U_.p, serr_.p);
   else
     hipLaunchKernelGGL(k_fwd<double>, dim3(nfwd_), dim3(256), 0, st_, fwd_.p, tick, fofs_.p, nr_.p, nc_.p,
                        first_.p, rowptr_.p, gptr_.p, gidx_.p, uofs_.p, child_ptr_.p, child_.p, nblk_.p,
@@ -1142,7 +1156,7 @@ void DeviceKKT::solve_async(const double* rhs, double* x) {
   if (prec_ == 1)
     hipLaunchKernelGGL(k_bwd<float>, dim3(nbwd_), dim3(256), 0, st_, bwd_.p, tick + FSTRIDE, fofs_.p, nr_.p,
                      nc_.p, first_.p, rowptr_.p, rows_.p, parent_.p, nblkb_.p, flag_ofs_.p, flags_.p,
-                     done_.p + (size_t)ns * FSTRIDE, gen_, (const float*)Ff_.p, D_.p, xp_.p, serr_.p);
+                     done_.p + (size_t)ns * FSTRIDE, gen_, reinterpret_cast<const float*>(F_.p), D_.p, xp_.p, serr_.p);
   else
     hipLaunchKernelGGL(k_bwd<double>, dim3(nbwd_), dim3(256), 0, st_, bwd_.p, tick + FSTRIDE, fofs_.p, nr_.p,
                      nc_.p, first_.p, rowptr_.p, rows_.p, parent_.p, nblkb_.p, flag_ofs_.p, flags_.p,

@@ -48,7 +48,6 @@ class DeviceKKT {
   // FP32 MFMA trailing updates); pivots and the solve sweeps stay FP64.
   void factorize_async(const double* Kx, double extra_p, double extra_a, int mode, double eps,
                        double delta, int prec = 0);
-  void enable_fp32(bool on);
   int factor_precision() const { return prec_; }
   FactorStats factor_stats();  // synchronizes the stream
 
@@ -109,8 +108,7 @@ class DeviceKKT {
   DevBuf<int> small_;
   DevBuf<PanelTask> panel_;
   DevBuf<TrailTask> trail_;
-  DevBuf<double> F_;   // fronts (FP64)
-  DevBuf<float> Ff_;   // fronts (FP32 factorization), allocated on first use
+  DevBuf<double> F_;   // lower-packed fronts: FP64 factor, or the FP32 factor in its first half
   int prec_ = 0;       // precision of the current factorization
   DevBuf<double> D_;   // pivots, permuted order
   DevBuf<int> fstat_;  // [0] zero pivot, [1] nreg
