@@ -152,8 +152,8 @@ void cx_get_kkt_stats(const cx_solver* h, double* stats8) {
   if (h && stats8) h->impl->kkt_stats(stats8);
 }
 
-void cx_get_profile(const cx_solver* h, double* out10) {
-  if (h && out10) h->impl->profile(out10);
+void cx_get_profile(const cx_solver* h, double* out13) {
+  if (h && out13) h->impl->profile(out13);
 }
 
 int cx_get_kernel_stats(const cx_solver* h, double* out, int max_kinds) {

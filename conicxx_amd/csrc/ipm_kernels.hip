@@ -477,6 +477,88 @@ void launch_csr_mv(int rows, const int* ptr, const int* col, const int* src, con
   CX_LAUNCH_CHECK();
 }
 
+// ---------------------------------------------------------------------------
+// Krylov helpers (GMRES on the KKT system with the FP32 factorization as the
+// preconditioner): block inner products, Gram-Schmidt update, combination
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_mdot_partial(int n, int k, const double* __restrict__ V, long ld,
+                                                      const double* __restrict__ w, double* partial) {
+  __shared__ double red[4][KRY_MAX];
+  double acc[KRY_MAX];
+#pragma unroll
+  for (int t = 0; t < KRY_MAX; ++t) acc[t] = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const double wi = w[i];
+#pragma unroll
+    for (int t = 0; t < KRY_MAX; ++t)
+      if (t < k) acc[t] += V[t * ld + i] * wi;
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int t = 0; t < KRY_MAX; ++t) {
+    if (t < k) {
+      double v = acc[t];
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+      if (lane == 0) red[wv][t] = v;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < k)
+    partial[threadIdx.x * gridDim.x + blockIdx.x] =
+        (red[0][threadIdx.x] + red[1][threadIdx.x]) + (red[2][threadIdx.x] + red[3][threadIdx.x]);
+}
+
+// one block per vector: h[t] = sum of its partials
+__global__ __launch_bounds__(256) void k_mdot_final(int nb, const double* __restrict__ partial, double* h) {
+  __shared__ double red[256];
+  const int t = blockIdx.x;
+  double v = 0.0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) v += partial[t * nb + b];
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) h[t] = red[0];
+}
+
+__global__ __launch_bounds__(256) void k_mgs_update(int n, int k, const double* __restrict__ V, long ld,
+                                                    const double* __restrict__ h, double* w) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double acc = w[i];
+  for (int t = 0; t < k; ++t) acc -= h[t] * V[t * ld + i];
+  w[i] = acc;
+}
+
+__global__ __launch_bounds__(256) void k_multi_axpy(int n, int k, const double* __restrict__ Z, long ld,
+                                                    KryCoef y, double* x) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double acc = x[i];
+  for (int t = 0; t < k; ++t) acc += y.c[t] * Z[t * ld + i];
+  x[i] = acc;
+}
+
+void launch_mdot(int n, int k, const double* V, long ld, const double* w, double* partial, double* h,
+                 hipStream_t st) {
+  if (k <= 0 || k > KRY_MAX) return;
+  hipLaunchKernelGGL(k_mdot_partial, dim3(KRY_BLOCKS), dim3(256), 0, st, n, k, V, ld, w, partial);
+  hipLaunchKernelGGL(k_mdot_final, dim3(k), dim3(256), 0, st, KRY_BLOCKS, partial, h);
+  CX_LAUNCH_CHECK();
+}
+void launch_mgs_update(int n, int k, const double* V, long ld, const double* h, double* w, hipStream_t st) {
+  if (n <= 0 || k <= 0) return;
+  hipLaunchKernelGGL(k_mgs_update, dim3(g1(n)), dim3(256), 0, st, n, k, V, ld, h, w);
+  CX_LAUNCH_CHECK();
+}
+void launch_multi_axpy(int n, int k, const double* Z, long ld, const KryCoef& y, double* x, hipStream_t st) {
+  if (n <= 0 || k <= 0) return;
+  hipLaunchKernelGGL(k_multi_axpy, dim3(g1(n)), dim3(256), 0, st, n, k, Z, ld, y, x);
+  CX_LAUNCH_CHECK();
+}
+
 void launch_axpby(int n, double a, const double* x, double b, double* y, hipStream_t st) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_axpby, dim3(g1(n)), dim3(256), 0, st, n, a, x, b, y);

@@ -79,6 +79,17 @@ void launch_lin2(int n, double a, const double* x, double b, const double* y, do
 void launch_lin3(int n, double a, const double* x, double b, const double* y, double c,
                  const double* z, double* out, hipStream_t st);
 void launch_scale(int n, double a, double* x, hipStream_t st);
+// Krylov helpers over k <= KRY_MAX vectors of length n stored ld apart (V + t*ld)
+constexpr int KRY_MAX = 32;
+constexpr int KRY_BLOCKS = 240;
+struct KryCoef { double c[KRY_MAX]; };
+// h[t] = V_t . w  (partial: KRY_MAX * KRY_BLOCKS scratch)
+void launch_mdot(int n, int k, const double* V, long ld, const double* w, double* partial, double* h,
+                 hipStream_t st);
+// w -= sum_t h[t] V_t  (h on the device)
+void launch_mgs_update(int n, int k, const double* V, long ld, const double* h, double* w, hipStream_t st);
+// x += sum_t y.c[t] Z_t
+void launch_multi_axpy(int n, int k, const double* Z, long ld, const KryCoef& y, double* x, hipStream_t st);
 void launch_copy(int n, const double* x, double* y, hipStream_t st);
 // Kx[slot[i]] = val[src[i]] + add (src < 0 -> add only)
 void launch_scatter(int n, const int* slot, const int* src, const double* val, double add,

@@ -522,9 +522,18 @@ __global__ __launch_bounds__(256) void k_fwd(
   for (int cb = wv; cb < ncbk; cb += 4) {
     const int cbase = cb * SB;
     const int cend = min(SB, nc - cbase);
+    // All 64 columns are read unconditionally (row rc lies below every one of
+    // them, so past the pivot columns they are in-bounds entries of the
+    // update block) and the unused ones dropped afterwards: every load of the
+    // block is in flight together.
+    const T* Lr = F + base + fpos(nr, rc, cbase);
+    const long step = nr - cbase - 1;  // fpos(rc, j+1) - fpos(rc, j) = nr - j - 1
+    T raw[SB];
+#pragma unroll
+    for (int c = 0; c < SB; ++c) raw[c] = Lr[(long)c * step - (c * (c - 1)) / 2];
     double Lb[SB];
 #pragma unroll
-    for (int c = 0; c < SB; ++c) Lb[c] = (c < cend) ? (double)F[base + fpos(nr, rc, cbase + c)] : 0.0;
+    for (int c = 0; c < SB; ++c) Lb[c] = (c < cend) ? (double)raw[c] : 0.0;
     if (cb > ready) {
       if (lane == 0) {
         int* fl = &flags[(flag_ofs[s] + ncbk - 1) * FSTRIDE];  // newest block this task needs
@@ -629,9 +638,20 @@ __global__ __launch_bounds__(256) void k_bwd(
     const bool rv = rr < nr;
     const int rrc = rv ? rr : nr - 1;
     constexpr int H = SB / 2;
+    // row rrc lies below columns c0 .. c0+kc-1; reads past them are clamped to
+    // the last one and dropped, so every load of a half is in flight together
+    const T* Lr = F + base + fpos(nr, rrc, c0);
+    const long step = nr - c0 - 1;  // fpos(rrc, j+1) - fpos(rrc, j) = nr - j - 1
+    auto lofs = [&](int c) {
+      const int cc = min(c, kc - 1);
+      return (long)cc * step - ((long)cc * (cc - 1)) / 2;
+    };
+    T raw[H];
+#pragma unroll
+    for (int c = 0; c < H; ++c) raw[c] = Lr[lofs(c)];
     double Lb[H];  // first half of the block, loaded before the flag is polled
 #pragma unroll
-    for (int c = 0; c < H; ++c) Lb[c] = (c < kc) ? (double)F[base + fpos(nr, rrc, c0 + c)] : 0.0;
+    for (int c = 0; c < H; ++c) Lb[c] = (c < kc) ? (double)raw[c] : 0.0;
     if (lane == 0 && R < nc) {
       // pivot rows [R, min(R+SB, nc)) belong to blocks b > cb of this front
       const int b0 = R / SB, b1 = (min(R + SB, nc) - 1) / SB;
@@ -647,7 +667,9 @@ __global__ __launch_bounds__(256) void k_bwd(
 #pragma unroll
     for (int c = 0; c < H; ++c) acc[c] += Lb[c] * xv;
 #pragma unroll
-    for (int c = 0; c < H; ++c) Lb[c] = (H + c < kc) ? (double)F[base + fpos(nr, rrc, c0 + H + c)] : 0.0;
+    for (int c = 0; c < H; ++c) raw[c] = Lr[lofs(H + c)];
+#pragma unroll
+    for (int c = 0; c < H; ++c) Lb[c] = (H + c < kc) ? (double)raw[c] : 0.0;
 #pragma unroll
     for (int c = 0; c < H; ++c) acc[H + c] += Lb[c] * xv;
   }
@@ -913,6 +935,10 @@ void DeviceKKT::build_schedule() {
     // algorithmic flops: 2 * K * (#updated lower elements in the tile)
     double cnt = 0;
     int jmax = std::min(tj + TM, chi);
+    if (ti >= jmax - 1 && ti >= rlo) {  // tile entirely on or below the diagonal
+      fl += 2.0 * (k1 - k0) * (double)(jmax - tj) * (double)std::max(0, std::min(ti + TM, nr) - ti);
+      return;
+    }
     for (int j = tj; j < jmax; ++j) {
       int i0 = std::max({ti, j, rlo});
       int i1 = std::min(ti + TM, nr);

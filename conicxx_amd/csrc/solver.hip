@@ -51,6 +51,7 @@ Solver::Solver() {
 
 Solver::~Solver() {
   if (host_red_) (void)hipHostFree(host_red_);
+  if (gm_host_) (void)hipHostFree(gm_host_);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -334,8 +335,23 @@ bool Solver::try_factor(double ep, double ea) {
   return std::isfinite(fs.min_abs_d) && fs.min_abs_d > st_.dynamic_reg_eps;
 }
 
+void Solver::leave_fp32(double rel) {
+  fp32_ = false;
+  prof_fp64_fallback_++;
+  prof_fallback_rel_ = rel;
+  prof_fallback_iter_ = iterations_;
+}
+
 bool Solver::factor_retry() {
   double ep = 0.0, ea = 0.0;
+  if (fp32_) {
+    if (try_factor(ep, ea)) {
+      factorized_ = true;
+      last_p_ = ep, last_a_ = ea;
+      return true;
+    }
+    leave_fp32(NAN);  // the FP32 factorization broke down: FP64 from here on
+  }
   for (int att = 0; att < 18; ++att) {
     if (try_factor(ep, ea)) {
       factorized_ = true;
@@ -350,6 +366,12 @@ bool Solver::factor_retry() {
 }
 
 bool Solver::escalate(const double* rhs, double* x) {
+  if (fp32_) {  // a non-finite FP32 solve: FP64 from here on
+    leave_fp32(NAN);
+    if (!factor_retry()) return false;
+    kkt_.solve_async(rhs, x);
+    if (count_nonfinite(x, N_) == 0) return true;
+  }
   double ep = last_p_, ea = last_a_;
   for (int att = 0; att < 10; ++att) {
     ep = std::max(ep * 1000.0, st_.dynamic_reg_delta);
@@ -404,11 +426,11 @@ double Solver::kkt_solve(const double* rhs, double* x) {
     rr = reduce(B2)[0];
   }
   double rel = std::sqrt(rr) / rn;
-  // An FP32 factorization is a preconditioner for FP64 refinement: it gets
-  // more steps, and when the residual still misses refine_tol the solve falls
-  // back to an FP64 factorization for the rest of this solve().
-  const int rmax = fp32_ ? std::max(st_.refine_max_iter, FP32_REFINE_MAX) : st_.refine_max_iter;
-  for (int it = 0; it < rmax && rel > st_.refine_tol; ++it) {
+  // An FP32 factorization is the preconditioner of FP64 GMRES on K; when the
+  // residual misses the acceptance bound below, the solve falls back to an
+  // FP64 factorization for the rest of this solve().
+  if (fp32_) rel = gmres_refine(rhs, x, rn, rel);
+  for (int it = 0; !fp32_ && it < st_.refine_max_iter && rel > st_.refine_tol; ++it) {
     kkt_.solve_async(r_.p, dsol_.p);
     prof_nsolve_++;
     launch_axpby(N_, 1.0, dsol_.p, 1.0, x, stream_);
@@ -416,16 +438,100 @@ double Solver::kkt_solve(const double* rhs, double* x) {
     launch_lin2(N_, 1.0, rhs, -1.0, t_.p, r_.p, stream_);
     RedBatch B3{};
     B3.op[B3.nops++] = dot_op(r_.p, r_.p, N_);
-    const double prev = rel;
     rel = std::sqrt(reduce(B3)[0]) / rn;
-    if (fp32_ && !(rel < 0.5 * prev)) break;  // FP32 refinement stalled
   }
   prof_solve_ms_ += (now_s() - t0) * 1e3;
-  if (fp32_ && !(rel <= st_.refine_tol)) {
-    fp32_ = false;
-    prof_fp64_fallback_++;
+  // An FP32 solve is accepted when GMRES reached refine_tol, or stopped within
+  // FP32_ACCEPT_REL: a residual no FP64 solve with refine_max_iter steps
+  // beats on the same ill-conditioned late-iteration systems.
+  const double accept = std::max(st_.refine_tol, FP32_ACCEPT_REL);
+  if (!fp32_ || rel <= accept) prof_max_rel_ = std::max(prof_max_rel_, rel);
+  if (fp32_ && !(rel <= accept)) {
+    leave_fp32(rel);
     if (!factor_retry()) return NAN;
     return kkt_solve(rhs, x);
+  }
+  return rel;
+}
+
+// Restarted GMRES (classical Gram-Schmidt applied twice, Givens rotations) on
+// K dx = r, right-preconditioned by the FP32 factorization M: the Krylov basis
+// V and the preconditioned vectors Z = M^-1 V stay on the device, the small
+// Hessenberg least-squares problem on the host.  Every residual and update is
+// FP64; each cycle ends with the true residual rhs - K x.
+double Solver::gmres_refine(const double* rhs, double* x, double rn, double rel) {
+  const int m = GM_RESTART;
+  const long ld = N_;
+  if (gm_v_.n < (size_t)(m + 1) * N_) gm_v_.alloc((size_t)(m + 1) * N_);
+  if (gm_z_.n < (size_t)m * N_) gm_z_.alloc((size_t)m * N_);
+  gm_part_.alloc((size_t)KRY_MAX * KRY_BLOCKS);
+  gm_h_.alloc(2 * KRY_MAX);
+  if (!gm_host_) CX_HIP(hipHostMalloc((void**)&gm_host_, 2 * KRY_MAX * sizeof(double), hipHostMallocDefault));
+  int solves = 0;
+  std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1);
+  auto h_at = [&](int i, int j) -> double& { return H[(size_t)j * (m + 1) + i]; };
+  while (std::isfinite(rel) && rel > st_.refine_tol && solves < GM_MAX_SOLVES) {
+    const double beta = rel * rn;  // r_ holds rhs - K x
+    double* V = gm_v_.p;
+    double* Z = gm_z_.p;
+    launch_lin2(N_, 1.0 / beta, r_.p, 0.0, r_.p, V, stream_);
+    std::fill(g.begin(), g.end(), 0.0);
+    g[0] = beta;
+    int k = 0;  // completed Arnoldi steps
+    bool breakdown = false;
+    while (k < m && solves < GM_MAX_SOLVES) {
+      const int j = k;
+      kkt_.solve_async(V + j * ld, Z + j * ld);
+      ++solves;
+      prof_nsolve_++;
+      double* w = V + (j + 1) * ld;
+      kkt_.spmv_async(Kx_.p, Z + j * ld, w);
+      for (int pass = 0; pass < 2; ++pass) {
+        launch_mdot(N_, j + 1, V, ld, w, gm_part_.p, gm_h_.p + pass * KRY_MAX, stream_);
+        launch_mgs_update(N_, j + 1, V, ld, gm_h_.p + pass * KRY_MAX, w, stream_);
+      }
+      CX_HIP(hipMemcpyAsync(gm_host_, gm_h_.p, 2 * KRY_MAX * sizeof(double), hipMemcpyDeviceToHost, stream_));
+      RedBatch B{};
+      B.op[B.nops++] = dot_op(w, w, N_);
+      const double hn = std::sqrt(reduce(B)[0]);  // synchronizes the stream
+      for (int i = 0; i <= j; ++i) h_at(i, j) = gm_host_[i] + gm_host_[KRY_MAX + i];
+      h_at(j + 1, j) = hn;
+      if (!std::isfinite(hn)) return NAN;
+      for (int i = 0; i < j; ++i) {  // previous rotations
+        const double a = h_at(i, j), b = h_at(i + 1, j);
+        h_at(i, j) = cs[i] * a + sn[i] * b;
+        h_at(i + 1, j) = -sn[i] * a + cs[i] * b;
+      }
+      const double a = h_at(j, j), b = h_at(j + 1, j);
+      const double den = std::hypot(a, b);
+      cs[j] = den > 0 ? a / den : 1.0;
+      sn[j] = den > 0 ? b / den : 0.0;
+      h_at(j, j) = den;
+      h_at(j + 1, j) = 0.0;
+      g[j + 1] = -sn[j] * g[j];
+      g[j] = cs[j] * g[j];
+      k = j + 1;
+      if (hn <= 1e-300 * beta) {
+        breakdown = true;
+        break;
+      }
+      launch_scale(N_, 1.0 / hn, w, stream_);
+      if (std::fabs(g[k]) <= st_.refine_tol * rn) break;
+    }
+    (void)breakdown;
+    // y = R^-1 g, x += Z y
+    KryCoef y{};
+    for (int i = k - 1; i >= 0; --i) {
+      double v = g[i];
+      for (int c = i + 1; c < k; ++c) v -= h_at(i, c) * y.c[c];
+      y.c[i] = h_at(i, i) != 0.0 ? v / h_at(i, i) : 0.0;
+    }
+    launch_multi_axpy(N_, k, Z, ld, y, x, stream_);
+    kkt_.spmv_async(Kx_.p, x, t_.p);
+    launch_lin2(N_, 1.0, rhs, -1.0, t_.p, r_.p, stream_);
+    RedBatch B{};
+    B.op[B.nops++] = dot_op(r_.p, r_.p, N_);
+    rel = std::sqrt(reduce(B)[0]) / rn;
   }
   return rel;
 }
@@ -764,6 +870,8 @@ int Solver::solve() {  // solver_impl.cpp:367-466
   bw_valid_ = false;
   fp32_ = st_.factor_precision == CX_FACTOR_FP32_REFINED;
   prof_nfact32_ = prof_fp64_fallback_ = 0;
+  prof_fallback_iter_ = -1;
+  prof_max_rel_ = prof_fallback_rel_ = 0;
   iterations_ = 0;
   objective_ = pres_ = dres_ = gap_ = mu_out_ = 0;
   prof_fact_ms_ = prof_solve_ms_ = 0;
@@ -793,6 +901,7 @@ int Solver::solve() {  // solver_impl.cpp:367-466
     int final_status = CX_MAX_ITERATIONS, iterations = 0;
     for (int iter = 0; iter < st_.max_iter; ++iter) {
       iterations = iter + 1;
+      iterations_ = iter;  // current iteration, for the profile
       launch_cone_scaling(cone_, s_.p, z_.p, stream_);
       if (!kkt_update_scaling() || !constant_solve()) {
         final_status = CX_NUMERICAL_ERROR;
@@ -924,8 +1033,8 @@ void Solver::kkt_stats(double* out8) const {
   out8[7] = P.nnzL_exact;
 }
 
-void Solver::profile(double* out10) const {
-  double* out8 = out10;
+void Solver::profile(double* out13) const {
+  double* out8 = out13;
   const DeviceKKT::KindStats& t = kkt_.kind_stats(DeviceKKT::T_TRAIL);
   out8[0] = prof_fact_ms_;
   out8[1] = prof_solve_ms_;
@@ -935,8 +1044,11 @@ void Solver::profile(double* out10) const {
   out8[5] = t.launches ? t.ms / t.launches : 0.0;
   out8[6] = (double)t.launches;
   out8[7] = t.work;
-  out10[8] = prof_nfact32_;
-  out10[9] = prof_fp64_fallback_;
+  out13[8] = prof_nfact32_;
+  out13[9] = prof_fp64_fallback_;
+  out13[10] = prof_max_rel_;
+  out13[11] = prof_fallback_rel_;
+  out13[12] = prof_fallback_iter_;
 }
 
 int Solver::kernel_stats(double* out, int max_kinds) const {

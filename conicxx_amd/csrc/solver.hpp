@@ -34,7 +34,7 @@ class Solver {
   int get_solution(double* x, double* s, double* z, double* obj, cx_info* info) const;
   void dims(int* n, int* m) const { *n = n_; *m = m_; }
   void kkt_stats(double* out8) const;
-  void profile(double* out10) const;
+  void profile(double* out13) const;
   int kernel_stats(double* out, int max_kinds) const;
   int backward(const double* wx, const double* ws, const double* wz, double* gP, double* gq,
                double* gA, double* gb);
@@ -46,7 +46,11 @@ class Solver {
   bool factor_retry();
   bool try_factor(double ep, double ea);
   bool escalate(const double* rhs, double* x);
+  void leave_fp32(double rel);  // FP64 factorizations for the rest of this solve()
   double kkt_solve(const double* rhs, double* x);
+  // GMRES on K with the FP32 factorization as right preconditioner, from x
+  // with residual r_ (relative residual rel); returns the final relative residual
+  double gmres_refine(const double* rhs, double* x, double rn, double rel);
   // IPM pieces (solver_impl.cpp)
   bool initial_point();
   void shift_cold(double* v);
@@ -128,8 +132,13 @@ class Solver {
   // FP32 factorization with FP64 refinement (factor_precision); falls back to
   // FP64 for the rest of a solve when refinement misses refine_tol
   bool fp32_ = false;
-  int prof_nfact32_ = 0, prof_fp64_fallback_ = 0;
-  static constexpr int FP32_REFINE_MAX = 10;
+  int prof_nfact32_ = 0, prof_fp64_fallback_ = 0, prof_fallback_iter_ = -1;
+  double prof_max_rel_ = 0, prof_fallback_rel_ = 0;
+  static constexpr int GM_RESTART = 20;     // Arnoldi steps per GMRES cycle
+  static constexpr int GM_MAX_SOLVES = 20;  // preconditioner applications per KKT solve
+  static constexpr double FP32_ACCEPT_REL = 1e-9;
+  DevBuf<double> gm_v_, gm_z_, gm_part_, gm_h_;
+  double* gm_host_ = nullptr;  // pinned: two Gram-Schmidt passes' coefficients
 };
 
 }  // namespace cx

@@ -277,10 +277,11 @@ class Solver:
         return dict(zip(keys, out.tolist()))
 
     def profile(self):
-        out = np.zeros(10)
+        out = np.zeros(13)
         _lib.lib().cx_get_profile(self._h, out.ctypes.data_as(_lib._f64p))
         keys = ["factor_ms", "solve_ms", "other_ms", "factorizations", "kkt_solves",
-                "trail_avg_ms", "trail_launches", "trail_flops", "fp32_factorizations", "fp64_fallbacks"]
+                "trail_avg_ms", "trail_launches", "trail_flops", "fp32_factorizations", "fp64_fallbacks",
+                "max_kkt_rel_residual", "fallback_rel_residual", "fallback_iteration"]
         return dict(zip(keys, out.tolist()))
 
     KERNEL_CLASSES = ["trail", "diag", "trsm", "assemble", "small", "fwd", "bwd"]

@@ -126,10 +126,12 @@ void cx_get_kkt_stats(const cx_solver* h, double* stats8);
  * [3] number of factorizations, [4] number of KKT solves (incl. refinement steps),
  * [5] average device duration of the dense trailing-update kernel (ms), [6] its
  * launches, [7] its algorithmic flops, [8] FP32 factorizations, [9] FP32 -> FP64
- * fallbacks (factor_precision = CX_FACTOR_FP32_REFINED).  [5..7] need
- * CONICXX_KERNEL_TIMING=1 in the environment at setup().  No counterpart in the
- * reference. */
-void cx_get_profile(const cx_solver* h, double* out10);
+ * fallbacks (factor_precision = CX_FACTOR_FP32_REFINED), [10] largest final
+ * relative residual |rhs - K x| / |rhs| of an accepted KKT solve, [11] that of
+ * the FP32 solve that triggered the fallback (0 if none), [12] the IPM
+ * iteration of the fallback (-1 if none).  [5..7] need CONICXX_KERNEL_TIMING=1
+ * in the environment at setup().  No counterpart in the reference. */
+void cx_get_profile(const cx_solver* h, double* out13);
 
 /* Device time per kernel class of the last solve, measured with HIP events on the
  * solver's stream (CONICXX_KERNEL_TIMING=1 at setup()): for class k, out[3k..3k+2]
