@@ -319,7 +319,7 @@ struct Mfma<float> {
 };
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_trail(const TrailTask* __restrict__ tasks,
+__global__ __launch_bounds__(256, 2) void k_trail(const TrailTask* __restrict__ tasks,
                                                const long* __restrict__ fofs,
                                                const int* __restrict__ nrv,
                                                const int* __restrict__ firstv,
@@ -329,6 +329,7 @@ __global__ __launch_bounds__(256) void k_trail(const TrailTask* __restrict__ tas
   constexpr int PER = KC * TM / 256;  // staged elements per thread and operand
   __shared__ T As[KC * LDA];
   __shared__ T Bs[KC * LDA];
+  __shared__ T Ds[KC];
   const TrailTask Tk = tasks[blockIdx.x];
   const int s = Tk.s, nr = nrv[s], first = firstv[s];
   const long base = fofs[s];
@@ -347,48 +348,44 @@ __global__ __launch_bounds__(256) void k_trail(const TrailTask* __restrict__ tas
   // the last row of the front, also below them)
   const T* Fa = F + base + (iok ? gi : nr - 1);
   const T* Fb = F + base + (jok ? gj : nr - 1);
-  // register prefetch of the first chunk
+  // Register prefetch of the next chunk, straight from clamped (always valid)
+  // addresses with no select, so all loads of a chunk are in flight together
+  // and only the two operand arrays stay live across the MFMA loop; the
+  // out-of-range values are zeroed on the way into LDS.  D of the chunk goes
+  // to LDS too and scales the B fragments as they are read.
   T ra[PER], rb[PER];
-  // Branch-free staging: every load goes to a clamped, valid address and the
-  // out-of-range values are zeroed by a select afterwards, so all PER loads of
-  // a chunk are in flight together (a guarded load would make the compiler
-  // wait for each one at the join of its branch).
+  T dn = T(0);
   auto load_chunk = [&](int k0) {
-    T va[PER], vb[PER];
-    double vd[PER];
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
       const int kq = min(k0 + lk + 2 * q, Tk.k1 - 1);
       const long col = colofs(nr, kq) - kq;
-      va[q] = Fa[col];
-      vb[q] = Fb[col];
-      vd[q] = Dg[first + kq];
+      ra[q] = Fa[col];
+      rb[q] = Fb[col];
     }
-#pragma unroll
-    for (int q = 0; q < PER; ++q) {
-      const bool kin = k0 + lk + 2 * q < Tk.k1;
-      ra[q] = (kin && iok) ? va[q] : T(0);
-      rb[q] = (kin && jok) ? vb[q] * (T)vd[q] : T(0);
-    }
+    if (tid < KC) dn = (T)Dg[first + min(k0 + tid, Tk.k1 - 1)];
   };
   load_chunk(Tk.k0);
   for (int k0 = Tk.k0; k0 < Tk.k1; k0 += KC) {
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
       const int kk = lk + 2 * q;
-      As[kk * LDA + lrow] = ra[q];
-      Bs[kk * LDA + lrow] = rb[q];
+      const bool kin = k0 + kk < Tk.k1;
+      As[kk * LDA + lrow] = (kin && iok) ? ra[q] : T(0);
+      Bs[kk * LDA + lrow] = (kin && jok) ? rb[q] : T(0);
     }
+    if (tid < KC) Ds[tid] = dn;
     __syncthreads();
     if (k0 + KC < Tk.k1) load_chunk(k0 + KC);  // in flight while the MFMAs run
 #pragma unroll
     for (int kk = 0; kk < KC; kk += 4) {
       T af[4], bf[4];
       const int kr = kk + (lane >> 4);
+      const T dk = Ds[kr];
 #pragma unroll
       for (int mi = 0; mi < 4; ++mi) af[mi] = As[kr * LDA + wr + mi * 16 + (lane & 15)];
 #pragma unroll
-      for (int ni = 0; ni < 4; ++ni) bf[ni] = Bs[kr * LDA + wc + ni * 16 + (lane & 15)];
+      for (int ni = 0; ni < 4; ++ni) bf[ni] = Bs[kr * LDA + wc + ni * 16 + (lane & 15)] * dk;
 #pragma unroll
       for (int mi = 0; mi < 4; ++mi)
 #pragma unroll
