@@ -158,13 +158,81 @@ def differentiation_form(K=1024):
     }
 
 
+FP32_MFMA_PEAK_TFLOPS = 157.3
+FACTOR_KINDS = ("trail", "diag", "trsm", "assemble", "small")
+
+
+def precision_form(scale=1.0, accuracy_scale=0.1):
+    """Precision form on the config-4 generator (n=100000, KKT order N=245000 at
+    scale 1): device time of one numeric factorization in FP32 (f32 MFMA
+    trailing updates) against FP64 (f64 MFMA) on the same KKT matrices (the
+    initial point and the first IPM iteration), and the full-solve error of
+    FP32 + FP64 GMRES refinement against the FP64 solve on the same generator
+    at `accuracy_scale`."""
+    import numpy as np
+    from conicxx_amd import ConeSpec, FactorPrecision, Settings, Solver, Status
+    from conicxx_amd import problems as PR
+
+    p = PR.config4(scale=scale)
+    cs = ConeSpec(p.zero_dim, p.nonneg_dim, p.soc_dims)
+    s = Solver()
+    t0 = time.perf_counter()
+    assert s.setup(p.P, p.q, p.A, p.b, cs, Settings(max_iter=1, warm_start=False)), "setup failed"
+    setup_s = time.perf_counter() - t0
+    kkt = s.kkt_stats()
+    res = {}
+    for name, prec in (("fp64", FactorPrecision.FP64), ("fp32", FactorPrecision.FP32Refined)):
+        s.setSettings(Settings(max_iter=1, warm_start=False, factor_precision=prec))
+        s.solve()
+        ks, pr = s.kernel_stats(), s.profile()
+        nf = max(1, int(pr["factorizations"]))
+        tr = ks.get("trail", {"ms": 0.0, "work": 0.0})
+        res[name] = {"factor_device_ms": sum(ks[k]["ms"] for k in FACTOR_KINDS if k in ks) / nf,
+                     "factorizations": nf, "fp32_factorizations": int(pr["fp32_factorizations"]),
+                     "trail_tflops": tr["work"] / max(1e-9, tr["ms"]) / 1e9,
+                     "per_factorization_ms": {k: ks[k]["ms"] / nf for k in FACTOR_KINDS if k in ks}}
+        print(f"[bench] precision {name}: {res[name]['factor_device_ms']:.0f} ms per factorization", file=sys.stderr, flush=True)
+    del s
+    # accuracy of the whole method: FP32 + refinement vs FP64, same problem
+    q = PR.config4(scale=accuracy_scale)
+    cq = ConeSpec(q.zero_dim, q.nonneg_dim, q.soc_dims)
+    sols, profs = {}, {}
+    for name, prec in (("fp64", FactorPrecision.FP64), ("fp32", FactorPrecision.FP32Refined)):
+        sv = Solver()
+        assert sv.setup(q.P, q.q, q.A, q.b, cq, Settings(warm_start=False, factor_precision=prec))
+        sols[name] = sv.solve()
+        profs[name] = sv.profile()
+    a, b = sols["fp64"], sols["fp32"]
+    rel = lambda u, v: float(np.linalg.norm(u - v) / max(1e-300, np.linalg.norm(v)))
+    return {
+        "workload": (f"config4 at scale {scale}: n={p.n}, m={p.m}, KKT N={kkt['N']:.0f}, nnz(L)={kkt['nnz_L']:.3g}, "
+                     f"factor flops={kkt['flops']:.3g}, largest front={kkt['max_front']:.0f}; setup {setup_s:.1f} s"),
+        "fp64": res["fp64"], "fp32": res["fp32"],
+        "factorization_speedup": res["fp64"]["factor_device_ms"] / max(1e-9, res["fp32"]["factor_device_ms"]),
+        "trail_roofline": {"fp64_frac": res["fp64"]["trail_tflops"] / FP64_MFMA_PEAK_TFLOPS,
+                           "fp32_frac": res["fp32"]["trail_tflops"] / FP32_MFMA_PEAK_TFLOPS},
+        "accuracy": {
+            "workload": f"config4 at scale {accuracy_scale}: n={q.n}, m={q.m}, full solves to the default 1e-8 tolerances",
+            "status": {"fp64": a.status.name, "fp32": b.status.name},
+            "iterations": {"fp64": a.info.iterations, "fp32": b.info.iterations},
+            "fp32_factorizations": int(profs["fp32"]["fp32_factorizations"]),
+            "fp64_factorizations_in_fp32_run": int(profs["fp32"]["factorizations"] - profs["fp32"]["fp32_factorizations"]),
+            "fp32_fallback_iteration": int(profs["fp32"]["fallback_iteration"]),
+            "rel_err_vs_fp64": {"x": rel(b.x, a.x), "s": rel(b.s, a.s), "z": rel(b.z, a.z),
+                                "objective": abs(b.objective - a.objective) / max(1e-300, abs(a.objective))},
+            "solve_s": {"fp64": a.info.solve_time_s, "fp32": b.info.solve_time_s},
+        },
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-forms", action="store_true", help="skip the batched and differentiation forms")
+    ap.add_argument("--no-forms", action="store_true",
+                    help="skip the batched, differentiation and precision forms")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the plain solve of one QP runs on a single device "
@@ -257,6 +325,7 @@ def main():
     if not args.no_forms:
         out["batched"] = batched_form()
         out["differentiation"] = differentiation_form()
+        out["precision"] = precision_form()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(kkt["flops"])
     print(json.dumps(out))
