@@ -42,6 +42,7 @@ struct BatchArgs {
   // settings
   double tol_feas, tol_gap_abs, tol_gap_rel, tol_infeas, sreg_p, sreg_a, dyn_eps, dyn_delta;
   double refine_tol, step_frac;
+  double adj_tol;  // refinement target of the adjoint (backward) solve
   int max_iter, refine_max_iter, mode, warm_start;
   int lds_doubles;
   // structure (entry indices global, row/column values problem-relative)
@@ -113,6 +114,7 @@ struct PV {
   double cur_a;  // (2,2) regularization of the current factorization
   int nfact, nsolve, nkkt;  // factorizations, reduced solves and KKT solves (with refinement)
   long long t_fact, t_solve, t_res;  // device clock ticks in factorization, reduced solves, residuals
+  long long t_kkt_last;              // device clock ticks of the latest KKT solve (with refinement)
   double* stage;      // LDS staging for panels when M is in global memory (else null)
   long stage_cap;
   const double* tcorr;  // SOC tangent correction of the adjoint matrix (null in the IPM)
@@ -683,7 +685,11 @@ __device__ bool escalate(const BatchArgs& A, PV& p, const double* rhs, double* x
 }
 
 // KktSystem::solve (kkt_system.cpp:296-311)
-__device__ double kkt_solve(const BatchArgs& A, PV& p, const double* rhs, double* x, double* colk, double* red) {
+// tol: relative residual target; steps past min_steps stop once a step no
+// longer halves the residual
+__device__ double kkt_solve(const BatchArgs& A, PV& p, const double* rhs, double* x, double* colk, double* red,
+                           double tol, int min_steps) {
+  const long long t_kkt0 = wall_clock64();
   ++p.nkkt;
   reduced_solve(A, p, rhs, x);
   const double nonfin = count_nonfinite(p.N, x, red);
@@ -704,19 +710,21 @@ __device__ double kkt_solve(const BatchArgs& A, PV& p, const double* rhs, double
   // The reduced factorization is a weaker preconditioner than a full LDL^T
   // once (Hs + eI)^-1 has entries near 1/e: refinement then continues past
   // refine_max_iter (up to BATCH_REFINE_MAX steps) while it still reduces the
-  // residual, so each solve meets refine_tol as the reference's solves do.
+  // residual, so each solve meets its target (refine_tol for the IPM's own
+  // solves, as the reference's solves do).
   const int rmax = A.refine_max_iter > 0 ? max(A.refine_max_iter, BATCH_REFINE_MAX) : 0;
-  for (int it = 0; it < rmax && rel > A.refine_tol; ++it) {
+  for (int it = 0; it < rmax && rel > tol; ++it) {
     reduced_solve(A, p, p.res, p.dsol);
     vaxpy(p.N, 1.0, p.dsol, x);
     kkt_residual(A, p, rhs, x, p.res);
     double r = 0.0;
     for (int i = threadIdx.x; i < p.N; i += BG) r += p.res[i] * p.res[i];
     const double rel_new = sqrt(gsum1(r, red)) / rn;
-    const bool stalled = it >= A.refine_max_iter && rel_new > 0.5 * rel;
+    const bool stalled = it >= min_steps && rel_new > 0.5 * rel;
     rel = rel_new;
     if (stalled) break;
   }
+  p.t_kkt_last = wall_clock64() - t_kkt0;
   return rel;
 }
 
@@ -754,17 +762,17 @@ __device__ bool initial_point(const BatchArgs& A, PV& p, double* colk, double* r
   if (p.pnnz == 0) {
     vfill(n, 0.0, p.rhs);
     vcopy(m, p.b, p.rhs + n);
-    kkt_solve(A, p, p.rhs, p.sol, colk, red);
+    kkt_solve(A, p, p.rhs, p.sol, colk, red, A.refine_tol, A.refine_max_iter);
     vcopy(n, p.sol, p.x);
     vlin2(m, -1.0, p.sol + n, 0.0, p.sol + n, p.s);
     vlin2(n, -1.0, p.q, 0.0, p.q, p.rhs);
     vfill(m, 0.0, p.rhs + n);
-    kkt_solve(A, p, p.rhs, p.sol, colk, red);
+    kkt_solve(A, p, p.rhs, p.sol, colk, red, A.refine_tol, A.refine_max_iter);
     vcopy(m, p.sol + n, p.z);
   } else {
     vlin2(n, -1.0, p.q, 0.0, p.q, p.rhs);
     vcopy(m, p.b, p.rhs + n);
-    kkt_solve(A, p, p.rhs, p.sol, colk, red);
+    kkt_solve(A, p, p.rhs, p.sol, colk, red, A.refine_tol, A.refine_max_iter);
     vcopy(n, p.sol, p.x);
     vcopy(m, p.sol + n, p.z);
     vlin2(m, -1.0, p.sol + n, 0.0, p.sol + n, p.s);
@@ -780,7 +788,7 @@ __device__ bool initial_point(const BatchArgs& A, PV& p, double* colk, double* r
 __device__ bool constant_solve(const BatchArgs& A, PV& p, double* colk, double* red) {  // solver_impl.cpp:165-175
   vlin2(p.n, -1.0, p.q, 0.0, p.q, p.rhs);
   vcopy(p.m, p.b, p.rhs + p.n);
-  kkt_solve(A, p, p.rhs, p.sol, colk, red);
+  kkt_solve(A, p, p.rhs, p.sol, colk, red, A.refine_tol, A.refine_max_iter);
   vcopy(p.n, p.sol, p.x1);
   vcopy(p.m, p.sol + p.n, p.z1);
   pmul(A, p, p.x1, p.Px1);
@@ -840,7 +848,7 @@ __device__ bool affine_step(const BatchArgs& A, PV& p, double* colk, double* red
   const int n = p.n, m = p.m;
   vcopy(n, p.rx, p.rhs);
   vlin2(m, 1.0, p.s, -1.0, p.rz, p.rhs + n);
-  kkt_solve(A, p, p.rhs, p.sol, colk, red);
+  kkt_solve(A, p, p.rhs, p.sol, colk, red, A.refine_tol, A.refine_max_iter);
   if (count_nonfinite(p.N, p.sol, red) != 0) return false;
   const double dtau_rhs = p.rtau, dkappa_rhs = p.tau * p.kappa;
   if (!tau_elim(A, p, p.sol, p.sol + n, dtau_rhs, dkappa_rhs, &p.dtaua, red)) return false;
@@ -856,7 +864,7 @@ __device__ bool combined_step(const BatchArgs& A, PV& p, double sigma, double mu
   const int n = p.n, m = p.m;  // solver_impl.cpp:233-286
   combined_rhs(p, p.z, p.dsa, p.dza, p.rz, sigma * mu, 1.0 - sigma, p.dsk, p.rhs + n);
   vlin2(n, 1.0 - sigma, p.rx, 0.0, p.rx, p.rhs);
-  kkt_solve(A, p, p.rhs, p.sol, colk, red);
+  kkt_solve(A, p, p.rhs, p.sol, colk, red, A.refine_tol, A.refine_max_iter);
   if (count_nonfinite(p.N, p.sol, red) != 0) return false;
   const double dtau_rhs = (1.0 - sigma) * p.rtau;
   const double dkappa_rhs = p.tau * p.kappa + p.dtaua * p.dkappaa - sigma * mu;
@@ -891,7 +899,7 @@ __device__ void setup_view(const BatchArgs& A, const BatchDesc& D, PV& p, double
   p.co = D.co, p.so = D.so, p.ko = D.ko, p.pfp = D.pfp, p.acp = D.acp, p.arp = D.arp;
   p.cur_a = 0.0;
   p.nfact = p.nsolve = p.nkkt = 0;
-  p.t_fact = p.t_solve = p.t_res = 0;
+  p.t_fact = p.t_solve = p.t_res = p.t_kkt_last = 0;
   p.ec = D.ec;
   p.q = A.q + D.xo;
   p.b = A.b + D.co;
@@ -1118,6 +1126,7 @@ __device__ void solve_problem(const BatchArgs& A, int pid, double* lds, double* 
     inf8[9] = (double)p.t_solve;
     inf8[10] = (double)p.t_res;
     inf8[11] = p.nkkt;
+    inf8[15] = (double)p.t_kkt_last;  // the final iteration's KKT solve
     A.status[pid] = status;
     A.iters[pid] = iterations;
     if (status == CX_SOLVED && A.warm_start) A.have_warm[pid] = 1;
@@ -1184,20 +1193,24 @@ __device__ void backward_problem(const BatchArgs& A, int pid, double* lds, doubl
   __syncthreads();
   neg_hs_plus(p, p.wm, p.wm2, p.rhs + n);
   soc_tangent_apply(p, p.wm, p.rhs + n);
-  kkt_solve(A, p, p.rhs, p.sol, colk, red);
+  // the adjoint stops at its own target or at the first step that stalls
+  kkt_solve(A, p, p.rhs, p.sol, colk, red, A.adj_tol, 1);
   const double* u = p.sol;
   const double* v = p.sol + n;
-  for (int j = threadIdx.x; j < n; j += BG) {
-    A.gq[D.xo + j] = -p.ec * ed[j] * u[j];
-    for (int e = A.ac_ptr[p.acp + j]; e < A.ac_ptr[p.acp + j + 1]; ++e) {
+  // gradient assembly: one wave per column of A / row of P, lanes along its entries
+  for (int j = threadIdx.x; j < n; j += BG) A.gq[D.xo + j] = -p.ec * ed[j] * u[j];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int j = wv; j < n; j += BG / 64) {
+    const double uj = u[j], xj = p.x[j], dj = ed[j];
+    for (int e = A.ac_ptr[p.acp + j] + lane; e < A.ac_ptr[p.acp + j + 1]; e += 64) {
       const int i = A.ac_row[e];
-      A.gA[e] = -(p.z[i] * u[j] + v[i] * p.x[j]) * ee[i] * ed[j];
+      A.gA[e] = -(p.z[i] * uj + v[i] * xj) * ee[i] * dj;
     }
-    for (int e = A.pf_ptr[p.pfp + j]; e < A.pf_ptr[p.pfp + j + 1]; ++e) {
+    for (int e = A.pf_ptr[p.pfp + j] + lane; e < A.pf_ptr[p.pfp + j + 1]; e += 64) {
       const int c = A.pf_col[e];
       if (c < j) continue;  // each stored upper entry once, from its row
-      A.gP[A.pf_src[e]] = (c == j) ? -u[j] * p.x[j] * p.ec * ed[j] * ed[j]
-                                   : -(u[j] * p.x[c] + u[c] * p.x[j]) * p.ec * ed[j] * ed[c];
+      A.gP[A.pf_src[e]] = (c == j) ? -uj * xj * p.ec * dj * dj
+                                   : -(uj * p.x[c] + u[c] * xj) * p.ec * dj * ed[c];
     }
   }
   for (int i = threadIdx.x; i < m; i += BG) A.gb[D.co + i] = ee[i] * v[i];
@@ -1501,6 +1514,9 @@ void BatchSolver::build_args(void* argp) {
   A.tol_infeas = set_.tol_infeas, A.sreg_p = set_.static_reg_P, A.sreg_a = set_.static_reg_A;
   A.dyn_eps = set_.dynamic_reg_eps, A.dyn_delta = set_.dynamic_reg_delta;
   A.refine_tol = set_.refine_tol, A.step_frac = set_.max_step_fraction;
+  // The adjoint solve differentiates a solution known to the IPM tolerances:
+  // its residual target sits two orders below them.
+  A.adj_tol = std::max(set_.refine_tol, 1e-2 * std::min(set_.tol_feas, set_.tol_gap_rel));
   A.max_iter = set_.max_iter, A.refine_max_iter = set_.refine_max_iter;
   A.mode = set_.linear_solver == CX_LS_REGULARIZED ? PV_REGULARIZE : PV_CHECK;
   A.warm_start = set_.warm_start;
@@ -1684,21 +1700,22 @@ int BatchSolver::autotune(int reps) {
   return 0;
 }
 
-void BatchSolver::stats(int k, double* out4) {
-  for (int i = 0; i < 11; ++i) out4[i] = 0.0;
+void BatchSolver::stats(int k, double* out12) {
+  for (int i = 0; i < 12; ++i) out12[i] = 0.0;
   if (!ready_ || k < 0 || k >= np_) return;
   fetch();
-  out4[0] = hiters_[k];
-  out4[1] = hinfo_[16L * k + 5];
-  out4[2] = hinfo_[16L * k + 6];
-  out4[3] = hinfo_[16L * k + 7] / (clock_khz_ * 1e-3);  // microseconds
-  out4[4] = hinfo_[16L * k + 8] / (clock_khz_ * 1e-3);
-  out4[5] = hinfo_[16L * k + 9] / (clock_khz_ * 1e-3);
-  out4[6] = hinfo_[16L * k + 10] / (clock_khz_ * 1e-3);
-  out4[7] = hinfo_[16L * k + 11];
-  out4[8] = hinfo_[16L * k + 12] / (clock_khz_ * 1e-3);
-  out4[9] = hinfo_[16L * k + 13] / (clock_khz_ * 1e-3);
-  out4[10] = hinfo_[16L * k + 14];
+  out12[0] = hiters_[k];
+  out12[1] = hinfo_[16L * k + 5];
+  out12[2] = hinfo_[16L * k + 6];
+  out12[3] = hinfo_[16L * k + 7] / (clock_khz_ * 1e-3);  // microseconds
+  out12[4] = hinfo_[16L * k + 8] / (clock_khz_ * 1e-3);
+  out12[5] = hinfo_[16L * k + 9] / (clock_khz_ * 1e-3);
+  out12[6] = hinfo_[16L * k + 10] / (clock_khz_ * 1e-3);
+  out12[7] = hinfo_[16L * k + 11];
+  out12[8] = hinfo_[16L * k + 12] / (clock_khz_ * 1e-3);
+  out12[9] = hinfo_[16L * k + 13] / (clock_khz_ * 1e-3);
+  out12[10] = hinfo_[16L * k + 14];
+  out12[11] = hinfo_[16L * k + 15] / (clock_khz_ * 1e-3);
 }
 
 void BatchSolver::tuning(double* out4) const {

@@ -192,8 +192,9 @@ void cx_batch_get_tuning(const cx_batch* h, double* out4);
  * factorizations, reduced solves incl. refinement, device microseconds its
  * workgroup spent on it, of which in factorizations, in triangular solves, in
  * refinement residuals, KKT solves, device microseconds of its last backward,
- * of which in its KKT solve, reduced solves of that KKT solve). */
-void cx_batch_get_stats(cx_batch* h, int k, double* out11);
+ * of which in its KKT solve, reduced solves of that KKT solve, device
+ * microseconds of the final IPM iteration's KKT solve). */
+void cx_batch_get_stats(cx_batch* h, int k, double* out12);
 /* Differentiation of the batch (no counterpart in the reference).  With it on,
  * every Solved problem's forward pass also factors its adjoint KKT matrix (the
  * final scaling with the SOC tangent correction, as cx_backward uses), so that
