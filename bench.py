@@ -165,7 +165,7 @@ def differentiation_form(K=1024):
 
 
 FP32_MFMA_PEAK_TFLOPS = 157.3
-FACTOR_KINDS = ("trail", "diag", "trsm", "assemble", "small")
+FACTOR_KINDS = ("trail", "panel", "assemble", "small")
 
 
 def precision_form(scale=1.0, accuracy_scale=0.1):

@@ -1,16 +1,18 @@
 // Device multifrontal LDL^T for the quasi-definite KKT matrix (gfx950).
 //
-// Fronts: supernode s owns a dense nr x nr column-major front F_s (only the
-// lower triangle is meaningful).  Its first nc columns hold, after the
-// partial factorization, L (strictly below the diagonal) and D (diagonal);
-// the trailing (nr-nc)^2 block holds the update matrix passed to the parent.
+// Fronts: supernode s owns a dense nr x nr front F_s, lower triangle stored
+// packed by columns.  Its first nc columns hold, after the partial
+// factorization, L (strictly below the diagonal) and D (diagonal); the
+// trailing lower (nr-nc)^2 block holds the update matrix passed to the parent.
 //
 // Factorization schedule, per level of the assembly tree (leaves first):
 //   k_assemble      zero-fill, scatter K entries (+ diagonal bump), extend-add
 //                   of the children's update matrices (pull, deterministic)
 //   k_factor_small  fronts with nr <= 64, factored whole in LDS
-//   k_diag          64-column sub-panel: LDL^T of the diagonal block in LDS
-//   k_trsm          the rows below it, L21 = R L11^-T D^-1 (64-row chunks, LDS)
+//   k_panel         64-column sub-panel, one workgroup per 64-row chunk below
+//                   it: LDL^T of the diagonal block in LDS (every workgroup of
+//                   the sub-panel factors it, the first one stores it), then
+//                   L21 = R L11^-T D^-1 for its chunk
 //   k_trail         lower-triangle trailing update C -= L D L^T on FP64 MFMA
 //                   (v_mfma_f64_16x16x4f64), 128x128 tiles
 // Triangular solves: one ticketed launch per sweep over the whole tree with
@@ -142,6 +144,54 @@ __device__ void ldl_lds(T* A, int n, int nc, const double* sgn, int mode, double
   }
 }
 
+// Wave-level broadcast of lane l's value.
+template <typename T>
+__device__ inline T lane_bcast(T v, int l);
+template <>
+__device__ inline double lane_bcast<double>(double v, int l) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                          __builtin_amdgcn_readlane(__double2loint(v), l));
+}
+template <>
+__device__ inline float lane_bcast<float>(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+// The same partial LDL^T as ldl_lds for n <= 64 rows, by one wave with no
+// workgroup barrier: lane i keeps row i in registers, pivot k is read from
+// lane k, and the unscaled column k goes through LDS (colk) so that every lane
+// reads entry j of it as a broadcast.  Called by the first wave only (threads
+// 0..63); the caller synchronizes the workgroup before and after.
+template <typename T>
+__device__ __forceinline__ void ldl_wave(T* A, int n, int nc, const double* sgn, int mode, double eps, double delta,
+                         int* stats, bool count, T* colk) {
+  const int lane = threadIdx.x & 63;
+  T a[64];
+#pragma unroll
+  for (int j = 0; j < 64; ++j) a[j] = (lane < n && j <= lane) ? A[lane * LDSP + j] : T(0);
+#pragma unroll
+  for (int k = 0; k < 64; ++k) {
+    if (k < nc) {
+      const T d = (T)pivot_fix((double)lane_bcast(a[k], k), sgn[k], mode, eps, delta, stats, count && lane == 0);
+      const T c = a[k];  // unscaled entry (lane, k)
+      colk[lane] = c;
+      __builtin_amdgcn_wave_barrier();
+      const T l = c / d;
+      a[k] = (lane == k) ? d : (lane > k ? l : a[k]);
+      // Unpredicated: lane i only needs j <= i, and for j > i the registers
+      // hold entries above the diagonal that are never stored.  Lanes <= k
+      // update only such entries (with values read from their own upper
+      // registers), so nothing they compute reaches the factor.
+#pragma unroll
+      for (int j = k + 1; j < 64; ++j) a[j] -= l * colk[j];  // colk[j]: entry (j, k) = l_jk d_k
+      __builtin_amdgcn_wave_barrier();  // every read of colk before the next write
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 64; ++j)
+    if (lane < n && j <= lane) A[lane * LDSP + j] = a[j];
+}
+
 // ---------------------------------------------------------------------------
 // assembly: zero-fill + K scatter + extend-add of children update matrices
 // ---------------------------------------------------------------------------
@@ -220,47 +270,31 @@ __global__ __launch_bounds__(256) void k_factor_small(
 }
 
 // ---------------------------------------------------------------------------
-// sub-panel, step 1: LDL^T of the diagonal block, one workgroup per front
+// sub-panels
 // ---------------------------------------------------------------------------
 constexpr int PANEL_ROWS = 64;
 
+// Sub-panel [a, a+w) of a front: LDL^T of the w x w diagonal block, then the
+// rows [r0, r1) below it, L21 = R L11^-T D^-1.  Every workgroup of the
+// sub-panel factors the (assembled, updated) diagonal block itself, with the
+// same deterministic code, so no launch separates the two steps; the one
+// whose chunk starts right below the block (r0 == a + w; a task with no rows
+// exists for a sub-panel at the bottom of its front) stores it, its D and its
+// pivot statistics.
 template <typename T>
-__global__ __launch_bounds__(256) void k_diag(
+__global__ __launch_bounds__(256) void k_panel(
     const PanelTask* __restrict__ tasks, const long* __restrict__ fofs, const int* __restrict__ nrv,
-    const int* __restrict__ firstv, const double* __restrict__ sgn, T* F, double* Dg, int mode,
+    const int* __restrict__ firstv, const double* __restrict__ sgn, T* F, T* Fd, double* Dg, int mode,
     double eps, double delta, int* stats) {
   __shared__ T L11[NB * LDSP];
+  __shared__ T X[PANEL_ROWS * LDSP];
   __shared__ T colk[NB];
+  __shared__ T dinv[NB];
   const PanelTask Tk = tasks[blockIdx.x];
   const int s = Tk.s, nr = nrv[s], first = firstv[s], a = Tk.a, w = Tk.w;
   const long base = fofs[s];
   const int tid = threadIdx.x;
-  for (int idx = tid; idx < w * w; idx += blockDim.x) {
-    int j = idx / w, i = idx % w;
-    if (i >= j) L11[i * LDSP + j] = F[base + fpos(nr, a + i, a + j)];
-  }
-  __syncthreads();
-  ldl_lds(L11, w, w, sgn + first + a, mode, eps, delta, stats, true, colk);
-  for (int idx = tid; idx < w * w; idx += blockDim.x) {
-    int j = idx / w, i = idx % w;
-    if (i >= j) F[base + fpos(nr, a + i, a + j)] = L11[i * LDSP + j];
-  }
-  for (int k = tid; k < w; k += blockDim.x) Dg[first + a + k] = L11[k * LDSP + k];
-}
-
-// sub-panel, step 2: rows below the diagonal block, L21 = R L11^-T D^-1, one
-// 64-row chunk per workgroup (the factored block is read back from F)
-template <typename T>
-__global__ __launch_bounds__(256) void k_trsm(const PanelTask* __restrict__ tasks,
-                                              const long* __restrict__ fofs,
-                                              const int* __restrict__ nrv, T* F) {
-  __shared__ T L11[NB * LDSP];
-  __shared__ T X[PANEL_ROWS * LDSP];
-  __shared__ T dinv[NB];
-  const PanelTask Tk = tasks[blockIdx.x];
-  const int s = Tk.s, nr = nrv[s], a = Tk.a, w = Tk.w;
-  const long base = fofs[s];
-  const int tid = threadIdx.x;
+  const bool owner = Tk.r0 == a + w;
   for (int idx = tid; idx < w * w; idx += blockDim.x) {
     int j = idx / w, i = idx % w;
     if (i >= j) L11[i * LDSP + j] = F[base + fpos(nr, a + i, a + j)];
@@ -271,23 +305,71 @@ __global__ __launch_bounds__(256) void k_trsm(const PanelTask* __restrict__ task
     X[rr * LDSP + k] = F[base + fpos(nr, Tk.r0 + rr, a + k)];
   }
   __syncthreads();
-  for (int k = tid; k < w; k += blockDim.x) dinv[k] = T(1) / L11[k * LDSP + k];
-  // X L11^T = R (unit lower L11): four lanes per row share the inner sums
-  const int rr = tid >> 2, q = tid & 3;
-  if (rr < rows) {
-    for (int k = 1; k < w; ++k) {
-      T part = T(0);
-      for (int t = q; t < k; t += 4) part += X[rr * LDSP + t] * L11[k * LDSP + t];
-      part += __shfl_xor(part, 1, 64);
-      part += __shfl_xor(part, 2, 64);
-      if (q == 0) X[rr * LDSP + k] -= part;
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  if (tid < 64) ldl_wave(L11, w, w, sgn + first + a, mode, eps, delta, stats, owner, colk);
+  __syncthreads();
+  if (owner) {
+    // other workgroups of this launch may still be reading the unfactored
+    // block from F: the factored one is staged and stored by k_diag_store
+    T* dst = Fd + (long)Tk.blk * NB * NB;
+    for (int idx = tid; idx < w * w; idx += blockDim.x) {
+      int j = idx / w, i = idx % w;
+      if (i >= j) dst[idx] = L11[i * LDSP + j];
     }
+    for (int k = tid; k < w; k += blockDim.x) Dg[first + a + k] = L11[k * LDSP + k];
+  }
+  if (rows <= 0) return;
+  for (int k = tid; k < w; k += blockDim.x) dinv[k] = T(1) / L11[k * LDSP + k];
+  __syncthreads();  // every read of the factored block done before it is overwritten
+  // L11^-1 (unit lower) by the first wave, lane c computing column c:
+  // y_i = [i == c] - sum_{t<i} L11(i, t) y_t, exactly zero above the diagonal
+  // (rows and columns at or past w are never read below).  Column c is then
+  // stored as row c of the buffer, which so holds Inv^T.
+  if (tid < 64) {
+    T y[64];
+#pragma unroll
+    for (int i = 0; i < 64; ++i) {
+      T acc[4] = {(tid == i) ? T(1) : T(0), T(0), T(0), T(0)};  // four chains: latency, not issue, bound
+#pragma unroll
+      for (int t = 0; t < i; ++t) acc[t & 3] -= L11[i * LDSP + t] * y[t];
+      y[i] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+    }
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int i = 0; i < 64; ++i) L11[tid * LDSP + i] = y[i];
   }
   __syncthreads();
-  for (int idx = tid; idx < rows * w; idx += blockDim.x) {
-    int k = idx / rows, r2 = idx % rows;
-    F[base + fpos(nr, Tk.r0 + r2, a + k)] = X[r2 * LDSP + k] * dinv[k];
+  // L21 = R Inv^T D^-1: lane = row of the chunk, wave g = columns [16g, 16g+16)
+  const int r = tid & 63, g = tid >> 6;
+  if (r < rows) {
+    T acc[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) acc[q] = T(0);
+    for (int t = 0; t < w; ++t) {
+      const T x = X[r * LDSP + t];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) acc[q] += x * L11[t * LDSP + 16 * g + q];  // Inv(16g+q, t)
+    }
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int k = 16 * g + q;
+      if (k < w) F[base + fpos(nr, Tk.r0 + r, a + k)] = acc[q] * dinv[k];
+    }
+  }
+}
+
+// stores every staged diagonal block into its front, one workgroup per block
+template <typename T>
+__global__ __launch_bounds__(256) void k_diag_store(const PanelTask* __restrict__ slots,
+                                                    const long* __restrict__ fofs,
+                                                    const int* __restrict__ nrv, const T* __restrict__ Fd,
+                                                    T* F) {
+  const PanelTask Tk = slots[blockIdx.x];
+  const int s = Tk.s, nr = nrv[s], a = Tk.a, w = Tk.w;
+  const long base = fofs[s];
+  const T* src = Fd + (long)Tk.blk * NB * NB;
+  for (int idx = threadIdx.x; idx < w * w; idx += blockDim.x) {
+    int j = idx / w, i = idx % w;
+    if (i >= j) F[base + fpos(nr, a + i, a + j)] = src[idx];
   }
 }
 
@@ -912,6 +994,7 @@ void DeviceKKT::build_schedule() {
   seg_h_.clear();
   small_h_.clear();
   panel_h_.clear();
+  dslot_h_.clear();
   trail_h_.clear();
   trail_launch_flops_.clear();
   trail_flops_static_ = 0;
@@ -989,24 +1072,20 @@ void DeviceKKT::build_schedule() {
     for (int s : bigs) maxnc = std::max(maxnc, P.sn_ncol[s]);
     for (int P0 = 0; P0 < maxnc; P0 += PW) {
       for (int a0 = P0; a0 < P0 + PW && a0 < maxnc; a0 += NB) {
-        int pbeg = (int)panel_h_.size();
         int tbeg = (int)trail_h_.size();
         double fl = 0;
-        for (int s : bigs) {
-          int nc = P.sn_ncol[s];
-          if (a0 >= nc) continue;
-          int pend = std::min(P0 + PW, nc);
-          int w = std::min(NB, pend - a0);
-          panel_h_.push_back({s, a0, w, -1, -1});
-        }
-        if ((int)panel_h_.size() > pbeg) launches_.push_back({L_DIAG, pbeg, (int)panel_h_.size()});
         const int rbeg = (int)panel_h_.size();
         for (int s : bigs) {
           int nc = P.sn_ncol[s], nr = P.nr(s);
           if (a0 >= nc) continue;
           int pend = std::min(P0 + PW, nc);
           int w = std::min(NB, pend - a0);
-          for (int r0 = a0 + w; r0 < nr; r0 += PANEL_ROWS) panel_h_.push_back({s, a0, w, r0, std::min(nr, r0 + PANEL_ROWS)});
+          // the first task of a sub-panel (r0 == a0 + w) stages the diagonal block
+          const int blk = (int)dslot_h_.size();
+          dslot_h_.push_back({s, a0, w, 0, 0, blk});
+          if (a0 + w >= nr) panel_h_.push_back({s, a0, w, a0 + w, a0 + w, blk});
+          for (int r0 = a0 + w; r0 < nr; r0 += PANEL_ROWS)
+            panel_h_.push_back({s, a0, w, r0, std::min(nr, r0 + PANEL_ROWS), blk});
           int b = a0 + w;
           if (b < pend)
             for (int ti = b; ti < nr; ti += TM)
@@ -1041,7 +1120,9 @@ void DeviceKKT::build_schedule() {
   asm_.upload(asm_h_, st_);
   seg_.upload(seg_h_.empty() ? std::vector<AsmSeg>{{0, 0, 0}} : seg_h_, st_);
   small_.upload(small_h_.empty() ? std::vector<int>{0} : small_h_, st_);
-  panel_.upload(panel_h_.empty() ? std::vector<PanelTask>{{0, 0, 0, 0, 0}} : panel_h_, st_);
+  panel_.upload(panel_h_.empty() ? std::vector<PanelTask>{{0, 0, 0, 0, 0, 0}} : panel_h_, st_);
+  dslot_.upload(dslot_h_.empty() ? std::vector<PanelTask>{{0, 0, 0, 0, 0, 0}} : dslot_h_, st_);
+  Fd_.alloc(std::max<size_t>(1, dslot_h_.size()) * NB * NB);
   trail_.upload(trail_h_.empty() ? std::vector<TrailTask>{{0, 0, 0, 0, 0, 0, 0}} : trail_h_, st_);
 }
 
@@ -1106,16 +1187,11 @@ void DeviceKKT::launch_factor(T* F, const double* Kx, double extra_p, double ext
                            nr_.p, nc_.p, first_.p, sgn_.p, F, D_.p, mode, eps, delta, fstat_.p);
         tend(T_SMALL, 0.0);
         break;
-      case L_DIAG:
-        tbegin();
-        hipLaunchKernelGGL(k_diag<T>, dim3(n), dim3(256), 0, st_, panel_.p + L.beg, fofs_.p, nr_.p,
-                           first_.p, sgn_.p, F, D_.p, mode, eps, delta, fstat_.p);
-        tend(T_DIAG, 0.0);
-        break;
       case L_PANEL:
         tbegin();
-        hipLaunchKernelGGL(k_trsm<T>, dim3(n), dim3(256), 0, st_, panel_.p + L.beg, fofs_.p, nr_.p, F);
-        tend(T_TRSM, 0.0);
+        hipLaunchKernelGGL(k_panel<T>, dim3(n), dim3(256), 0, st_, panel_.p + L.beg, fofs_.p, nr_.p, first_.p,
+                           sgn_.p, F, reinterpret_cast<T*>(Fd_.p), D_.p, mode, eps, delta, fstat_.p);
+        tend(T_PANEL, 0.0);
         break;
       case L_TRAIL:
         tbegin();
@@ -1125,6 +1201,13 @@ void DeviceKKT::launch_factor(T* F, const double* Kx, double extra_p, double ext
         ++tl;
         break;
     }
+    CX_LAUNCH_CHECK();
+  }
+  if (!dslot_h_.empty()) {
+    tbegin();
+    hipLaunchKernelGGL(k_diag_store<T>, dim3((unsigned)dslot_h_.size()), dim3(256), 0, st_, dslot_.p, fofs_.p, nr_.p,
+                       reinterpret_cast<const T*>(Fd_.p), F);
+    tend(T_PANEL, 0.0);
     CX_LAUNCH_CHECK();
   }
 }

@@ -22,11 +22,13 @@ struct FactorStats {
 
 struct AsmTask { int s, j0, j1, seg_beg, seg_end; };
 struct AsmSeg { int child, i0, i1; };
-struct PanelTask { int s, a, w, r0, r1; };  // r0 < 0: diagonal-block task
+// rows [r0, r1) of sub-panel [a, a+w) of front s; the task with r0 == a + w
+// also stores the factored diagonal block, in slot blk of the staging buffer
+struct PanelTask { int s, a, w, r0, r1, blk; };
 struct TrailTask { int s, ti, tj, k0, k1, rlo, chi; };
 struct SolveTask { int s, blk; };
 
-enum LaunchKind { L_ASSEMBLE = 0, L_SMALL = 1, L_DIAG = 2, L_PANEL = 3, L_TRAIL = 4 };
+enum LaunchKind { L_ASSEMBLE = 0, L_SMALL = 1, L_PANEL = 3, L_TRAIL = 4 };
 struct Launch { int kind, beg, end; };
 
 // Pivot modes.
@@ -65,7 +67,7 @@ class DeviceKKT {
   // Per-kernel-class device timing with pooled HIP events recorded on this
   // stream around each launch (enabled by CONICXX_KERNEL_TIMING=1); events are
   // resolved only where the solver synchronizes anyway.
-  enum TimedKind { T_TRAIL = 0, T_DIAG, T_TRSM, T_ASM, T_SMALL, T_FWD, T_BWD, T_NKINDS };
+  enum TimedKind { T_TRAIL = 0, T_PANEL, T_ASM, T_SMALL, T_FWD, T_BWD, T_NKINDS };
   struct KindStats {
     double ms = 0, work = 0;  // work: algorithmic flops (factorization kinds) or bytes (sweeps)
     long launches = 0;
@@ -91,6 +93,7 @@ class DeviceKKT {
   std::vector<AsmSeg> seg_h_;
   std::vector<int> small_h_;
   std::vector<PanelTask> panel_h_;
+  std::vector<PanelTask> dslot_h_;  // one per sub-panel: where its diagonal block goes
   std::vector<TrailTask> trail_h_;
   double trail_flops_static_ = 0;
   std::vector<double> trail_launch_flops_;
@@ -106,7 +109,8 @@ class DeviceKKT {
   DevBuf<AsmTask> asm_;
   DevBuf<AsmSeg> seg_;
   DevBuf<int> small_;
-  DevBuf<PanelTask> panel_;
+  DevBuf<PanelTask> panel_, dslot_;
+  DevBuf<double> Fd_;  // factored diagonal blocks (NB x NB per sub-panel), stored into F after the sweep of launches
   DevBuf<TrailTask> trail_;
   DevBuf<double> F_;   // lower-packed fronts: FP64 factor, or the FP32 factor in its first half
   int prec_ = 0;       // precision of the current factorization

@@ -284,7 +284,7 @@ class Solver:
                 "max_kkt_rel_residual", "fallback_rel_residual", "fallback_iteration"]
         return dict(zip(keys, out.tolist()))
 
-    KERNEL_CLASSES = ["trail", "diag", "trsm", "assemble", "small", "fwd", "bwd"]
+    KERNEL_CLASSES = ["trail", "panel", "assemble", "small", "fwd", "bwd"]
 
     def kernel_stats(self):
         """Device time per kernel class of the last solve (CONICXX_KERNEL_TIMING=1)."""

@@ -136,10 +136,11 @@ void cx_get_profile(const cx_solver* h, double* out13);
 /* Device time per kernel class of the last solve, measured with HIP events on the
  * solver's stream (CONICXX_KERNEL_TIMING=1 at setup()): for class k, out[3k..3k+2]
  * = (milliseconds, launches, algorithmic work); work is flops for the trailing
- * update (class 0) and bytes for the forward/backward sweeps (classes 5, 6).
- * Classes: 0 trailing update, 1 diagonal block, 2 panel solve, 3 assembly,
- * 4 small fronts, 5 forward sweep, 6 backward sweep.  Returns the number of
- * classes written.  No counterpart in the reference. */
+ * update (class 0) and bytes for the forward/backward sweeps (classes 4, 5).
+ * Classes: 0 trailing update, 1 sub-panel (diagonal block LDL^T + panel solve,
+ * and the store of the diagonal blocks), 2 assembly, 3 small fronts, 4 forward
+ * sweep, 5 backward sweep.  Returns the number of classes written.  No
+ * counterpart in the reference. */
 int cx_get_kernel_stats(const cx_solver* h, double* out, int max_kinds);
 
 /* Backward (adjoint) pass of the last solve, which must have returned
