@@ -80,6 +80,11 @@ def batched_form(K=4096, timesteps=3, sample=32):
     b = BatchSolver()
     assert b.setup([(p.P, p.q, p.A, p.b, ConeSpec(p.zero_dim, p.nonneg_dim, p.soc_dims)) for p in probs], st)
     b.solve()  # warm-up: graph capture
+    t0 = time.perf_counter()
+    ncand = b.autotune(1)  # launch configuration timed on this batch (untimed here)
+    tune_s = time.perf_counter() - t0
+    print(f"[bench] batch autotune: {ncand} candidates in {tune_s:.1f} s -> {b.tuning()}", file=sys.stderr, flush=True)
+    b.solve()  # graph capture with the chosen configuration
     solved, wall, dev_ms = 0, 0.0, []
     for t in range(1, timesteps + 1):
         qs, bs = [], []
@@ -115,7 +120,8 @@ def batched_form(K=4096, timesteps=3, sample=32):
         "solved_fraction": solved / (K * timesteps),
         "plain_loop_qps": loop_qps,
         "plain_loop_sample": f"{sample} problems of the same batch, one plain Solver each, solve() timed",
-        "speedup_vs_plain_loop": qps / loop_qps, "tuning": b.tuning(), "generate_s": gen_s,
+        "speedup_vs_plain_loop": qps / loop_qps, "tuning": b.tuning(), "autotune_candidates": ncand,
+        "autotune_s": tune_s, "generate_s": gen_s,
     }
 
 

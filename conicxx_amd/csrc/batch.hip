@@ -1695,9 +1695,59 @@ int BatchSolver::get_solution(int k, double* x, double* s, double* z, double* ob
   return hstatus_[k];
 }
 
+// Launch-configuration autotuner.  A candidate is an LDS budget per
+// workgroup (all of it: one resident workgroup per CU with every factor that
+// fits kept on chip; or a half, third, quarter: more resident workgroups,
+// larger factors in global memory with their panels staged) and the matching
+// persistent grid (resident slots, then twice that).  Each candidate solves
+// this batch reps times from the same starting state (the warm-start flags
+// are restored before every trial and at the end); the fastest device time
+// wins.  Solutions left in place are those of the last trial.
 int BatchSolver::autotune(int reps) {
-  (void)reps;
-  return 0;
+  if (!ready_) return -1;
+  reps = std::max(1, reps);
+  int dev = 0, ncu = 0;
+  CX_HIP(hipGetDevice(&dev));
+  CX_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  std::vector<int> warm(np_);
+  have_warm_.download(warm.data(), np_, st_);
+  CX_HIP(hipStreamSynchronize(st_));
+  const long full = std::min<long>(max_lds_need_, LDS_MAX_BYTES / 8);
+  std::vector<BatchTuning> cand;
+  for (int per_cu = 1; per_cu <= 4; ++per_cu) {
+    const int lds = (int)std::min<long>(full, (LDS_MAX_BYTES / 8) / per_cu);
+    const int slots = ncu * std::max(1, (int)((LDS_MAX_BYTES / 8) / std::max(1, lds)));
+    for (int over = 1; over <= 2; ++over) {
+      BatchTuning t;
+      t.lds_doubles = lds;
+      t.grid = std::max(1, std::min(np_, slots * over));
+      bool dup = false;
+      for (const BatchTuning& c : cand) dup |= c.lds_doubles == t.lds_doubles && c.grid == t.grid;
+      if (!dup) cand.push_back(t);
+    }
+  }
+  const BatchTuning keep = tune_;
+  double best_ms = 1e300;
+  BatchTuning best = keep;
+  for (const BatchTuning& t : cand) {
+    tune_ = t;
+    if (gexec_) (void)hipGraphExecDestroy(gexec_), gexec_ = nullptr;
+    if (graph_) (void)hipGraphDestroy(graph_), graph_ = nullptr;
+    double ms = 1e300;
+    for (int r = 0; r < reps; ++r) {
+      have_warm_.upload(warm, st_);
+      if (solve(nullptr) < 0) break;
+      ms = std::min(ms, last_ms_);
+    }
+    if (ms < best_ms) best_ms = ms, best = t;
+  }
+  tune_ = best;
+  tuned_ = true;
+  if (gexec_) (void)hipGraphExecDestroy(gexec_), gexec_ = nullptr;
+  if (graph_) (void)hipGraphDestroy(graph_), graph_ = nullptr;
+  have_warm_.upload(warm, st_);
+  CX_HIP(hipStreamSynchronize(st_));
+  return (int)cand.size();
 }
 
 void BatchSolver::stats(int k, double* out12) {

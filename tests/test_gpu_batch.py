@@ -55,6 +55,27 @@ def test_batch_mixed_sizes_and_global_M(oracle):
         assert_parity(b.solution(k), oracle_runs(oracle, p), p)
 
 
+def test_batch_autotune_keeps_results():
+    """The autotuner times every candidate launch configuration (LDS budget per
+    workgroup, persistent grid) on the batch itself: all of them must give the
+    untuned results, and the warm-start state must be the one it found."""
+    probs = [PR.config2_problem(i) for i in range(40)] + [PR.config0(seed=0), PR.config3_problem(0)]
+    ref = batch_of(probs, Settings(warm_start=False))
+    ref.solve()
+    b = batch_of(probs, Settings(warm_start=False))
+    ntried = b.autotune(1)
+    t = b.tuning()
+    print("candidates", ntried, "chosen", t)
+    assert ntried >= 2 and t["autotuned"]
+    st = b.solve()
+    for k, p in enumerate(probs):
+        a, c = ref.solution(k), b.solution(k)
+        assert int(st[k]) == int(a.status), k
+        if a.status == Status.Solved:
+            err = np.abs(c.x - a.x).max() / max(1.0, np.abs(a.x).max())
+            assert err < 1e-6, (k, err)
+
+
 def test_batch_timesteps_with_update_data(oracle):
     """Per-timestep q/b updates replayed through the captured graph, each
     timestep cold-started (warm_start off)."""
