@@ -245,11 +245,18 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-forms", action="store_true",
                     help="skip the batched, differentiation and precision forms")
+    ap.add_argument("--form", choices=["batched", "differentiation", "precision"],
+                    help="run only this form and print its JSON object (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the plain solve of one QP runs on a single device "
               "(run without --gpus or with --gpus 1)", file=sys.stderr)
         sys.exit(2)
+
+    if args.form:
+        fn = {"batched": batched_form, "differentiation": differentiation_form, "precision": precision_form}
+        print(json.dumps({"form": args.form, **fn[args.form]()}))
+        return
 
     import numpy as np  # noqa: F401
     from conicxx_amd import ConeSpec, Settings, Solver, Status
