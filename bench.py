@@ -26,6 +26,9 @@ sys.path.insert(0, ROOT)
 os.environ.setdefault("CONICXX_KERNEL_TIMING", "1")
 
 HBM_PEAK_GBS = 8000.0
+# k_trail HBM bytes per launch on config 1 (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0
+# per the copy calibration, averaged over the launches of one solve)
+K_TRAIL_C1_TRAFFIC_BYTES = 4.28e8
 FP64_MFMA_PEAK_TFLOPS = 78.6
 
 
@@ -237,6 +240,24 @@ def precision_form(scale=1.0, accuracy_scale=0.1):
     }
 
 
+def copy_calibration(nbytes=1 << 30, reps=3):
+    """Three plain device-to-device copies of 1 GiB through the HIP runtime (its
+    own copy kernel; no torch kernels, which do not survive counter collection
+    here): in a rocprofv3 --pmc run their FETCH_SIZE / WRITE_SIZE per dispatch
+    give the scale of the counters against the known 1 GiB moved."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) == 0
+    for _ in range(reps):
+        assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -247,6 +268,9 @@ def main():
                     help="skip the batched, differentiation and precision forms")
     ap.add_argument("--form", choices=["batched", "differentiation", "precision"],
                     help="run only this form and print its JSON object (for profiling)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="end with three 1 GiB device-to-device copies: the reference kernel that "
+                         "scales the FETCH_SIZE / WRITE_SIZE counters of a profiled run")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the plain solve of one QP runs on a single device "
@@ -305,7 +329,10 @@ def main():
             ach = per_launch / avg_s / 1e12
             roof = {"kernel": "k_trail (FP64 MFMA trailing update)", "bound": "mfma",
                     "achieved": ach, "peak": FP64_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
-                    "frac": ach / FP64_MFMA_PEAK_TFLOPS, "traffic": None}
+                    "frac": ach / FP64_MFMA_PEAK_TFLOPS,
+                    # HBM bytes per launch on this workload from the FETCH_SIZE / WRITE_SIZE
+                    # counter passes, scaled by a 1 GiB copy (profiles/k_trail_config1_traffic.md)
+                    "traffic": K_TRAIL_C1_TRAFFIC_BYTES, "traffic_unit": "bytes per launch (counters)"}
         else:
             ach = per_launch / avg_s / 1e9 if per_launch else 0.0
             roof = {"kernel": f"k_{dom}", "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
@@ -348,6 +375,8 @@ def main():
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(kkt["flops"])
     print(json.dumps(out))
+    if args.copy_calibration:
+        copy_calibration()
 
 
 if __name__ == "__main__":
