@@ -401,7 +401,7 @@ struct Mfma<float> {
 };
 
 template <typename T>
-__global__ __launch_bounds__(256, 2) void k_trail(const TrailTask* __restrict__ tasks,
+__global__ __launch_bounds__(256, sizeof(T) == 4 ? 4 : 2) void k_trail(const TrailTask* __restrict__ tasks,
                                                const long* __restrict__ fofs,
                                                const int* __restrict__ nrv,
                                                const int* __restrict__ firstv,
