@@ -169,10 +169,12 @@ __device__ __forceinline__ void ldl_wave(T* A, int n, int nc, const double* sgn,
   T a[64];
 #pragma unroll
   for (int j = 0; j < 64; ++j) a[j] = (lane < n && j <= lane) ? A[lane * LDSP + j] : T(0);
+  const double sg = lane < nc ? sgn[lane] : 1.0;  // pivot signs, read from lane k at step k
 #pragma unroll
   for (int k = 0; k < 64; ++k) {
     if (k < nc) {
-      const T d = (T)pivot_fix((double)lane_bcast(a[k], k), sgn[k], mode, eps, delta, stats, count && lane == 0);
+      const T d = (T)pivot_fix((double)lane_bcast(a[k], k), lane_bcast(sg, k), mode, eps, delta, stats,
+                               count && lane == 0);
       const T c = a[k];  // unscaled entry (lane, k)
       colk[lane] = c;
       __builtin_amdgcn_wave_barrier();
@@ -295,14 +297,28 @@ __global__ __launch_bounds__(256) void k_panel(
   const long base = fofs[s];
   const int tid = threadIdx.x;
   const bool owner = Tk.r0 == a + w;
-  for (int idx = tid; idx < w * w; idx += blockDim.x) {
-    int j = idx / w, i = idx % w;
-    if (i >= j) L11[i * LDSP + j] = F[base + fpos(nr, a + i, a + j)];
-  }
   const int rows = Tk.r1 - Tk.r0;
-  for (int idx = tid; idx < rows * w; idx += blockDim.x) {
-    int k = idx / rows, rr = idx % rows;
-    X[rr * LDSP + k] = F[base + fpos(nr, Tk.r0 + rr, a + k)];
+  {
+    // lane = row, wave g = columns g, g+4, ...: all 32 loads of a thread are
+    // issued from clamped (always valid, lower-triangle) addresses before any
+    // is stored, then the ones in range go to LDS
+    const int li = tid & 63, g = tid >> 6;
+    const int iL = min(li, w - 1);                                   // diagonal-block row
+    const int rX = rows > 0 ? Tk.r0 + min(li, rows - 1) : a + w - 1;  // chunk row, below every column
+    T vl[16], vx[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int j = min(g + 4 * q, w - 1);
+      const long c0 = base + colofs(nr, a + j) - (a + j);  // + row = element (row, a + j)
+      vl[q] = F[c0 + a + max(iL, j)];
+      vx[q] = F[c0 + rX];
+    }
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int j = g + 4 * q;
+      if (j < w && li < w && li >= j) L11[li * LDSP + j] = vl[q];
+      if (j < w && li < rows) X[li * LDSP + j] = vx[q];
+    }
   }
   __syncthreads();
   if (tid < 64) ldl_wave(L11, w, w, sgn + first + a, mode, eps, delta, stats, owner, colk);
