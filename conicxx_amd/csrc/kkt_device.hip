@@ -416,15 +416,20 @@ struct Mfma<float> {
   static __device__ inline int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
-template <typename T>
-__global__ __launch_bounds__(256, sizeof(T) == 4 ? 4 : 2) void k_trail(const TrailTask* __restrict__ tasks,
-                                               const long* __restrict__ fofs,
-                                               const int* __restrict__ nrv,
-                                               const int* __restrict__ firstv,
-                                               const double* __restrict__ Dg, T* F) {
+// WN: waves across the 128 tile columns (2: four waves of 64 x 64; 4: eight
+// waves of 64 x 32, half the accumulator registers per wave and twice the
+// waves to hide latency).  Two wave rows of 64.
+// (the second launch bound is the minimum number of waves per SIMD)
+template <typename T, int WN>
+__global__ __launch_bounds__(128 * WN, (WN == 4 || sizeof(T) == 4) ? 4 : 2) void k_trail(
+    const TrailTask* __restrict__ tasks, const long* __restrict__ fofs, const int* __restrict__ nrv,
+    const int* __restrict__ firstv, const double* __restrict__ Dg, T* F) {
   typedef typename Mfma<T>::acc acc_t;
+  constexpr int NT = 128 * WN;          // threads
+  constexpr int NI = 8 / WN;            // 16-column MFMA blocks per wave
   constexpr int LDA = TM + 4;
-  constexpr int PER = KC * TM / 256;  // staged elements per thread and operand
+  constexpr int RS = NT / TM;           // staged column stride per thread
+  constexpr int PER = KC * TM / NT;     // staged elements per thread and operand
   __shared__ T As[KC * LDA];
   __shared__ T Bs[KC * LDA];
   __shared__ T Ds[KC];
@@ -432,14 +437,14 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 4 : 2) void k_trail(const Tra
   const int s = Tk.s, nr = nrv[s], first = firstv[s];
   const long base = fofs[s];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int wr = (wv >> 1) * 64, wc = (wv & 1) * 64;  // wave tile origin within the block tile
-  acc_t acc[4][4];
+  const int wr = (wv / WN) * 64, wc = (wv % WN) * (16 * NI);  // wave tile origin within the block tile
+  acc_t acc[4][NI];
 #pragma unroll
   for (int mi = 0; mi < 4; ++mi)
 #pragma unroll
-    for (int ni = 0; ni < 4; ++ni) acc[mi][ni] = (acc_t){0, 0, 0, 0};
+    for (int ni = 0; ni < NI; ++ni) acc[mi][ni] = (acc_t){0, 0, 0, 0};
   const int lrow = tid & (TM - 1);  // staged row (0..127)
-  const int lk = tid >> 7;          // staged column parity (0..1)
+  const int lk = tid / TM;          // first staged column
   const int gi = Tk.ti + lrow, gj = Tk.tj + lrow;
   const bool iok = gi < nr, jok = gj < nr;
   // rows of the staged operands are below the staged columns (clamped rows to
@@ -456,7 +461,7 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 4 : 2) void k_trail(const Tra
   auto load_chunk = [&](int k0) {
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
-      const int kq = min(k0 + lk + 2 * q, Tk.k1 - 1);
+      const int kq = min(k0 + lk + RS * q, Tk.k1 - 1);
       const long col = colofs(nr, kq) - kq;
       ra[q] = Fa[col];
       rb[q] = Fb[col];
@@ -467,7 +472,7 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 4 : 2) void k_trail(const Tra
   for (int k0 = Tk.k0; k0 < Tk.k1; k0 += KC) {
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
-      const int kk = lk + 2 * q;
+      const int kk = lk + RS * q;
       const bool kin = k0 + kk < Tk.k1;
       As[kk * LDA + lrow] = (kin && iok) ? ra[q] : T(0);
       Bs[kk * LDA + lrow] = (kin && jok) ? rb[q] : T(0);
@@ -477,29 +482,29 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 4 : 2) void k_trail(const Tra
     if (k0 + KC < Tk.k1) load_chunk(k0 + KC);  // in flight while the MFMAs run
 #pragma unroll
     for (int kk = 0; kk < KC; kk += 4) {
-      T af[4], bf[4];
+      T af[4], bf[NI];
       const int kr = kk + (lane >> 4);
       const T dk = Ds[kr];
 #pragma unroll
       for (int mi = 0; mi < 4; ++mi) af[mi] = As[kr * LDA + wr + mi * 16 + (lane & 15)];
 #pragma unroll
-      for (int ni = 0; ni < 4; ++ni) bf[ni] = Bs[kr * LDA + wc + ni * 16 + (lane & 15)] * dk;
+      for (int ni = 0; ni < NI; ++ni) bf[ni] = Bs[kr * LDA + wc + ni * 16 + (lane & 15)] * dk;
 #pragma unroll
       for (int mi = 0; mi < 4; ++mi)
 #pragma unroll
-        for (int ni = 0; ni < 4; ++ni) acc[mi][ni] = Mfma<T>::op(af[mi], bf[ni], acc[mi][ni]);
+        for (int ni = 0; ni < NI; ++ni) acc[mi][ni] = Mfma<T>::op(af[mi], bf[ni], acc[mi][ni]);
     }
     __syncthreads();
   }
-  // write back: per row block mi, the 16 old values are loaded together from
+  // write back: per row block mi, the old values are loaded together from
   // clamped addresses, then the valid ones are stored
 #pragma unroll
   for (int mi = 0; mi < 4; ++mi) {
-    T old[4][4];
-    long adr[4][4];
-    bool ok[4][4];
+    T old[NI][4];
+    long adr[NI][4];
+    bool ok[NI][4];
 #pragma unroll
-    for (int ni = 0; ni < 4; ++ni) {
+    for (int ni = 0; ni < NI; ++ni) {
       const int j = Tk.tj + wc + ni * 16 + (lane & 15);
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
@@ -510,7 +515,7 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 4 : 2) void k_trail(const Tra
       }
     }
 #pragma unroll
-    for (int ni = 0; ni < 4; ++ni)
+    for (int ni = 0; ni < NI; ++ni)
 #pragma unroll
       for (int r = 0; r < 4; ++r)
         if (ok[ni][r]) F[adr[ni][r]] = old[ni][r] - acc[mi][ni][r];
@@ -1211,8 +1216,12 @@ void DeviceKKT::launch_factor(T* F, const double* Kx, double extra_p, double ext
         break;
       case L_TRAIL:
         tbegin();
-        hipLaunchKernelGGL(k_trail<T>, dim3(n), dim3(256), 0, st_, trail_.p + L.beg, fofs_.p, nr_.p,
-                           first_.p, D_.p, F);
+        if constexpr (sizeof(T) == 4)
+          hipLaunchKernelGGL((k_trail<T, 2>), dim3(n), dim3(256), 0, st_, trail_.p + L.beg, fofs_.p, nr_.p,
+                             first_.p, D_.p, F);
+        else
+          hipLaunchKernelGGL((k_trail<T, 4>), dim3(n), dim3(512), 0, st_, trail_.p + L.beg, fofs_.p, nr_.p,
+                             first_.p, D_.p, F);
         tend(T_TRAIL, trail_launch_flops_[tl]);
         ++tl;
         break;
