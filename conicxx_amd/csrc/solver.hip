@@ -578,7 +578,10 @@ bool Solver::update_data(const int* Pp, const int* Pi, const double* Px, const d
     launch_scatter(na_, a_slot_.p, a_src_.p, dAx_new_.p, 0.0, Kx_.p, stream_);
     CX_HIP(hipStreamSynchronize(stream_));
   }
-  if (!factor_retry()) return false;
+  // The reference refactorizes here even when only q and b change
+  // (kkt_system.cpp:136); with K untouched since a successful factorization
+  // that factorization would succeed again, so it is skipped.
+  if ((hasP || hasA || !factorized_) && !factor_retry()) return false;
   if (hasP) launch_copy(nP, dPx_new_.p, dPx_.p, stream_);
   if (hasA) launch_copy(nA, dAx_new_.p, dAx_.p, stream_);
   if (q) {
