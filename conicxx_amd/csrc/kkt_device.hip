@@ -34,7 +34,7 @@ namespace {
 
 constexpr int SMALL_NR = 64;  // fronts up to this size are factored in LDS
 constexpr int NB = 64;        // sub-panel width
-constexpr int PW = 256;       // outer panel width
+constexpr int PW = 512;       // outer panel width
 constexpr int TM = 128;       // trailing-update tile
 constexpr int KC = 32;        // K chunk staged in LDS per step
 constexpr int SB = 64;        // solve block
