@@ -34,7 +34,9 @@ namespace {
 
 constexpr int SMALL_NR = 64;  // fronts up to this size are factored in LDS
 constexpr int NB = 64;        // sub-panel width
-constexpr int PW = 512;       // outer panel width
+// outer panel width: the K depth of the big trailing updates; wider for the
+// largest fronts, where the pass count over the trailing triangle dominates
+inline int panel_width(int nr) { return nr > 32768 ? 1024 : 512; }
 constexpr int TM = 128;       // trailing-update tile
 constexpr int KC = 32;        // K chunk staged in LDS per step
 constexpr int SB = 64;        // solve block
@@ -1127,49 +1129,42 @@ void DeviceKKT::build_schedule() {
     for (int t = P.level_ptr[l]; t < P.level_ptr[l + 1]; ++t)
       if (P.nr(P.level_sn[t]) <= SMALL_NR) ++sb;
     if (sb) launches_.push_back({L_SMALL, (int)small_h_.size() - sb, (int)small_h_.size()});
-    // big fronts: blocked right-looking, outer panels of PW, sub-panels of NB
+    // big fronts: blocked right-looking in sub-panels of NB columns; each front
+    // takes the trailing update of its outer panel (panel_width columns) once
+    // the panel's last sub-panel is done.  One panel launch and one trailing
+    // launch per sub-panel step: the in-panel updates of fronts inside a panel
+    // and the outer updates of fronts whose panel just ended share it.
     int maxnc = 0;
     for (int s : bigs) maxnc = std::max(maxnc, P.sn_ncol[s]);
-    for (int P0 = 0; P0 < maxnc; P0 += PW) {
-      for (int a0 = P0; a0 < P0 + PW && a0 < maxnc; a0 += NB) {
-        int tbeg = (int)trail_h_.size();
-        double fl = 0;
-        const int rbeg = (int)panel_h_.size();
-        for (int s : bigs) {
-          int nc = P.sn_ncol[s], nr = P.nr(s);
-          if (a0 >= nc) continue;
-          int pend = std::min(P0 + PW, nc);
-          int w = std::min(NB, pend - a0);
-          // the first task of a sub-panel (r0 == a0 + w) stages the diagonal block
-          const int blk = (int)dslot_h_.size();
-          dslot_h_.push_back({s, a0, w, 0, 0, blk});
-          if (a0 + w >= nr) panel_h_.push_back({s, a0, w, a0 + w, a0 + w, blk});
-          for (int r0 = a0 + w; r0 < nr; r0 += PANEL_ROWS)
-            panel_h_.push_back({s, a0, w, r0, std::min(nr, r0 + PANEL_ROWS), blk});
-          int b = a0 + w;
-          if (b < pend)
-            for (int ti = b; ti < nr; ti += TM)
-              for (int tj = b; tj < pend; tj += TM)
-                if (ti + TM - 1 >= tj) push_trail(s, ti, tj, a0, b, b, pend, nr, fl);
-        }
-        if ((int)panel_h_.size() > rbeg) launches_.push_back({L_PANEL, rbeg, (int)panel_h_.size()});
-        if ((int)trail_h_.size() > tbeg) {
-          launches_.push_back({L_TRAIL, tbeg, (int)trail_h_.size()});
-          trail_launch_flops_.push_back(fl);
-          trail_flops_static_ += fl;
-        }
-      }
+    for (int a0 = 0; a0 < maxnc; a0 += NB) {
       int tbeg = (int)trail_h_.size();
       double fl = 0;
+      const int rbeg = (int)panel_h_.size();
       for (int s : bigs) {
         int nc = P.sn_ncol[s], nr = P.nr(s);
-        if (P0 >= nc) continue;
-        int P1 = std::min(P0 + PW, nc);
-        if (P1 >= nr) continue;
-        for (int ti = P1; ti < nr; ti += TM)
-          for (int tj = P1; tj < nr; tj += TM)
-            if (ti + TM - 1 >= tj) push_trail(s, ti, tj, P0, P1, P1, nr, nr, fl);
+        if (a0 >= nc) continue;
+        const int pw = panel_width(nr);
+        const int P0 = (a0 / pw) * pw;
+        int pend = std::min(P0 + pw, nc);
+        int w = std::min(NB, pend - a0);
+        // the first task of a sub-panel (r0 == a0 + w) stages the diagonal block
+        const int blk = (int)dslot_h_.size();
+        dslot_h_.push_back({s, a0, w, 0, 0, blk});
+        if (a0 + w >= nr) panel_h_.push_back({s, a0, w, a0 + w, a0 + w, blk});
+        for (int r0 = a0 + w; r0 < nr; r0 += PANEL_ROWS)
+          panel_h_.push_back({s, a0, w, r0, std::min(nr, r0 + PANEL_ROWS), blk});
+        int b = a0 + w;
+        if (b < pend) {
+          for (int ti = b; ti < nr; ti += TM)
+            for (int tj = b; tj < pend; tj += TM)
+              if (ti + TM - 1 >= tj) push_trail(s, ti, tj, a0, b, b, pend, nr, fl);
+        } else if (pend < nr) {  // panel [P0, pend) complete: update everything right of it
+          for (int ti = pend; ti < nr; ti += TM)
+            for (int tj = pend; tj < nr; tj += TM)
+              if (ti + TM - 1 >= tj) push_trail(s, ti, tj, P0, pend, pend, nr, nr, fl);
+        }
       }
+      if ((int)panel_h_.size() > rbeg) launches_.push_back({L_PANEL, rbeg, (int)panel_h_.size()});
       if ((int)trail_h_.size() > tbeg) {
         launches_.push_back({L_TRAIL, tbeg, (int)trail_h_.size()});
         trail_launch_flops_.push_back(fl);
