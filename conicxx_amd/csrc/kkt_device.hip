@@ -106,6 +106,31 @@ __device__ inline bool wait_flag(int* f, int want, int* err) {
   return true;
 }
 
+// Hand-off of a block's values to the one task whose next step waits on
+// them: the values are also written to a hand-off buffer that was filled
+// with HANDOFF_EMPTY (a signalling-NaN pattern, which no arithmetic result
+// has) before the sweep, and the waiting wave polls the values themselves,
+// skipping the writer's store-completion wait and the flag round trip.
+constexpr long long HANDOFF_EMPTY = 0x7FF4C0C0A5A5DEADLL;
+
+// Wave-wide: every active lane's value at p has been written.  Returns it.
+__device__ inline double wait_value(const double* p, bool active, int* err) {
+  long spins = 0;
+  double v = 0.0;
+  for (;;) {
+    v = active ? load_acq(p) : 0.0;
+    if (!__any(active && __double_as_longlong(v) == HANDOFF_EMPTY)) break;
+    if (++spins > SPIN_LIMIT ||
+        ((spins & 63) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+      if ((threadIdx.x & 63) == 0) atomicOr(err, 1);
+      break;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  return v;
+}
+
 __device__ inline bool wait_count(int* f, int want, int* err) {
   long spins = 0;
   while (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < want) {
@@ -440,6 +465,11 @@ __global__ __launch_bounds__(128 * WN, (WN == 4 || sizeof(T) == 4) ? 4 : 2) void
   const long base = fofs[s];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int wr = (wv / WN) * 64, wc = (wv % WN) * (16 * NI);  // wave tile origin within the block tile
+  // a wave whose 64 x 16NI block holds no stored element (columns past chi,
+  // rows past the front, or entirely above the diagonal or below rlo) only
+  // helps stage the operands
+  const int wi0 = Tk.ti + wr, wj0 = Tk.tj + wc;
+  const bool wact = wj0 < Tk.chi && wi0 < nr && wi0 + 63 >= wj0 && wi0 + 63 >= Tk.rlo;
   acc_t acc[4][NI];
 #pragma unroll
   for (int mi = 0; mi < 4; ++mi)
@@ -482,19 +512,21 @@ __global__ __launch_bounds__(128 * WN, (WN == 4 || sizeof(T) == 4) ? 4 : 2) void
     if (tid < KC) Ds[tid] = dn;
     __syncthreads();
     if (k0 + KC < Tk.k1) load_chunk(k0 + KC);  // in flight while the MFMAs run
+    if (wact) {
 #pragma unroll
-    for (int kk = 0; kk < KC; kk += 4) {
-      T af[4], bf[NI];
-      const int kr = kk + (lane >> 4);
-      const T dk = Ds[kr];
+      for (int kk = 0; kk < KC; kk += 4) {
+        T af[4], bf[NI];
+        const int kr = kk + (lane >> 4);
+        const T dk = Ds[kr];
 #pragma unroll
-      for (int mi = 0; mi < 4; ++mi) af[mi] = As[kr * LDA + wr + mi * 16 + (lane & 15)];
+        for (int mi = 0; mi < 4; ++mi) af[mi] = As[kr * LDA + wr + mi * 16 + (lane & 15)];
 #pragma unroll
-      for (int ni = 0; ni < NI; ++ni) bf[ni] = Bs[kr * LDA + wc + ni * 16 + (lane & 15)] * dk;
+        for (int ni = 0; ni < NI; ++ni) bf[ni] = Bs[kr * LDA + wc + ni * 16 + (lane & 15)] * dk;
 #pragma unroll
-      for (int mi = 0; mi < 4; ++mi)
+        for (int mi = 0; mi < 4; ++mi)
 #pragma unroll
-        for (int ni = 0; ni < NI; ++ni) acc[mi][ni] = Mfma<T>::op(af[mi], bf[ni], acc[mi][ni]);
+          for (int ni = 0; ni < NI; ++ni) acc[mi][ni] = Mfma<T>::op(af[mi], bf[ni], acc[mi][ni]);
+      }
     }
     __syncthreads();
   }
@@ -555,10 +587,15 @@ __global__ void k_zero_int(int* p, int n) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
 }
 
+// also empties both hand-off buffers (hand[0, 2N)) for the sweeps
 __global__ void k_perm_in(const int* __restrict__ perm, const double* __restrict__ rhs, double* xp,
-                          int N) {
+                          double* hand, int N) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < N) xp[k] = rhs[perm[k]];
+  if (k < N) {
+    xp[k] = rhs[perm[k]];
+    hand[k] = __longlong_as_double(HANDOFF_EMPTY);
+    hand[N + k] = __longlong_as_double(HANDOFF_EMPTY);
+  }
 }
 __global__ void k_perm_out(const int* __restrict__ perm, const double* __restrict__ xp, double* x,
                            int N) {
@@ -584,7 +621,7 @@ __global__ __launch_bounds__(256) void k_fwd(
     const int* __restrict__ rowptr, const int* __restrict__ gptr, const int* __restrict__ gidx,
     const long* __restrict__ uofs, const int* __restrict__ child_ptr, const int* __restrict__ child,
     const int* __restrict__ nblk, const int* __restrict__ flag_ofs, int* flags, int* done, int gen,
-    const T* __restrict__ F, double* xp, double* U, int* err) {
+    const T* __restrict__ F, double* xp, double* yh, double* U, int* err) {
   __shared__ int tk;
   __shared__ double part[4][SB];
   __shared__ double Ld[SB * LDSP];  // Ld[row][col] of the diagonal block
@@ -636,16 +673,22 @@ __global__ __launch_bounds__(256) void k_fwd(
     double Lb[SB];
 #pragma unroll
     for (int c = 0; c < SB; ++c) Lb[c] = (c < cend) ? (double)raw[c] : 0.0;
-    if (cb > ready) {
-      if (lane == 0) {
-        int* fl = &flags[(flag_ofs[s] + ncbk - 1) * FSTRIDE];  // newest block this task needs
-        if (__hip_atomic_load(fl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != gen)
-          wait_flag(&flags[(flag_ofs[s] + cb) * FSTRIDE], gen, err);
+    double yv;
+    if (cb == ncbk - 1) {
+      // the newest block: its values are polled in the hand-off buffer
+      yv = wait_value(&yh[first + cbase + lane], lane < cend, err);
+    } else {
+      if (cb > ready) {
+        if (lane == 0) {
+          int* fl = &flags[(flag_ofs[s] + ncbk - 1) * FSTRIDE];  // newest block this task needs
+          if (__hip_atomic_load(fl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != gen)
+            wait_flag(&flags[(flag_ofs[s] + cb) * FSTRIDE], gen, err);
+        }
+        __builtin_amdgcn_wave_barrier();
+        ready = cb;
       }
-      __builtin_amdgcn_wave_barrier();
-      ready = cb;
+      yv = (lane < cend) ? load_acq(&xp[first + cbase + lane]) : 0.0;
     }
-    const double yv = (lane < cend) ? load_acq(&xp[first + cbase + lane]) : 0.0;
 #pragma unroll
     for (int c = 0; c < SB; ++c) acc += Lb[c] * bcast(yv, c);
   }
@@ -665,9 +708,10 @@ __global__ __launch_bounds__(256) void k_fwd(
       }
     }
     if (rv) {
-      if (r < nc)
+      if (r < nc) {
+        store_shared(&yh[first + r], val);
         store_shared(&xp[first + r], val);
-      else
+      } else
         store_shared(&U[uofs[s] + (r - nc)], val);
     }
     stores_complete();
@@ -716,7 +760,7 @@ __global__ __launch_bounds__(256) void k_bwd(
     const int* __restrict__ nrv, const int* __restrict__ ncv, const int* __restrict__ firstv,
     const int* __restrict__ rowptr, const int* __restrict__ rows, const int* __restrict__ parent,
     const int* __restrict__ nblkb, const int* __restrict__ flag_ofs, int* flags, int* done, int gen,
-    const T* __restrict__ F, const double* __restrict__ Dg, double* xp, int* err) {
+    const T* __restrict__ F, const double* __restrict__ Dg, double* xp, double* xh, int* err) {
   __shared__ int tk;
   __shared__ double part[4][SB];
   __shared__ double Ld[SB * LDSP];  // Ld[row][col] of the diagonal block
@@ -799,18 +843,12 @@ __global__ __launch_bounds__(256) void k_bwd(
   if (wv == 0) {
     double pnear = 0.0;
     if (nchunks > 0) {
-      if (lane == 0 && rstart < nc) {
-        const int b0 = rstart / SB, b1 = (min(rstart + SB, nc) - 1) / SB;
-        for (int b = b1; b >= b0; --b)
-          if (b > cb && b < ncb) wait_flag(&flags[(flag_ofs[s] + b) * FSTRIDE], gen, err);
-      }
-      __builtin_amdgcn_wave_barrier();
+      // pivot rows right below the block (block cb + 1) are polled in the
+      // hand-off buffer; rows past the pivots are final since the parent is done
       const int rr = rstart + lane;
-      double xv = 0.0;
-      if (rr < nr) {
-        const int g = (rr < nc) ? first + rr : rows[rowptr[s] + rr];
-        xv = load_acq(&xp[g]);
-      }
+      const bool piv = rr < nc;
+      double xv = wait_value(&xh[first + rr], piv, err);
+      if (!piv && rr < nr) xv = load_acq(&xp[rows[rowptr[s] + rr]]);
       xn[lane] = xv;
       __builtin_amdgcn_wave_barrier();
 #pragma unroll
@@ -829,7 +867,10 @@ __global__ __launch_bounds__(256) void k_bwd(
         if (lane < k) y -= Ld[k * LDSP + lane] * xk;
       }
     }
-    if (lane < kc) store_shared(&xp[first + c0 + lane], y);
+    if (lane < kc) {
+      store_shared(&xh[first + c0 + lane], y);
+      store_shared(&xp[first + c0 + lane], y);
+    }
     stores_complete();
     if (lane == 0) {
       __hip_atomic_store(&flags[(flag_ofs[s] + cb) * FSTRIDE], gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1014,6 +1055,7 @@ void DeviceKKT::setup(const LowerPattern& K, int nx, hipStream_t stream) {
   serr_.alloc(1);
   serr_.zero(st_);
   xp_.alloc(N);
+  hand_.alloc(2 * (size_t)std::max(1, N));
   U_.alloc(std::max<long>(1, uofs[ns]));
   gen_ = 0;
 
@@ -1303,29 +1345,29 @@ void DeviceKKT::solve_async(const double* rhs, double* x) {
   if (N == 0) return;
   const int nctr = (2 * ns + 2) * FSTRIDE;
   hipLaunchKernelGGL(k_zero_int, dim3(std::min(1024, ceil_div(nctr, 256))), dim3(256), 0, st_, done_.p, nctr);
-  hipLaunchKernelGGL(k_perm_in, dim3(ceil_div(N, 256)), dim3(256), 0, st_, perm_.p, rhs, xp_.p, N);
+  hipLaunchKernelGGL(k_perm_in, dim3(ceil_div(N, 256)), dim3(256), 0, st_, perm_.p, rhs, xp_.p, hand_.p, N);
   int* tick = done_.p + (size_t)2 * ns * FSTRIDE;
   ++gen_;
   tbegin();
   if (prec_ == 1)
     hipLaunchKernelGGL(k_fwd<float>, dim3(nfwd_), dim3(256), 0, st_, fwd_.p, tick, fofs_.p, nr_.p, nc_.p,
                        first_.p, rowptr_.p, gptr_.p, gidx_.p, uofs_.p, child_ptr_.p, child_.p, nblk_.p,
-                       flag_ofs_.p, flags_.p, done_.p, gen_, reinterpret_cast<const float*>(F_.p), xp_.p, U_.p, serr_.p);
+                       flag_ofs_.p, flags_.p, done_.p, gen_, reinterpret_cast<const float*>(F_.p), xp_.p, hand_.p, U_.p, serr_.p);
   else
     hipLaunchKernelGGL(k_fwd<double>, dim3(nfwd_), dim3(256), 0, st_, fwd_.p, tick, fofs_.p, nr_.p, nc_.p,
                        first_.p, rowptr_.p, gptr_.p, gidx_.p, uofs_.p, child_ptr_.p, child_.p, nblk_.p,
-                       flag_ofs_.p, flags_.p, done_.p, gen_, (const double*)F_.p, xp_.p, U_.p, serr_.p);
+                       flag_ofs_.p, flags_.p, done_.p, gen_, (const double*)F_.p, xp_.p, hand_.p, U_.p, serr_.p);
   tend(T_FWD, sweep_bytes_);
   ++gen_;
   tbegin();
   if (prec_ == 1)
     hipLaunchKernelGGL(k_bwd<float>, dim3(nbwd_), dim3(256), 0, st_, bwd_.p, tick + FSTRIDE, fofs_.p, nr_.p,
                      nc_.p, first_.p, rowptr_.p, rows_.p, parent_.p, nblkb_.p, flag_ofs_.p, flags_.p,
-                     done_.p + (size_t)ns * FSTRIDE, gen_, reinterpret_cast<const float*>(F_.p), D_.p, xp_.p, serr_.p);
+                     done_.p + (size_t)ns * FSTRIDE, gen_, reinterpret_cast<const float*>(F_.p), D_.p, xp_.p, hand_.p + N, serr_.p);
   else
     hipLaunchKernelGGL(k_bwd<double>, dim3(nbwd_), dim3(256), 0, st_, bwd_.p, tick + FSTRIDE, fofs_.p, nr_.p,
                      nc_.p, first_.p, rowptr_.p, rows_.p, parent_.p, nblkb_.p, flag_ofs_.p, flags_.p,
-                     done_.p + (size_t)ns * FSTRIDE, gen_, (const double*)F_.p, D_.p, xp_.p, serr_.p);
+                     done_.p + (size_t)ns * FSTRIDE, gen_, (const double*)F_.p, D_.p, xp_.p, hand_.p + N, serr_.p);
   tend(T_BWD, sweep_bytes_);
   hipLaunchKernelGGL(k_perm_out, dim3(ceil_div(N, 256)), dim3(256), 0, st_, perm_.p, xp_.p, x, N);
   CX_LAUNCH_CHECK();

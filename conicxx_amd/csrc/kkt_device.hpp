@@ -126,6 +126,7 @@ class DeviceKKT {
   DevBuf<int> flag_ofs_, nblk_, nblkb_, parent_;
   DevBuf<int> flags_, done_, tickets_, serr_;
   DevBuf<double> xp_, U_;
+  DevBuf<double> hand_;  // hand-off buffers of the forward [0, N) and backward [N, 2N) sweeps
   int gen_ = 0;
   // K full symmetric CSR for spmv
   DevBuf<int> kfptr_, kfcol_, kfsrc_;
