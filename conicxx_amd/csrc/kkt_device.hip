@@ -28,6 +28,8 @@
 
 #include "kkt_device.hpp"
 
+#include <utility>
+
 namespace cx {
 
 namespace {
@@ -184,6 +186,56 @@ __device__ inline float lane_bcast<float>(float v, int l) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
 
+// Step K of ldl_wave (K a template parameter, so that every register index
+// and loop bound below is a compile-time constant).
+template <typename T, int K>
+__device__ __forceinline__ void ldl_wave_step(T (&a)[64], int nc, double sg, int lane, int mode, double eps,
+                                              double delta, int* stats, bool count, T* colk) {
+  if (K >= nc) return;
+  const T d = (T)pivot_fix((double)lane_bcast(a[K], K), lane_bcast(sg, K), mode, eps, delta, stats,
+                           count && lane == 0);
+  const T c = a[K];  // unscaled entry (lane, K)
+  colk[lane] = c;
+  __builtin_amdgcn_wave_barrier();
+  const T l = c / d;
+  a[K] = (lane == K) ? d : (lane > K ? l : a[K]);
+  // Unpredicated: lane i only needs j <= i, and for j > i the registers hold
+  // entries above the diagonal that are never stored.  Lanes <= K update only
+  // such entries (with values read from their own upper registers), so
+  // nothing they compute reaches the factor.
+  // colk[j] (entry (j, K) = l_jK d_K) is read in groups of 16, the reads of
+  // the next group issued before the FMAs of the current one; the scheduling
+  // barriers keep the reads from being sunk next to their uses, where the LDS
+  // latency would be exposed at every pair of columns.
+  constexpr int J = K + 1;
+  T cb[2][16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q)
+    if (J + q < 64) cb[0][q] = colk[J + q];
+#pragma unroll
+  for (int g = 0; g < 4; ++g) {
+    const int j0 = J + 16 * g;
+    if (j0 < 64) {
+#pragma unroll
+      for (int q = 0; q < 16; ++q)
+        if (j0 + 16 + q < 64) cb[(g + 1) & 1][q] = colk[j0 + 16 + q];
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int q = 0; q < 16; ++q)
+        if (j0 + q < 64) a[j0 + q] -= l * cb[g & 1][q];
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+  __builtin_amdgcn_wave_barrier();  // every read of colk before the next write
+}
+
+template <typename T, int... Ks>
+__device__ __forceinline__ void ldl_wave_steps(std::integer_sequence<int, Ks...>, T (&a)[64], int nc, double sg,
+                                               int lane, int mode, double eps, double delta, int* stats,
+                                               bool count, T* colk) {
+  (ldl_wave_step<T, Ks>(a, nc, sg, lane, mode, eps, delta, stats, count, colk), ...);
+}
+
 // The same partial LDL^T as ldl_lds for n <= 64 rows, by one wave with no
 // workgroup barrier: lane i keeps row i in registers, pivot k is read from
 // lane k, and the unscaled column k goes through LDS (colk) so that every lane
@@ -197,28 +249,69 @@ __device__ __forceinline__ void ldl_wave(T* A, int n, int nc, const double* sgn,
 #pragma unroll
   for (int j = 0; j < 64; ++j) a[j] = (lane < n && j <= lane) ? A[lane * LDSP + j] : T(0);
   const double sg = lane < nc ? sgn[lane] : 1.0;  // pivot signs, read from lane k at step k
-#pragma unroll
-  for (int k = 0; k < 64; ++k) {
-    if (k < nc) {
-      const T d = (T)pivot_fix((double)lane_bcast(a[k], k), lane_bcast(sg, k), mode, eps, delta, stats,
-                               count && lane == 0);
-      const T c = a[k];  // unscaled entry (lane, k)
-      colk[lane] = c;
-      __builtin_amdgcn_wave_barrier();
-      const T l = c / d;
-      a[k] = (lane == k) ? d : (lane > k ? l : a[k]);
-      // Unpredicated: lane i only needs j <= i, and for j > i the registers
-      // hold entries above the diagonal that are never stored.  Lanes <= k
-      // update only such entries (with values read from their own upper
-      // registers), so nothing they compute reaches the factor.
-#pragma unroll
-      for (int j = k + 1; j < 64; ++j) a[j] -= l * colk[j];  // colk[j]: entry (j, k) = l_jk d_k
-      __builtin_amdgcn_wave_barrier();  // every read of colk before the next write
-    }
-  }
+  ldl_wave_steps<T>(std::make_integer_sequence<int, 64>{}, a, nc, sg, lane, mode, eps, delta, stats, count, colk);
 #pragma unroll
   for (int j = 0; j < 64; ++j)
     if (lane < n && j <= lane) A[lane * LDSP + j] = a[j];
+}
+
+// Column c (= lane) of the inverse of the unit-lower part of a 64 x 64 block
+// held row-major in LDS (stride LDSP):  y_i = [i == c] - sum_{t<i} L(i, t) y_t,
+// four accumulator chains per row (t mod 4).  The row entries L(i, t), which
+// do not depend on y, are read in chunks of 16 (row i takes ceil(i/16)
+// chunks), each chunk's reads issued before the FMAs of the previous one,
+// across row boundaries; chunk indices, rows and register indices are
+// compile-time constants.
+constexpr int inv_row_chunks(int i) { return (i + 15) / 16; }
+constexpr int inv_chunk_row(int c) {
+  int i = 1;
+  while (c >= inv_row_chunks(i)) c -= inv_row_chunks(i++);
+  return i;
+}
+constexpr int inv_chunk_grp(int c) {
+  int i = 1;
+  while (c >= inv_row_chunks(i)) c -= inv_row_chunks(i++);
+  return c;
+}
+constexpr int inv_nchunks() {
+  int n = 0;
+  for (int i = 1; i < 64; ++i) n += inv_row_chunks(i);
+  return n;
+}
+
+template <typename T, int C>
+__device__ __forceinline__ void inv_chunk(const T* L, T (&y)[64], T (&acc)[4], T (&cb)[2][16], int lane) {
+  constexpr int I = inv_chunk_row(C), G = inv_chunk_grp(C), T0 = 16 * G;
+  if constexpr (G == 0) {
+    acc[0] = (lane == I) ? T(1) : T(0);
+    acc[1] = acc[2] = acc[3] = T(0);
+  }
+  if constexpr (C + 1 < inv_nchunks()) {
+    constexpr int I1 = inv_chunk_row(C + 1), T1 = 16 * inv_chunk_grp(C + 1);
+#pragma unroll
+    for (int q = 0; q < 16; ++q)
+      if (T1 + q < I1) cb[(C + 1) & 1][q] = L[I1 * LDSP + T1 + q];
+  }
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int q = 0; q < 16; ++q)
+    if (T0 + q < I) acc[(T0 + q) & 3] -= cb[C & 1][q] * y[T0 + q];
+  __builtin_amdgcn_sched_barrier(0);
+  if constexpr (T0 + 16 >= I) y[I] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+}
+
+template <typename T, int... Cs>
+__device__ __forceinline__ void inv_chunks(std::integer_sequence<int, Cs...>, const T* L, T (&y)[64], T (&acc)[4],
+                                           T (&cb)[2][16], int lane) {
+  (inv_chunk<T, Cs>(L, y, acc, cb, lane), ...);
+}
+
+template <typename T>
+__device__ __forceinline__ void unit_lower_inverse_column(const T* L, T (&y)[64], int lane) {
+  T acc[4], cb[2][16];
+  y[0] = (lane == 0) ? T(1) : T(0);
+  cb[0][0] = L[1 * LDSP + 0];  // chunk 0: row 1, t = 0
+  inv_chunks<T>(std::make_integer_sequence<int, inv_nchunks()>{}, L, y, acc, cb, lane);
 }
 
 // ---------------------------------------------------------------------------
@@ -260,12 +353,17 @@ __global__ __launch_bounds__(256) void k_assemble(
     const int c = S.child, nrc = nrv[c], ncc = ncv[c];
     const int nu = nrc - ncc;
     const int* rm = rmap + rmap_ofs[c];
-    for (int i = S.i0 + wv; i < S.i1; i += 4) {
+    // a wave per child column, or, for the few columns a task of a large
+    // front gets, the whole workgroup along each column
+    const bool wide = S.i1 - S.i0 < 4;
+    const int i0 = wide ? S.i0 : S.i0 + wv, istep = wide ? 1 : 4;
+    const int t0 = wide ? tid : lane, tstep = wide ? 256 : 64;
+    for (int i = i0; i < S.i1; i += istep) {
       const int pc = rm[i];
       T* Fp = F + base + colofs(nr, pc) - pc;  // Fp[row] = element (row, pc)
       // Ucol[i'] = element (ncc + i', ncc + i) of the child, i' >= i
       const T* Ucol = F + fofs[c] + colofs(nrc, ncc + i) - i;
-      for (int ip = i + lane; ip < nu; ip += 64) Fp[rm[ip]] += Ucol[ip];
+      for (int ip = i + t0; ip < nu; ip += tstep) Fp[rm[ip]] += Ucol[ip];
     }
     __syncthreads();
   }
@@ -369,13 +467,7 @@ __global__ __launch_bounds__(256) void k_panel(
   // stored as row c of the buffer, which so holds Inv^T.
   if (tid < 64) {
     T y[64];
-#pragma unroll
-    for (int i = 0; i < 64; ++i) {
-      T acc[4] = {(tid == i) ? T(1) : T(0), T(0), T(0), T(0)};  // four chains: latency, not issue, bound
-#pragma unroll
-      for (int t = 0; t < i; ++t) acc[t & 3] -= L11[i * LDSP + t] * y[t];
-      y[i] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
-    }
+    unit_lower_inverse_column(L11, y, tid);
     __builtin_amdgcn_wave_barrier();
 #pragma unroll
     for (int i = 0; i < 64; ++i) L11[tid * LDSP + i] = y[i];
