@@ -363,28 +363,19 @@ std::vector<int> column_counts(const LowerPattern& K, const std::vector<int>& pe
 }
 
 // ---------------------------------------------------------------------------
-SupernodalPlan plan_supernodes(const LowerPattern& K, const RelaxParams& rp) {
+namespace {
+
+// Supernodal plan for a fixed elimination order perm (new -> old) that lists
+// every column after its elimination-tree descendants; parent and cc are the
+// etree and the column counts of L in the new labels.  force (new labels,
+// optional): a node whose first column is marked is merged into its parent
+// whatever its zero count.
+SupernodalPlan build_plan(const LowerPattern& K, const std::vector<int>& perm,
+                          const std::vector<int>& parent, const std::vector<int>& cc,
+                          const RelaxParams& rp, const std::vector<char>* force = nullptr) {
   SupernodalPlan P;
   const int N = K.N;
   P.N = N;
-  if (N == 0) return P;
-  std::vector<int> perm0 = amd_order(K);
-  std::vector<int> par0 = etree_of(K, perm0);
-  // Column counts do not depend on which postorder is used, so count once in a
-  // plain postorder, then visit children by ascending count: the child whose
-  // front is closest to its parent's sits next to it, where relaxed
-  // amalgamation can absorb it (dense separators then form one wide front).
-  std::vector<int> cc0 = column_counts(K, perm0, par0, postorder(par0));
-  std::vector<int> post = postorder(par0, cc0);
-  // relabel so that the elimination tree is postordered
-  std::vector<int> perm(N);
-  for (int k = 0; k < N; ++k) perm[k] = perm0[post[k]];
-  std::vector<int> ipost(N);
-  for (int k = 0; k < N; ++k) ipost[post[k]] = k;
-  std::vector<int> parent(N);
-  for (int k = 0; k < N; ++k) parent[k] = par0[post[k]] == -1 ? -1 : ipost[par0[post[k]]];
-  std::vector<int> cc(N);
-  for (int k = 0; k < N; ++k) cc[k] = cc0[post[k]];
   P.nnzL_exact = 0;
   for (int j = 0; j < N; ++j) P.nnzL_exact += cc[j];
 
@@ -418,7 +409,8 @@ SupernodalPlan plan_supernodes(const LowerPattern& K, const RelaxParams& rp) {
       double z = sm - actual;
       double frac = sm > 0 ? z / sm : 0.0;
       bool ok = k <= rp.nrelax0 || (k <= rp.nrelax1 && frac < rp.zrelax0) ||
-                (k <= rp.nrelax2 && frac < rp.zrelax1) || frac < rp.zrelax2;
+                (k <= rp.nrelax2 && frac < rp.zrelax1) || frac < rp.zrelax2 ||
+                (force && (*force)[c.first]);
       if (!ok) break;
       s = Node{c.first, s.last, k, r, z};
       stack.pop_back();
@@ -511,6 +503,128 @@ SupernodalPlan plan_supernodes(const LowerPattern& K, const RelaxParams& rp) {
   }
   P.perm = perm;
   P.iperm = iperm;
+  return P;
+}
+
+// Thin fronts (a few pivot columns over a large update matrix, such as a
+// short separator beside a dense one) cost a memory-bound trailing pass over
+// their whole update matrix and an extend-add of it into the parent, and
+// relaxed amalgamation can absorb a front only when it immediately precedes
+// its parent, which a dense separator's own chain usually does instead.  This
+// order puts every such front (and, transitively, thin fronts absorbed into
+// it) right before its parent's columns, after the subtrees of all the
+// non-absorbed children.  Each column still follows its descendants, so the
+// fill is that of the first order; amalgamation then merges the thin fronts
+// into the parent (explicit zeros in their columns instead of an update
+// matrix of their own; force marks their columns in the new order).  Returns
+// false when no front qualifies.
+bool thin_front_order(const SupernodalPlan& P, const RelaxParams& rp, std::vector<int>& perm2,
+                      std::vector<char>& force) {
+  const int ns = P.ns;
+  std::vector<char> absorb(ns, 0);
+  int nabs = 0;
+  for (int s = 0; s < ns; ++s) {
+    const int p = P.sn_parent[s];
+    if (p < 0) continue;
+    const int nc = P.sn_ncol[s], nu = P.nr(s) - nc;
+    if ((nc <= rp.thin_ncol || 64L * nc <= nu) && nu >= rp.thin_min_update && 2L * nu >= P.nr(p))
+      absorb[s] = 1, ++nabs;
+  }
+  if (nabs == 0) return false;
+  perm2.clear();
+  perm2.reserve(P.N);
+  force.clear();
+  force.reserve(P.N);
+  // A frame emits the subtree of s: the non-absorbed children's subtrees of
+  // every member of s's cluster (s and the fronts absorbed into it,
+  // descendants first), then the members' columns.
+  struct Frame {
+    std::vector<int> mem;
+    size_t mi = 0;
+    int ci = 0;
+  };
+  std::vector<Frame> st;
+  std::vector<std::pair<int, int>> dfs;  // (node, next child slot)
+  auto push = [&](int s) {
+    Frame f;
+    dfs.assign(1, {s, P.child_ptr[s]});
+    while (!dfs.empty()) {
+      const int v = dfs.back().first;
+      int c = dfs.back().second;
+      while (c < P.child_ptr[v + 1] && !absorb[P.child[c]]) ++c;
+      if (c < P.child_ptr[v + 1]) {
+        dfs.back().second = c + 1;
+        const int a = P.child[c];
+        dfs.push_back({a, P.child_ptr[a]});
+      } else {
+        f.mem.push_back(v);
+        dfs.pop_back();
+      }
+    }
+    f.ci = P.child_ptr[f.mem[0]];
+    st.push_back(std::move(f));
+  };
+  for (int r = 0; r < ns; ++r) {
+    if (P.sn_parent[r] >= 0) continue;
+    push(r);
+    while (!st.empty()) {
+      Frame& f = st.back();
+      if (f.mi < f.mem.size()) {
+        const int m = f.mem[f.mi];
+        if (f.ci < P.child_ptr[m + 1]) {
+          const int q = P.child[f.ci++];
+          if (!absorb[q]) push(q);
+          continue;
+        }
+        if (++f.mi < f.mem.size()) f.ci = P.child_ptr[f.mem[f.mi]];
+        continue;
+      }
+      for (int m : f.mem)
+        for (int j = P.sn_first[m]; j < P.sn_first[m] + P.sn_ncol[m]; ++j) {
+          perm2.push_back(P.perm[j]);
+          force.push_back(absorb[m]);
+        }
+      st.pop_back();
+    }
+  }
+  return (int)perm2.size() == P.N;
+}
+
+}  // namespace
+
+SupernodalPlan plan_supernodes(const LowerPattern& K, const RelaxParams& rp) {
+  const int N = K.N;
+  if (N == 0) {
+    SupernodalPlan P;
+    P.N = 0;
+    return P;
+  }
+  std::vector<int> perm0 = amd_order(K);
+  std::vector<int> par0 = etree_of(K, perm0);
+  // Column counts do not depend on which postorder is used, so count once in a
+  // plain postorder, then visit children by ascending count: the child whose
+  // front is closest to its parent's sits next to it, where relaxed
+  // amalgamation can absorb it (dense separators then form one wide front).
+  std::vector<int> cc0 = column_counts(K, perm0, par0, postorder(par0));
+  std::vector<int> post = postorder(par0, cc0);
+  // relabel so that the elimination tree is postordered
+  std::vector<int> perm(N);
+  for (int k = 0; k < N; ++k) perm[k] = perm0[post[k]];
+  std::vector<int> ipost(N);
+  for (int k = 0; k < N; ++k) ipost[post[k]] = k;
+  std::vector<int> parent(N);
+  for (int k = 0; k < N; ++k) parent[k] = par0[post[k]] == -1 ? -1 : ipost[par0[post[k]]];
+  std::vector<int> cc(N);
+  for (int k = 0; k < N; ++k) cc[k] = cc0[post[k]];
+  SupernodalPlan P = build_plan(K, perm, parent, cc, rp);
+  std::vector<int> perm2;
+  std::vector<char> force;
+  if (rp.thin_ncol > 0 && thin_front_order(P, rp, perm2, force)) {
+    std::vector<int> par2 = etree_of(K, perm2);
+    std::vector<int> cc2 = column_counts(K, perm2, par2, postorder(par2));
+    SupernodalPlan P2 = build_plan(K, perm2, par2, cc2, rp, &force);
+    if (P2.nnzL_exact == P.nnzL_exact) return P2;  // an equivalent order: same fill
+  }
   return P;
 }
 

@@ -65,6 +65,11 @@ struct RelaxParams {
   // supernode: splitting it into a chain makes every link re-assemble an
   // almost equally large contribution block into the next.
   int max_merge_cols = 1 << 30;
+  // Thin fronts absorbed into their parent by a second, equivalent order:
+  // at most thin_ncol pivot columns or at most 1/64 of the update matrix's
+  // rows, an update matrix of at least thin_min_update rows covering at least
+  // half of the parent's rows (thin_ncol = 0: off).
+  int thin_ncol = 32, thin_min_update = 1024;
 };
 
 SupernodalPlan plan_supernodes(const LowerPattern& K, const RelaxParams& rp = RelaxParams());
