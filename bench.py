@@ -23,7 +23,10 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
-os.environ.setdefault("CONICXX_KERNEL_TIMING", "1")
+# HIP events around the launches of every 4th factorization and KKT solve: an
+# event pair around every launch (about a thousand per factorization) adds
+# 6% to a config-1 solve
+os.environ.setdefault("CONICXX_KERNEL_TIMING", "4")
 
 HBM_PEAK_GBS = 8000.0
 # k_trail HBM bytes per launch on config 1 (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0

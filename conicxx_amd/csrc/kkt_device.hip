@@ -1175,7 +1175,9 @@ void DeviceKKT::setup(const LowerPattern& K, int nx, hipStream_t stream) {
   kfsrc_.upload(fsrc, st_);
   {
     const char* e = std::getenv("CONICXX_KERNEL_TIMING");
-    timing_ = e && e[0] == '1';
+    const int every = e ? std::atoi(e) : 0;
+    timing_ = every > 0;
+    timing_every_ = std::max(1, every);
   }
   // algorithmic bytes of one triangular sweep: every stored entry of L once
   sweep_bytes_ = 8.0 * P_.nnzL + 3.0 * 8.0 * N;
@@ -1316,7 +1318,7 @@ void DeviceKKT::build_schedule() {
 }
 
 void DeviceKKT::tbegin() {
-  if (!timing_) return;
+  if (!sample_) return;
   if (evused_ + 2 > evpool_.size()) {
     size_t add = std::max<size_t>(256, evpool_.size());
     for (size_t i = 0; i < add; ++i) {
@@ -1330,9 +1332,21 @@ void DeviceKKT::tbegin() {
 
 void DeviceKKT::tend(int kind, double work) {
   if (!timing_) return;
+  kraw_[kind].all += 1;
+  kraw_[kind].work += work;
+  if (!sample_) return;
   CX_HIP(hipEventRecord(evpool_[evused_ + 1], st_));
-  evpending_.push_back({kind, work});
+  evpending_.push_back(kind);
   evused_ += 2;
+}
+
+DeviceKKT::KindStats DeviceKKT::kind_stats(int k) const {
+  const KindRaw& r = kraw_[k];
+  KindStats t;
+  t.ms = r.timed ? r.ms_timed * (double)r.all / (double)r.timed : 0.0;
+  t.work = r.work;
+  t.launches = r.all;
+  return t;
 }
 
 void DeviceKKT::collect_timing() {
@@ -1341,10 +1355,9 @@ void DeviceKKT::collect_timing() {
   for (size_t i = 0; i < evpending_.size(); ++i) {
     float ms = 0;
     CX_HIP(hipEventElapsedTime(&ms, evpool_[2 * i], evpool_[2 * i + 1]));
-    KindStats& k = kstats_[evpending_[i].first];
-    k.ms += ms;
-    k.work += evpending_[i].second;
-    k.launches += 1;
+    KindRaw& k = kraw_[evpending_[i]];
+    k.ms_timed += ms;
+    k.timed += 1;
   }
   evpending_.clear();
   evused_ = 0;
@@ -1352,13 +1365,15 @@ void DeviceKKT::collect_timing() {
 
 void DeviceKKT::reset_timing() {
   collect_timing();
-  for (auto& k : kstats_) k = KindStats{};
+  for (auto& k : kraw_) k = KindRaw{};
+  fact_seq_ = sweep_seq_ = 0;
 }
 
 template <typename T>
 void DeviceKKT::launch_factor(T* F, const double* Kx, double extra_p, double extra_a, int mode, double eps,
                               double delta) {
   size_t tl = 0;
+  sample_ = timing_ && fact_seq_++ % timing_every_ == 0;
   for (const Launch& L : launches_) {
     const int n = L.end - L.beg;
     if (n <= 0) continue;
@@ -1403,6 +1418,7 @@ void DeviceKKT::launch_factor(T* F, const double* Kx, double extra_p, double ext
     tend(T_PANEL, 0.0);
     CX_LAUNCH_CHECK();
   }
+  sample_ = false;
 }
 
 void DeviceKKT::factorize_async(const double* Kx, double extra_p, double extra_a, int mode,
@@ -1440,6 +1456,7 @@ void DeviceKKT::solve_async(const double* rhs, double* x) {
   hipLaunchKernelGGL(k_perm_in, dim3(ceil_div(N, 256)), dim3(256), 0, st_, perm_.p, rhs, xp_.p, hand_.p, N);
   int* tick = done_.p + (size_t)2 * ns * FSTRIDE;
   ++gen_;
+  sample_ = timing_ && sweep_seq_++ % timing_every_ == 0;
   tbegin();
   if (prec_ == 1)
     hipLaunchKernelGGL(k_fwd<float>, dim3(nfwd_), dim3(256), 0, st_, fwd_.p, tick, fofs_.p, nr_.p, nc_.p,
@@ -1461,6 +1478,7 @@ void DeviceKKT::solve_async(const double* rhs, double* x) {
                      nc_.p, first_.p, rowptr_.p, rows_.p, parent_.p, nblkb_.p, flag_ofs_.p, flags_.p,
                      done_.p + (size_t)ns * FSTRIDE, gen_, (const double*)F_.p, D_.p, xp_.p, hand_.p + N, serr_.p);
   tend(T_BWD, sweep_bytes_);
+  sample_ = false;
   hipLaunchKernelGGL(k_perm_out, dim3(ceil_div(N, 256)), dim3(256), 0, st_, perm_.p, xp_.p, x, N);
   CX_LAUNCH_CHECK();
 }

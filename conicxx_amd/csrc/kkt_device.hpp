@@ -65,17 +65,19 @@ class DeviceKKT {
   hipStream_t stream() const { return st_; }
 
   // Per-kernel-class device timing with pooled HIP events recorded on this
-  // stream around each launch (enabled by CONICXX_KERNEL_TIMING=1); events are
-  // resolved only where the solver synchronizes anyway.
+  // stream around each launch of every Nth factorization and every Nth solve
+  // (CONICXX_KERNEL_TIMING=N, N >= 1; every launch and its work are counted);
+  // events are resolved only where the solver synchronizes anyway.
   enum TimedKind { T_TRAIL = 0, T_PANEL, T_ASM, T_SMALL, T_FWD, T_BWD, T_NKINDS };
   struct KindStats {
-    double ms = 0, work = 0;  // work: algorithmic flops (factorization kinds) or bytes (sweeps)
+    double ms = 0;    // device time: the timed launches' total scaled to all launches
+    double work = 0;  // algorithmic flops (factorization kinds) or bytes (sweeps), all launches
     long launches = 0;
   };
   bool timing() const { return timing_; }
   void collect_timing();
   void reset_timing();
-  const KindStats& kind_stats(int k) const { return kstats_[k]; }
+  KindStats kind_stats(int k) const;
   double sweep_bytes() const { return sweep_bytes_; }
 
  private:
@@ -134,10 +136,17 @@ class DeviceKKT {
   void tbegin();
   void tend(int kind, double work);
   bool timing_ = false;
+  int timing_every_ = 1;
+  bool sample_ = false;               // the current factorization / solve is timed
+  long fact_seq_ = 0, sweep_seq_ = 0;  // since the last reset_timing()
   std::vector<hipEvent_t> evpool_;
   size_t evused_ = 0;
-  std::vector<std::pair<int, double>> evpending_;
-  KindStats kstats_[T_NKINDS];
+  std::vector<int> evpending_;  // kind of each timed launch
+  struct KindRaw {
+    double ms_timed = 0, work = 0;
+    long timed = 0, all = 0;
+  };
+  KindRaw kraw_[T_NKINDS];
   std::vector<double> trail_launch_flops_h_;
   double sweep_bytes_ = 0;
 };

@@ -287,7 +287,9 @@ class Solver:
     KERNEL_CLASSES = ["trail", "panel", "assemble", "small", "fwd", "bwd"]
 
     def kernel_stats(self):
-        """Device time per kernel class of the last solve (CONICXX_KERNEL_TIMING=1)."""
+        """Device time per kernel class of the last solve (CONICXX_KERNEL_TIMING=N:
+        HIP events around the launches of every Nth factorization and KKT solve,
+        scaled to all launches)."""
         out = np.zeros(3 * len(self.KERNEL_CLASSES))
         n = _lib.lib().cx_get_kernel_stats(self._h, out.ctypes.data_as(_lib._f64p), len(self.KERNEL_CLASSES))
         return {self.KERNEL_CLASSES[k]: dict(ms=out[3 * k], launches=int(out[3 * k + 1]), work=out[3 * k + 2])

@@ -129,13 +129,16 @@ void cx_get_kkt_stats(const cx_solver* h, double* stats8);
  * fallbacks (factor_precision = CX_FACTOR_FP32_REFINED), [10] largest final
  * relative residual |rhs - K x| / |rhs| of an accepted KKT solve, [11] that of
  * the FP32 solve that triggered the fallback (0 if none), [12] the IPM
- * iteration of the fallback (-1 if none).  [5..7] need CONICXX_KERNEL_TIMING=1
- * in the environment at setup().  No counterpart in the reference. */
+ * iteration of the fallback (-1 if none).  [5..7] need CONICXX_KERNEL_TIMING=N
+ * (N >= 1) in the environment at setup().  No counterpart in the reference. */
 void cx_get_profile(const cx_solver* h, double* out13);
 
 /* Device time per kernel class of the last solve, measured with HIP events on the
- * solver's stream (CONICXX_KERNEL_TIMING=1 at setup()): for class k, out[3k..3k+2]
- * = (milliseconds, launches, algorithmic work); work is flops for the trailing
+ * solver's stream around the launches of every Nth numeric factorization and
+ * every Nth KKT solve (CONICXX_KERNEL_TIMING=N, N >= 1, at setup(); the
+ * schedule of launches is the same in every factorization): for class k,
+ * out[3k..3k+2] = (milliseconds, the timed launches' total scaled to all
+ * launches; launches; algorithmic work of all launches); work is flops for the trailing
  * update (class 0) and bytes for the forward/backward sweeps (classes 4, 5).
  * Classes: 0 trailing update, 1 sub-panel (diagonal block LDL^T + panel solve,
  * and the store of the diagonal blocks), 2 assembly, 3 small fronts, 4 forward
