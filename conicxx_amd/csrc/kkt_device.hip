@@ -732,6 +732,9 @@ __global__ __launch_bounds__(256) void k_fwd(
   const int kc = diag ? min(SB, nc - r0) : 0;
   // (entries above the diagonal are never used: read the diagonal instead)
   for (int c = wv; c < kc; c += 4) Ld[lane * LDSP + c] = (double)F[base + fpos(nr, max(rc, r0 + c), r0 + c)];
+  // right-hand side of this block's pivot rows, read now: only this task
+  // writes them, and the read stays off the dependency chain
+  const double rhs_r = (wv == 0 && rv && r < nc) ? xp[first + r] : 0.0;
   // Children's update vectors first: they do not depend on this front's own
   // blocks, so gathering them here keeps them off the front's dependency
   // chain.  Every thread waits for a share of the children, every wave sums a
@@ -788,10 +791,7 @@ __global__ __launch_bounds__(256) void k_fwd(
   __syncthreads();
   if (wv == 0) {
     double val = 0.0;
-    if (rv) {
-      val = (r < nc) ? xp[first + r] : 0.0;
-      val -= (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
-    }
+    if (rv) val = rhs_r - ((part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]));
 #pragma unroll
     for (int k = 0; k < SB; ++k) {
       if (k < kc) {
@@ -868,6 +868,10 @@ __global__ __launch_bounds__(256) void k_bwd(
   const int ncb = (nc + SB - 1) / SB;
   const int rstart = c0 + kc;
   const int nchunks = (nr - rstart + SB - 1) / SB;
+  // this block's forward result and pivots, read (and divided) now: only this
+  // task writes the block, and both stay off the dependency chain
+  double yd = 0.0;
+  if (wv == 0 && lane < kc) yd = xp[first + c0 + lane] * (1.0 / Dg[first + c0 + lane]);
   if (nchunks > 0) {
     // lane = row, wave g = columns g, g+4, ...: loads from clamped (valid,
     // lower-triangle) addresses all in flight, then the ones in range stored
@@ -947,11 +951,7 @@ __global__ __launch_bounds__(256) void k_bwd(
       for (int r = 0; r < SB; ++r) pnear += Ln[r * LDSP + lane] * xn[r];  // lane = column
     }
     double y = 0.0;
-    if (lane < kc) {
-      const double dinv = 1.0 / Dg[first + c0 + lane];
-      y = xp[first + c0 + lane] * dinv -
-          (((part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane])) + pnear);
-    }
+    if (lane < kc) y = yd - (((part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane])) + pnear);
 #pragma unroll
     for (int k = SB - 1; k >= 0; --k) {
       if (k < kc) {
