@@ -792,11 +792,25 @@ __global__ __launch_bounds__(256) void k_fwd(
   if (wv == 0) {
     double val = 0.0;
     if (rv) val = rhs_r - ((part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]));
+    if (kc == SB) {
+      // full block: the lane's row of L11 in registers (zero from the
+      // diagonal on), then a branch-free chain of broadcast + FMA; a zero
+      // entry leaves val unchanged while the broadcast value is finite
+      double lr[SB];
 #pragma unroll
-    for (int k = 0; k < SB; ++k) {
-      if (k < kc) {
-        const double yk = bcast(val, k);
-        if (lane > k) val -= Ld[lane * LDSP + k] * yk;
+      for (int k = 0; k < SB; ++k) {
+        const double v = Ld[lane * LDSP + k];
+        lr[k] = (k < lane) ? v : 0.0;
+      }
+#pragma unroll
+      for (int k = 0; k < SB; ++k) val -= lr[k] * bcast(val, k);
+    } else {
+#pragma unroll
+      for (int k = 0; k < SB; ++k) {
+        if (k < kc) {
+          const double yk = bcast(val, k);
+          if (lane > k) val -= Ld[lane * LDSP + k] * yk;
+        }
       }
     }
     if (rv) {
@@ -952,11 +966,24 @@ __global__ __launch_bounds__(256) void k_bwd(
     }
     double y = 0.0;
     if (lane < kc) y = yd - (((part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane])) + pnear);
+    if (kc == SB) {
+      // full block: the lane's column of L11 in registers (zero from the
+      // diagonal up), then a branch-free chain of broadcast + FMA
+      double lc[SB];
 #pragma unroll
-    for (int k = SB - 1; k >= 0; --k) {
-      if (k < kc) {
-        const double xk = bcast(y, k);
-        if (lane < k) y -= Ld[k * LDSP + lane] * xk;
+      for (int k = 0; k < SB; ++k) {
+        const double v = Ld[k * LDSP + lane];
+        lc[k] = (lane < k) ? v : 0.0;
+      }
+#pragma unroll
+      for (int k = SB - 1; k >= 0; --k) y -= lc[k] * bcast(y, k);
+    } else {
+#pragma unroll
+      for (int k = SB - 1; k >= 0; --k) {
+        if (k < kc) {
+          const double xk = bcast(y, k);
+          if (lane < k) y -= Ld[k * LDSP + lane] * xk;
+        }
       }
     }
     if (lane < kc) {
