@@ -255,13 +255,13 @@ __device__ __forceinline__ void ldl_wave(T* A, int n, int nc, const double* sgn,
     if (lane < n && j <= lane) A[lane * LDSP + j] = a[j];
 }
 
-// Column c (= lane) of the inverse of the unit-lower part of a 64 x 64 block
-// held row-major in LDS (stride LDSP):  y_i = [i == c] - sum_{t<i} L(i, t) y_t,
-// four accumulator chains per row (t mod 4).  The row entries L(i, t), which
-// do not depend on y, are read in chunks of 16 (row i takes ceil(i/16)
-// chunks), each chunk's reads issued before the FMAs of the previous one,
-// across row boundaries; chunk indices, rows and register indices are
-// compile-time constants.
+// Forward substitution with the unit-lower part of a 64 x 64 block held
+// row-major in LDS (stride LDSP), one right-hand side per lane:
+// y_i = b_i - sum_{t<i} L(i, t) y_t, four accumulator chains per row (t mod 4).
+// The row entries L(i, t), which do not depend on y, are read in chunks of 16
+// (row i takes ceil(i/16) chunks), each chunk's reads issued before the FMAs
+// of the previous one, across row boundaries; chunk indices, rows and register
+// indices are compile-time constants.
 constexpr int inv_row_chunks(int i) { return (i + 15) / 16; }
 constexpr int inv_chunk_row(int c) {
   int i = 1;
@@ -280,10 +280,10 @@ constexpr int inv_nchunks() {
 }
 
 template <typename T, int C>
-__device__ __forceinline__ void inv_chunk(const T* L, T (&y)[64], T (&acc)[4], T (&cb)[2][16], int lane) {
+__device__ __forceinline__ void inv_chunk(const T* L, const T* b, T (&y)[64], T (&acc)[4], T (&cb)[2][16]) {
   constexpr int I = inv_chunk_row(C), G = inv_chunk_grp(C), T0 = 16 * G;
   if constexpr (G == 0) {
-    acc[0] = (lane == I) ? T(1) : T(0);
+    acc[0] = b[I];
     acc[1] = acc[2] = acc[3] = T(0);
   }
   if constexpr (C + 1 < inv_nchunks()) {
@@ -301,17 +301,18 @@ __device__ __forceinline__ void inv_chunk(const T* L, T (&y)[64], T (&acc)[4], T
 }
 
 template <typename T, int... Cs>
-__device__ __forceinline__ void inv_chunks(std::integer_sequence<int, Cs...>, const T* L, T (&y)[64], T (&acc)[4],
-                                           T (&cb)[2][16], int lane) {
-  (inv_chunk<T, Cs>(L, y, acc, cb, lane), ...);
+__device__ __forceinline__ void inv_chunks(std::integer_sequence<int, Cs...>, const T* L, const T* b, T (&y)[64],
+                                           T (&acc)[4], T (&cb)[2][16]) {
+  (inv_chunk<T, Cs>(L, b, y, acc, cb), ...);
 }
 
+// b: the lane's right-hand side (64 entries, LDS)
 template <typename T>
-__device__ __forceinline__ void unit_lower_inverse_column(const T* L, T (&y)[64], int lane) {
+__device__ __forceinline__ void unit_lower_solve(const T* L, const T* b, T (&y)[64]) {
   T acc[4], cb[2][16];
-  y[0] = (lane == 0) ? T(1) : T(0);
+  y[0] = b[0];
   cb[0][0] = L[1 * LDSP + 0];  // chunk 0: row 1, t = 0
-  inv_chunks<T>(std::make_integer_sequence<int, inv_nchunks()>{}, L, y, acc, cb, lane);
+  inv_chunks<T>(std::make_integer_sequence<int, inv_nchunks()>{}, L, b, y, acc, cb);
 }
 
 // ---------------------------------------------------------------------------
@@ -460,35 +461,25 @@ __global__ __launch_bounds__(256) void k_panel(
   }
   if (rows <= 0) return;
   for (int k = tid; k < w; k += blockDim.x) dinv[k] = T(1) / L11[k * LDSP + k];
-  __syncthreads();  // every read of the factored block done before it is overwritten
-  // L11^-1 (unit lower) by the first wave, lane c computing column c:
-  // y_i = [i == c] - sum_{t<i} L11(i, t) y_t, exactly zero above the diagonal
-  // (rows and columns at or past w are never read below).  Column c is then
-  // stored as row c of the buffer, which so holds Inv^T.
+  __syncthreads();
+  // L21 = R L11^-T D^-1 by forward substitution, lane = row r of the chunk:
+  // x_k = R(r, k) - sum_{t<k} L11(k, t) x_t, L21(r, k) = x_k / d_k.  A
+  // triangular solve rather than a product with an explicit inverse: on the
+  // badly scaled diagonal blocks of late interior-point iterations the
+  // inverse loses accuracy that the refinement cannot recover.  Columns at or
+  // past w (and lanes past the chunk's rows) compute unused values.
   if (tid < 64) {
     T y[64];
-    unit_lower_inverse_column(L11, y, tid);
-    __builtin_amdgcn_wave_barrier();
+    unit_lower_solve(L11, X + tid * LDSP, y);
+    __builtin_amdgcn_wave_barrier();  // the lane's right-hand side is read: its row takes the result
 #pragma unroll
-    for (int i = 0; i < 64; ++i) L11[tid * LDSP + i] = y[i];
+    for (int k = 0; k < 64; ++k) X[tid * LDSP + k] = y[k];
   }
   __syncthreads();
-  // L21 = R Inv^T D^-1: lane = row of the chunk, wave g = columns [16g, 16g+16)
-  const int r = tid & 63, g = tid >> 6;
-  if (r < rows) {
-    T acc[16];
-#pragma unroll
-    for (int q = 0; q < 16; ++q) acc[q] = T(0);
-    for (int t = 0; t < w; ++t) {
-      const T x = X[r * LDSP + t];
-#pragma unroll
-      for (int q = 0; q < 16; ++q) acc[q] += x * L11[t * LDSP + 16 * g + q];  // Inv(16g+q, t)
-    }
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      const int k = 16 * g + q;
-      if (k < w) F[base + fpos(nr, Tk.r0 + r, a + k)] = acc[q] * dinv[k];
-    }
+  // column by column, lanes along the rows: coalesced stores by all waves
+  for (int idx = tid; idx < rows * w; idx += blockDim.x) {
+    const int k = idx / rows, rr = idx - k * rows;
+    F[base + fpos(nr, Tk.r0 + rr, a + k)] = X[rr * LDSP + k] * dinv[k];
   }
 }
 
@@ -856,10 +847,12 @@ __device__ inline double wave_reduce_scatter64(double (&a)[SB], int lane) {
 // below the block are visited farthest first (non-pivot rows, final once the
 // parent front is done, then later pivot blocks in descending order) with
 // lane = row: every lane keeps 64 column partials (reduce-scattered once),
-// the L block is loaded into registers before the flag is polled.  The 64
-// rows right below the block, the task's last dependency, are staged in LDS
-// at the start and applied after the far partials are reduced, with lane =
-// column: that dependency costs one flag wait and 64 FMAs per lane.
+// the L block is loaded into registers before the flag is polled.  The two
+// 64-row chunks right below the block (blocks cb + 1 and cb + 2 of a dense
+// front: the task's last two dependencies) are staged in LDS at the start and
+// applied after the far partials are reduced, with lane = column, each from
+// the hand-off buffer: the far waves, with their reduce-scatter, then run a
+// step ahead of the chain.
 template <typename T>
 __global__ __launch_bounds__(256) void k_bwd(
     const SolveTask* __restrict__ tasks, int* ticket, const long* __restrict__ fofs,
@@ -870,8 +863,8 @@ __global__ __launch_bounds__(256) void k_bwd(
   __shared__ int tk;
   __shared__ double part[4][SB];
   __shared__ double Ld[SB * LDSP];  // Ld[row][col] of the diagonal block
-  __shared__ double Ln[SB * LDSP];  // Ln[row][col] of the rows right below it
-  __shared__ double xn[SB];
+  __shared__ double Ln[2][SB * LDSP];  // Ln[h][row][col] of the two chunks right below it
+  __shared__ double xn[2][SB];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   if (tid == 0) tk = atomicAdd(ticket, 1);
   __syncthreads();
@@ -886,18 +879,19 @@ __global__ __launch_bounds__(256) void k_bwd(
   // task writes the block, and both stay off the dependency chain
   double yd = 0.0;
   if (wv == 0 && lane < kc) yd = xp[first + c0 + lane] * (1.0 / Dg[first + c0 + lane]);
-  if (nchunks > 0) {
+  for (int h = 0; h < 2 && h < nchunks; ++h) {
     // lane = row, wave g = columns g, g+4, ...: loads from clamped (valid,
     // lower-triangle) addresses all in flight, then the ones in range stored
-    const int nnear = min(SB, nr - rstart);
-    const int rrow = rstart + min(lane, nnear - 1);
+    const int r0h = rstart + h * SB;
+    const int nnear = min(SB, nr - r0h);
+    const int rrow = r0h + min(lane, nnear - 1);
     T v[16];
 #pragma unroll
     for (int q = 0; q < 16; ++q) v[q] = F[base + fpos(nr, rrow, c0 + min(wv + 4 * q, kc - 1))];
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
       const int c = wv + 4 * q;
-      Ln[lane * LDSP + c] = (lane < nnear && c < kc) ? (double)v[q] : 0.0;
+      Ln[h][lane * LDSP + c] = (lane < nnear && c < kc) ? (double)v[q] : 0.0;
     }
   }
   for (int c = wv; c < kc; c += 4)
@@ -907,7 +901,7 @@ __global__ __launch_bounds__(256) void k_bwd(
   double acc[SB];
 #pragma unroll
   for (int c = 0; c < SB; ++c) acc[c] = 0.0;
-  for (int q = nchunks - 1 - wv; q >= 1; q -= 4) {
+  for (int q = nchunks - 1 - wv; q >= 2; q -= 4) {
     const int R = rstart + q * SB;
     const int rr = R + lane;
     const bool rv = rr < nr;
@@ -948,24 +942,28 @@ __global__ __launch_bounds__(256) void k_bwd(
 #pragma unroll
     for (int c = 0; c < H; ++c) acc[H + c] += Lb[c] * xv;
   }
-  part[wv][lane] = (nchunks - 1 - wv >= 1) ? wave_reduce_scatter64(acc, lane) : 0.0;
+  part[wv][lane] = (nchunks - 1 - wv >= 2) ? wave_reduce_scatter64(acc, lane) : 0.0;
   __syncthreads();
   if (wv == 0) {
-    double pnear = 0.0;
-    if (nchunks > 0) {
-      // pivot rows right below the block (block cb + 1) are polled in the
-      // hand-off buffer; rows past the pivots are final since the parent is done
-      const int rr = rstart + lane;
+    // the second chunk first (its pivot rows were published a step earlier),
+    // then the first; pivot rows are polled in the hand-off buffer, rows past
+    // the pivots are final since the parent is done
+    double pn[2] = {0.0, 0.0};
+    for (int h = min(nchunks, 2) - 1; h >= 0; --h) {
+      const int rr = rstart + h * SB + lane;
       const bool piv = rr < nc;
       double xv = wait_value(&xh[first + rr], piv, err);
       if (!piv && rr < nr) xv = load_acq(&xp[rows[rowptr[s] + rr]]);
-      xn[lane] = xv;
+      xn[h][lane] = xv;
       __builtin_amdgcn_wave_barrier();
+      double p4[4] = {0.0, 0.0, 0.0, 0.0};  // four partial sums (shorter FMA chain)
 #pragma unroll
-      for (int r = 0; r < SB; ++r) pnear += Ln[r * LDSP + lane] * xn[r];  // lane = column
+      for (int r = 0; r < SB; ++r) p4[r & 3] += Ln[h][r * LDSP + lane] * xn[h][r];  // lane = column
+      pn[h] = (p4[0] + p4[1]) + (p4[2] + p4[3]);
     }
     double y = 0.0;
-    if (lane < kc) y = yd - (((part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane])) + pnear);
+    if (lane < kc)
+      y = yd - ((((part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane])) + pn[1]) + pn[0]);
     if (kc == SB) {
       // full block: the lane's column of L11 in registers (zero from the
       // diagonal up), then a branch-free chain of broadcast + FMA

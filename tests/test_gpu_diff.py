@@ -53,17 +53,27 @@ def test_backward_matches_finite_differences():
     rng = np.random.default_rng(3)
     w = p.meta["w"]
 
+    # The finite differences need loss values accurate to far below h times
+    # the 2e-2 tolerance: a perturbed solve counts when it ends Solved or
+    # stalls within 1e-6 residuals and gap of the optimum.  At the 1e-9
+    # tolerances these degenerate problems can stall a few decades short (the
+    # oracle, rerun on the same perturbed problems with 1e-15 relative noise
+    # on the data, stalls in 1 of 320 solves); a direction with a solve that
+    # neither converges nor stalls close is skipped.
     def loss(**kw):
         t = Solver()
         d = dict(P=p.P, q=p.q, A=p.A, b=p.b)
         d.update(kw)
         assert t.setup(d["P"], d["q"], d["A"], d["b"], ConeSpec(p.zero_dim, p.nonneg_dim, p.soc_dims), TIGHT)
         r = t.solve()
-        assert r.status in (Status.Solved, Status.MaxIterations)
-        return w @ r.x
+        close = max(r.info.primal_residual, r.info.dual_residual, abs(r.info.duality_gap)) < 1e-6
+        return w @ r.x if (r.status == Status.Solved or close) else None
 
     def d4(f):
-        return (f(-2 * h) - 8 * f(-h) + 8 * f(h) - f(2 * h)) / (12 * h)
+        v = [f(t) for t in (-2 * h, -h, h, 2 * h)]
+        if any(x is None for x in v):
+            return None
+        return (v[0] - 8 * v[1] + 8 * v[2] - v[3]) / (12 * h)
     fd, gv = [], []
     for i in rng.choice(p.n, 4, replace=False):
         e = np.zeros(p.n); e[i] = 1.0
@@ -83,8 +93,10 @@ def test_backward_matches_finite_differences():
             B = P.copy(); B.data[k] += t
             return loss(P=B)
         fd.append(d4(fp)); gv.append(g["P"].data[k])
-    err = relerr(gv, fd)
-    print("fd relerr", err)
+    ok = [k for k in range(len(fd)) if fd[k] is not None]
+    assert len(ok) >= 14, f"only {len(ok)} of {len(fd)} directions had four solved perturbations"
+    err = relerr([gv[k] for k in ok], [fd[k] for k in ok])
+    print("fd relerr", err, "directions", len(ok), "of", len(fd))
     assert err < 2e-2
 
 
