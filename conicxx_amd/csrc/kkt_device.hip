@@ -414,10 +414,19 @@ __global__ __launch_bounds__(256) void k_panel(
     const PanelTask* __restrict__ tasks, const long* __restrict__ fofs, const int* __restrict__ nrv,
     const int* __restrict__ firstv, const double* __restrict__ sgn, T* F, T* Fd, double* Dg, int mode,
     double eps, double delta, int* stats) {
-  __shared__ T L11[NB * LDSP];
-  __shared__ T X[PANEL_ROWS * LDSP];
-  __shared__ T colk[NB];
-  __shared__ T dinv[NB];
+  // one block in a fixed order, the broadcast column first: its reads then
+  // take immediate LDS offsets (below 64 KB) instead of an address register each
+  struct Lds {
+    T colk[NB];
+    T dinv[NB];
+    T L11[NB * LDSP];
+    T X[PANEL_ROWS * LDSP];
+  };
+  __shared__ Lds sm;
+  T* colk = sm.colk;
+  T* dinv = sm.dinv;
+  T* L11 = sm.L11;
+  T* X = sm.X;
   const PanelTask Tk = tasks[blockIdx.x];
   const int s = Tk.s, nr = nrv[s], first = firstv[s], a = Tk.a, w = Tk.w;
   const long base = fofs[s];
@@ -860,11 +869,21 @@ __global__ __launch_bounds__(256) void k_bwd(
     const int* __restrict__ rowptr, const int* __restrict__ rows, const int* __restrict__ parent,
     const int* __restrict__ nblkb, const int* __restrict__ flag_ofs, int* flags, int* done, int gen,
     const T* __restrict__ F, const double* __restrict__ Dg, double* xp, double* xh, int* err) {
-  __shared__ int tk;
-  __shared__ double part[4][SB];
-  __shared__ double Ld[SB * LDSP];  // Ld[row][col] of the diagonal block
-  __shared__ double Ln[2][SB * LDSP];  // Ln[h][row][col] of the two chunks right below it
-  __shared__ double xn[2][SB];
+  // one block in a fixed order: the broadcast vectors below 64 KB, where
+  // their reads take immediate offsets
+  struct Lds {
+    double xn[2][SB];
+    double part[4][SB];
+    double Ld[SB * LDSP];     // Ld[row][col] of the diagonal block
+    double Ln[2][SB * LDSP];  // Ln[h][row][col] of the two chunks right below it
+    int tk;
+  };
+  __shared__ Lds sm;
+  auto& xn = sm.xn;
+  auto& part = sm.part;
+  auto& Ld = sm.Ld;
+  auto& Ln = sm.Ln;
+  int& tk = sm.tk;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   if (tid == 0) tk = atomicAdd(ticket, 1);
   __syncthreads();
