@@ -787,21 +787,24 @@ __global__ __launch_bounds__(256) void k_fwd(
 #pragma unroll
     for (int c = 0; c < SB; ++c) acc += Lb[c] * bcast(yv, c);
   }
+  // full block: the lane's row of L11 in registers (zero from the diagonal
+  // on), read before the workgroup barrier, off the dependency chain
+  double lr[SB];
+  if (wv == 0 && kc == SB) {
+#pragma unroll
+    for (int k = 0; k < SB; ++k) {
+      const double v = Ld[lane * LDSP + k];
+      lr[k] = (k < lane) ? v : 0.0;
+    }
+  }
   part[wv][lane] = rv ? acc : 0.0;
   __syncthreads();
   if (wv == 0) {
     double val = 0.0;
     if (rv) val = rhs_r - ((part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]));
     if (kc == SB) {
-      // full block: the lane's row of L11 in registers (zero from the
-      // diagonal on), then a branch-free chain of broadcast + FMA; a zero
-      // entry leaves val unchanged while the broadcast value is finite
-      double lr[SB];
-#pragma unroll
-      for (int k = 0; k < SB; ++k) {
-        const double v = Ld[lane * LDSP + k];
-        lr[k] = (k < lane) ? v : 0.0;
-      }
+      // a branch-free chain of broadcast + FMA; a zero entry leaves val
+      // unchanged while the broadcast value is finite
 #pragma unroll
       for (int k = 0; k < SB; ++k) val -= lr[k] * bcast(val, k);
     } else {
@@ -964,6 +967,16 @@ __global__ __launch_bounds__(256) void k_bwd(
   part[wv][lane] = (nchunks - 1 - wv >= 2) ? wave_reduce_scatter64(acc, lane) : 0.0;
   __syncthreads();
   if (wv == 0) {
+    // full block: the lane's column of L11 in registers (zero from the
+    // diagonal up), read before the waits, off the dependency chain
+    double lc[SB];
+    if (kc == SB) {
+#pragma unroll
+      for (int k = 0; k < SB; ++k) {
+        const double v = Ld[k * LDSP + lane];
+        lc[k] = (lane < k) ? v : 0.0;
+      }
+    }
     // the second chunk first (its pivot rows were published a step earlier),
     // then the first; pivot rows are polled in the hand-off buffer, rows past
     // the pivots are final since the parent is done
@@ -984,14 +997,7 @@ __global__ __launch_bounds__(256) void k_bwd(
     if (lane < kc)
       y = yd - ((((part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane])) + pn[1]) + pn[0]);
     if (kc == SB) {
-      // full block: the lane's column of L11 in registers (zero from the
-      // diagonal up), then a branch-free chain of broadcast + FMA
-      double lc[SB];
-#pragma unroll
-      for (int k = 0; k < SB; ++k) {
-        const double v = Ld[k * LDSP + lane];
-        lc[k] = (lane < k) ? v : 0.0;
-      }
+      // a branch-free chain of broadcast + FMA
 #pragma unroll
       for (int k = SB - 1; k >= 0; --k) y -= lc[k] * bcast(y, k);
     } else {
