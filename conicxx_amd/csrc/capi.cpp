@@ -311,11 +311,10 @@ int cx_analyze_pattern(int N, const int* colptr, const int* rowind, int* perm, i
     if (perm)
       for (int k = 0; k < N; ++k) perm[k] = P.perm[k];
     if (colcount) {
-      // exact counts of the final (postordered) order
+      // exact counts of the final order (a topological order of its
+      // elimination tree, not necessarily a postorder: count in one)
       std::vector<int> par = cx::etree_of(K, P.perm);
-      std::vector<int> ident(N);
-      for (int k = 0; k < N; ++k) ident[k] = k;
-      std::vector<int> cc = cx::column_counts(K, P.perm, par, ident);
+      std::vector<int> cc = cx::column_counts(K, P.perm, par, cx::postorder(par));
       for (int k = 0; k < N; ++k) colcount[k] = cc[k];
     }
     if (stats8) {
