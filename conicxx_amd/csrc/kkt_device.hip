@@ -1245,7 +1245,6 @@ void DeviceKKT::build_schedule() {
   dslot_h_.clear();
   trail_h_.clear();
   trail_launch_flops_.clear();
-  trail_flops_static_ = 0;
   std::vector<int> rmap_ofs(ns + 1, 0);
   for (int s = 0; s < ns; ++s) rmap_ofs[s + 1] = rmap_ofs[s] + (P.nr(s) - P.sn_ncol[s]);
   // parent-relative maps again (host copy for segment search)
@@ -1354,7 +1353,6 @@ void DeviceKKT::build_schedule() {
       if ((int)trail_h_.size() > tbeg) {
         launches_.push_back({L_TRAIL, tbeg, (int)trail_h_.size()});
         trail_launch_flops_.push_back(fl);
-        trail_flops_static_ += fl;
       }
     }
   }

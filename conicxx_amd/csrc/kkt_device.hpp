@@ -97,9 +97,7 @@ class DeviceKKT {
   std::vector<PanelTask> panel_h_;
   std::vector<PanelTask> dslot_h_;  // one per sub-panel: where its diagonal block goes
   std::vector<TrailTask> trail_h_;
-  double trail_flops_static_ = 0;
   std::vector<double> trail_launch_flops_;
-  std::vector<double> diag_launch_flops_, trsm_launch_flops_;
   // device plan
   DevBuf<long> fofs_;
   DevBuf<int> nr_, nc_, first_, rowptr_, rows_, child_ptr_, child_;
@@ -147,7 +145,6 @@ class DeviceKKT {
     long timed = 0, all = 0;
   };
   KindRaw kraw_[T_NKINDS];
-  std::vector<double> trail_launch_flops_h_;
   double sweep_bytes_ = 0;
 };
 
