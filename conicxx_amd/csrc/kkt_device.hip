@@ -547,7 +547,6 @@ __global__ __launch_bounds__(128 * WN, (WN == 4 || sizeof(T) == 4) ? 4 : 2) void
   constexpr int NT = 128 * WN;          // threads
   constexpr int NI = 8 / WN;            // 16-column MFMA blocks per wave
   constexpr int LDA = TM + 4;
-  constexpr int RS = NT / TM;           // staged column stride per thread
   constexpr int PER = KC * TM / NT;     // staged elements per thread and operand
   __shared__ T As[KC * LDA];
   __shared__ T Bs[KC * LDA];
@@ -567,39 +566,50 @@ __global__ __launch_bounds__(128 * WN, (WN == 4 || sizeof(T) == 4) ? 4 : 2) void
   for (int mi = 0; mi < 4; ++mi)
 #pragma unroll
     for (int ni = 0; ni < NI; ++ni) acc[mi][ni] = (acc_t){0, 0, 0, 0};
-  const int lrow = tid & (TM - 1);  // staged row (0..127)
-  const int lk = tid / TM;          // first staged column
-  const int gi = Tk.ti + lrow, gj = Tk.tj + lrow;
-  const bool iok = gi < nr, jok = gj < nr;
-  // rows of the staged operands are below the staged columns (clamped rows to
-  // the last row of the front, also below them)
-  const T* Fa = F + base + (iok ? gi : nr - 1);
-  const T* Fb = F + base + (jok ? gj : nr - 1);
-  // Register prefetch of the next chunk, straight from clamped (always valid)
-  // addresses with no select, so all loads of a chunk are in flight together
-  // and only the two operand arrays stay live across the MFMA loop; the
-  // out-of-range values are zeroed on the way into LDS.  D of the chunk goes
-  // to LDS too and scales the B fragments as they are read.
-  T ra[PER], rb[PER];
+  // Staging: a thread loads 16 bytes of consecutive rows (VR of them) of a
+  // column at a time.  Rows of the staged operands lie below the staged
+  // columns; rows past the front run on into the next columns' storage (the
+  // factor buffer is padded past its last front for the tiles at its end)
+  // and are zeroed on the way into LDS, as are columns past the K range.
+  // Register prefetch of the next chunk, so all loads of a chunk are in
+  // flight together while the MFMAs run.  D of the chunk goes to LDS too and
+  // scales the B fragments as they are read.
+  constexpr int VR = 16 / (int)sizeof(T);  // rows per load
+  constexpr int RG = TM / VR;              // row groups of a column
+  constexpr int CS = NT / RG;              // column stride of a thread's loads
+  constexpr int NQ = KC / CS;              // loads per operand and chunk
+  static_assert(NQ * VR == PER, "staging covers the chunk");
+  typedef T vrow_t __attribute__((ext_vector_type(VR), aligned(sizeof(T))));
+  const int rg0 = (tid % RG) * VR;  // first staged row of the thread
+  const int lk = tid / RG;          // first staged column
+  const T* Fa = F + base + Tk.ti + rg0;
+  const T* Fb = F + base + Tk.tj + rg0;
+  vrow_t ra[NQ], rb[NQ];
   T dn = T(0);
   auto load_chunk = [&](int k0) {
 #pragma unroll
-    for (int q = 0; q < PER; ++q) {
-      const int kq = min(k0 + lk + RS * q, Tk.k1 - 1);
+    for (int q = 0; q < NQ; ++q) {
+      const int kq = min(k0 + lk + CS * q, Tk.k1 - 1);
       const long col = colofs(nr, kq) - kq;
-      ra[q] = Fa[col];
-      rb[q] = Fb[col];
+      ra[q] = *reinterpret_cast<const vrow_t*>(Fa + col);
+      rb[q] = *reinterpret_cast<const vrow_t*>(Fb + col);
     }
     if (tid < KC) dn = (T)Dg[first + min(k0 + tid, Tk.k1 - 1)];
   };
   load_chunk(Tk.k0);
   for (int k0 = Tk.k0; k0 < Tk.k1; k0 += KC) {
 #pragma unroll
-    for (int q = 0; q < PER; ++q) {
-      const int kk = lk + RS * q;
+    for (int q = 0; q < NQ; ++q) {
+      const int kk = lk + CS * q;
       const bool kin = k0 + kk < Tk.k1;
-      As[kk * LDA + lrow] = (kin && iok) ? ra[q] : T(0);
-      Bs[kk * LDA + lrow] = (kin && jok) ? rb[q] : T(0);
+      vrow_t va, vb;
+#pragma unroll
+      for (int e = 0; e < VR; ++e) {
+        va[e] = (kin && Tk.ti + rg0 + e < nr) ? ra[q][e] : T(0);
+        vb[e] = (kin && Tk.tj + rg0 + e < nr) ? rb[q][e] : T(0);
+      }
+      *reinterpret_cast<vrow_t*>(&As[kk * LDA + rg0]) = va;
+      *reinterpret_cast<vrow_t*>(&Bs[kk * LDA + rg0]) = vb;
     }
     if (tid < KC) Ds[tid] = dn;
     __syncthreads();
@@ -1126,7 +1136,7 @@ void DeviceKKT::setup(const LowerPattern& K, int nx, hipStream_t stream) {
   kdiag_.upload(kdiag, st_);
   rmap_ofs_.upload(rmap_ofs, st_);
   rmap_.upload(rmap, st_);
-  F_.alloc(fofs[ns]);
+  F_.alloc(fofs[ns] + TM);  // + TM: the trailing update's staging reads run past the last front
   D_.alloc(N);
   fstat_.alloc(2);
   dstat_.alloc(1);
