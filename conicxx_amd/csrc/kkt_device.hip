@@ -632,8 +632,38 @@ __global__ __launch_bounds__(128 * WN, (WN == 4 || sizeof(T) == 4) ? 4 : 2) void
     }
     __syncthreads();
   }
-  // write back: per row block mi, the old values are loaded together from
-  // clamped addresses, then the valid ones are stored
+  // write back.  An FP32 tile whose entries are all stored (strictly below
+  // the diagonal, inside the front and the column range) takes the vector
+  // path: a lane's four accumulator rows are consecutive rows of one column,
+  // read and written as one 16-byte vector.
+  if constexpr (sizeof(T) == 4) {
+    const bool interior = Tk.ti + TM <= nr && Tk.tj + TM <= Tk.chi && Tk.ti >= Tk.tj + TM && Tk.ti >= Tk.rlo;
+    if (interior) {
+      typedef T v4u __attribute__((ext_vector_type(4), aligned(sizeof(T))));
+#pragma unroll
+      for (int mi = 0; mi < 4; ++mi) {
+        v4u old[NI];
+        T* adr[NI];
+#pragma unroll
+        for (int ni = 0; ni < NI; ++ni) {
+          const int j = Tk.tj + wc + ni * 16 + (lane & 15);
+          const int i0 = Tk.ti + wr + mi * 16 + Mfma<T>::row(lane, 0);
+          adr[ni] = F + base + fpos(nr, i0, j);
+          old[ni] = *reinterpret_cast<const v4u*>(adr[ni]);
+        }
+#pragma unroll
+        for (int ni = 0; ni < NI; ++ni) {
+          v4u nv;
+#pragma unroll
+          for (int r = 0; r < 4; ++r) nv[r] = old[ni][r] - acc[mi][ni][r];
+          *reinterpret_cast<v4u*>(adr[ni]) = nv;
+        }
+      }
+      return;
+    }
+  }
+  // per row block mi, the old values are loaded together from clamped
+  // addresses, then the valid ones are stored
 #pragma unroll
   for (int mi = 0; mi < 4; ++mi) {
     T old[NI][4];
