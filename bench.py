@@ -31,7 +31,7 @@ os.environ.setdefault("CONICXX_KERNEL_TIMING", "4")
 HBM_PEAK_GBS = 8000.0
 # k_trail HBM bytes per launch on config 1 (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0
 # per the copy calibration, averaged over the launches of one solve)
-K_TRAIL_C1_TRAFFIC_BYTES = 3.16e8
+K_TRAIL_C1_TRAFFIC_BYTES = 3.17e8
 FP64_MFMA_PEAK_TFLOPS = 78.6
 
 
