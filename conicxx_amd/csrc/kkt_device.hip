@@ -632,25 +632,27 @@ __global__ __launch_bounds__(128 * WN, (WN == 4 || sizeof(T) == 4) ? 4 : 2) void
     }
     __syncthreads();
   }
-  // write back.  An FP32 tile whose entries are all stored (strictly below
-  // the diagonal, inside the front and the column range) takes the vector
-  // path: a lane's four accumulator rows are consecutive rows of one column,
-  // read and written as one 16-byte vector.
-  if constexpr (sizeof(T) == 4) {
-    const bool interior = Tk.ti + TM <= nr && Tk.tj + TM <= Tk.chi && Tk.ti >= Tk.tj + TM && Tk.ti >= Tk.rlo;
-    if (interior) {
-      typedef T v4u __attribute__((ext_vector_type(4), aligned(sizeof(T))));
+  // write back.  A tile whose entries are all stored (strictly below the
+  // diagonal, inside the front and the column range) skips the per-entry
+  // checks: one address per row block and column, the lane's four rows at
+  // fixed offsets from it; in FP32 they are consecutive rows, read and
+  // written as one 16-byte vector.
+  const bool interior = Tk.ti + TM <= nr && Tk.tj + TM <= Tk.chi && Tk.ti >= Tk.tj + TM && Tk.ti >= Tk.rlo;
+  if (interior) {
 #pragma unroll
-      for (int mi = 0; mi < 4; ++mi) {
+    for (int mi = 0; mi < 4; ++mi) {
+      T* adr[NI];
+#pragma unroll
+      for (int ni = 0; ni < NI; ++ni) {
+        const int j = Tk.tj + wc + ni * 16 + (lane & 15);
+        const int i0 = Tk.ti + wr + mi * 16 + Mfma<T>::row(lane, 0);
+        adr[ni] = F + base + fpos(nr, i0, j);
+      }
+      if constexpr (sizeof(T) == 4) {
+        typedef T v4u __attribute__((ext_vector_type(4), aligned(sizeof(T))));
         v4u old[NI];
-        T* adr[NI];
 #pragma unroll
-        for (int ni = 0; ni < NI; ++ni) {
-          const int j = Tk.tj + wc + ni * 16 + (lane & 15);
-          const int i0 = Tk.ti + wr + mi * 16 + Mfma<T>::row(lane, 0);
-          adr[ni] = F + base + fpos(nr, i0, j);
-          old[ni] = *reinterpret_cast<const v4u*>(adr[ni]);
-        }
+        for (int ni = 0; ni < NI; ++ni) old[ni] = *reinterpret_cast<const v4u*>(adr[ni]);
 #pragma unroll
         for (int ni = 0; ni < NI; ++ni) {
           v4u nv;
@@ -658,9 +660,20 @@ __global__ __launch_bounds__(128 * WN, (WN == 4 || sizeof(T) == 4) ? 4 : 2) void
           for (int r = 0; r < 4; ++r) nv[r] = old[ni][r] - acc[mi][ni][r];
           *reinterpret_cast<v4u*>(adr[ni]) = nv;
         }
+      } else {
+        T old[NI][4];
+#pragma unroll
+        for (int ni = 0; ni < NI; ++ni)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) old[ni][r] = adr[ni][Mfma<T>::row(lane, r) - Mfma<T>::row(lane, 0)];
+#pragma unroll
+        for (int ni = 0; ni < NI; ++ni)
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            adr[ni][Mfma<T>::row(lane, r) - Mfma<T>::row(lane, 0)] = old[ni][r] - acc[mi][ni][r];
       }
-      return;
     }
+    return;
   }
   // per row block mi, the old values are loaded together from clamped
   // addresses, then the valid ones are stored
