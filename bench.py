@@ -21,6 +21,8 @@ import platform
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 # HIP events around the launches of every 4th factorization and KKT solve: an
@@ -72,16 +74,86 @@ def cpu_baseline(full_flops, scale=0.1):
     }
 
 
-def batched_form(K=4096, timesteps=3, sample=32):
+def reduced_order(p):
+    """Order of the batch's reduced KKT matrix for problem p: the variables plus
+    every row except the bound rows (nonnegative rows with a single entry),
+    which the batched solver folds into the variable block."""
+    cnt = np.diff(p.A.tocsr().indptr)
+    lo, hi = p.zero_dim, p.zero_dim + p.nonneg_dim
+    return p.n + p.m - int((cnt[lo:hi] == 1).sum())
+
+
+def batch_work(probs):
+    """Per-problem algorithmic flops of one dense LDL^T of the reduced matrix
+    (order r: sum over columns of c(c-1) with c = r..1, i.e. (r^3 - r)/3) and of
+    one reduced solve with its refinement residual (forward and backward
+    substitution over the packed triangle 2r(r-1) + r, plus the residual
+    rhs - K x: 2 flops per stored entry of P (both triangles), A (twice: A x
+    and A'y) and the cone blocks)."""
+    r = np.array([reduced_order(p) for p in probs], dtype=np.float64)
+    f_fact = (r ** 3 - r) / 3.0
+    cone = np.array([p.nonneg_dim + sum(d * d for d in p.soc_dims) for p in probs], dtype=np.float64)
+    nnz = np.array([p.P_full().nnz + 2 * p.A.nnz for p in probs], dtype=np.float64)
+    f_solve = 2.0 * r * (r - 1.0) + r + 2.0 * (nnz + cone)
+    return r, f_fact, f_solve
+
+
+def oracle_contact_qps(probs, workers):
+    """The oracle (single-threaded restatement of the reference method) over
+    the given problems: a loop on one core, or a process pool."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_py
+    t0 = time.perf_counter()
+    if workers <= 1:
+        st = [_oracle_solve(p) for p in probs]
+    else:
+        import multiprocessing as mp
+        with mp.get_context("fork").Pool(workers) as pool:
+            st = pool.map(_oracle_solve, probs, chunksize=8)
+    t = time.perf_counter() - t0
+    return sum(1 for v in st if v == 1) / t, t, st
+
+
+def _oracle_solve(p):
+    import oracle_py
+    r = oracle_py.solve(p.P, p.q, p.A, p.b, p.zero_dim, p.nonneg_dim, p.soc_dims, warm_start=0)
+    return r["status"]
+
+
+def batched_cpu_baseline(K=4096, cpu_sample=256):
+    """The oracle on the host over the first timestep's contact problems: a
+    single-core loop over a sample and a process pool over the whole batch.
+    Runs before the process touches the GPU (the pool forks)."""
+    from conicxx_amd import problems as PR
+    p1 = []
+    for i in range(K):
+        p = PR.config2_problem(i)
+        dq, db = PR.config2_perturbation(i, 1, p.n, p.m)
+        p1.append(PR.Problem(p.P, p.q + dq, p.A, p.b + db, p.zero_dim, p.nonneg_dim, p.soc_dims))
+    cpu1, cpu1_s, _ = oracle_contact_qps(p1[:cpu_sample], 1)
+    workers = max(1, min(16, os.cpu_count() or 1))
+    cpuN, cpuN_s, ost = oracle_contact_qps(p1, workers)
+    return {"value": cpu1, "unit": "QPs/s", "cores": 1, "kind": "port",
+            "sample": f"oracle solves of the first {cpu_sample} timestep-1 problems on one core ({cpu1_s:.1f} s)",
+            "all_cores": {"value": cpuN, "cores": workers,
+                          "sample": f"all {K} timestep-1 problems over a pool of {workers} processes ({cpuN_s:.1f} s)",
+                          "non_solved": [i for i, v in enumerate(ost) if v != 1]},
+            "cpu_model": cpu_model(), "nproc": os.cpu_count()}
+
+
+def batched_form(K=4096, timesteps=100, plain_sample=512, cpu=None):
     """The 4096-problem contact batch: update_data + solve per timestep on the
-    batched engine, against a per-problem loop of the plain GPU solve timed on a
-    sample of the same problems (solve() only, setup untimed)."""
+    batched engine (T timesteps of the workload definition), against a
+    per-problem loop of the plain GPU solve timed on a sample of the same
+    timestep's problems (solve() only, setup untimed), and the oracle on the
+    host (single core on a sample, and a process pool over the whole batch)."""
     from conicxx_amd import BatchSolver, ConeSpec, Settings, Solver, Status
     from conicxx_amd import problems as PR
 
     t0 = time.perf_counter()
     probs = [PR.config2_problem(i) for i in range(K)]
     gen_s = time.perf_counter() - t0
+    r, f_fact, f_solve = batch_work(probs)
     st = Settings(warm_start=False)
     b = BatchSolver()
     assert b.setup([(p.P, p.q, p.A, p.b, ConeSpec(p.zero_dim, p.nonneg_dim, p.soc_dims)) for p in probs], st)
@@ -91,7 +163,8 @@ def batched_form(K=4096, timesteps=3, sample=32):
     tune_s = time.perf_counter() - t0
     print(f"[bench] batch autotune: {ncand} candidates in {tune_s:.1f} s -> {b.tuning()}", file=sys.stderr, flush=True)
     b.solve()  # graph capture with the chosen configuration
-    solved, wall, dev_ms = 0, 0.0, []
+    solved, wall, dev_ms, flops, nonsolved = 0, 0.0, [], 0.0, []
+    last = None
     for t in range(1, timesteps + 1):
         qs, bs = [], []
         for i, p in enumerate(probs):
@@ -103,20 +176,33 @@ def batched_form(K=4096, timesteps=3, sample=32):
         stt = b.solve()
         wall += time.perf_counter() - t1
         dev_ms.append(b.last_solve_ms())
-        solved += int((stt == int(Status.Solved)).sum())
-        print(f"[bench] batch timestep {t}: {int((stt == 1).sum())}/{K} Solved, {dev_ms[-1]:.0f} ms",
-              file=sys.stderr, flush=True)
+        ns = int((stt == int(Status.Solved)).sum())
+        solved += ns
+        nonsolved.append(K - ns)
+        cnt = np.array([[s["factorizations"], s["solves"]] for s in (b.stats(k) for k in range(K))], dtype=np.float64)
+        flops += float((cnt[:, 0] * f_fact + cnt[:, 1] * f_solve).sum())
+        if t == 1:
+            last = (stt.copy(), qs, bs, cnt)
+        if t == 1 or t % 10 == 0:
+            print(f"[bench] batch timestep {t}: {ns}/{K} Solved, {dev_ms[-1]:.0f} ms", file=sys.stderr, flush=True)
     qps = solved / wall
-    # per-problem loop of the plain GPU solver on a sample
-    loop_s, loop_solved = 0.0, 0
-    for i in range(sample):
+    dev_s = sum(dev_ms) / 1e3
+    achieved = flops / dev_s / 1e12
+    # compulsory HBM bytes per timestep: P, A (values + indices), q, b in; x, s, z out
+    hbm = sum(12.0 * (p.P.nnz + p.A.nnz) + 8.0 * (p.n + p.m) + 8.0 * (p.n + 2 * p.m) for p in probs)
+    # per-problem loop of the plain GPU solver on the first timestep's problems
+    st1, qs1, bs1, cnt1 = last
+    loop_s, loop_solved, mism = 0.0, 0, []
+    s = Solver()
+    for i in range(plain_sample):
         p = probs[i]
-        s = Solver()
-        assert s.setup(p.P, p.q, p.A, p.b, ConeSpec(p.zero_dim, p.nonneg_dim, p.soc_dims), st)
+        assert s.setup(p.P, qs1[i], p.A, bs1[i], ConeSpec(p.zero_dim, p.nonneg_dim, p.soc_dims), st)
         t1 = time.perf_counter()
-        r = s.solve()
+        res = s.solve()
         loop_s += time.perf_counter() - t1
-        loop_solved += r.status == Status.Solved
+        loop_solved += res.status == Status.Solved
+        if int(res.status) != int(st1[i]):
+            mism.append(i)
     loop_qps = loop_solved / loop_s
     return {
         "workload": f"config2: {K} friction-contact QPs (c~U{{4..64}}, n=3c, 2n box rows + c SOC dim 3), "
@@ -124,9 +210,22 @@ def batched_form(K=4096, timesteps=3, sample=32):
         "qps": qps, "unit": "QPs/s", "ms_per_timestep": 1e3 * wall / timesteps,
         "device_ms_per_timestep": sum(dev_ms) / len(dev_ms),
         "solved_fraction": solved / (K * timesteps),
+        "non_solved_per_timestep": {"min": min(nonsolved), "max": max(nonsolved),
+                                    "mean": sum(nonsolved) / len(nonsolved)},
         "plain_loop_qps": loop_qps,
-        "plain_loop_sample": f"{sample} problems of the same batch, one plain Solver each, solve() timed",
+        "plain_loop_sample": f"the first {plain_sample} problems of timestep 1, one plain Solver, solve() timed",
+        "status_mismatches_vs_plain": mism,
         "speedup_vs_plain_loop": qps / loop_qps, "tuning": b.tuning(), "autotune_candidates": ncand,
+        "roofline": {"bound": "mfma", "kernel": "k_batch_solve (persistent, FP64)",
+                     "achieved": achieved, "peak": FP64_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_MFMA_PEAK_TFLOPS,
+                     "work": "sum over problems and timesteps of factorizations x (r^3-r)/3 + reduced solves x "
+                             "(2r(r-1)+r + refinement residual), r = reduced order (mean "
+                             f"{float(r.mean()):.0f}, max {int(r.max())}); device time from HIP events around "
+                             "each timestep's graph",
+                     "flops_per_timestep": flops / timesteps,
+                     "compulsory_hbm_gbs": hbm / (dev_s / timesteps) / 1e9},
+        "cpu_baseline": cpu,
         "autotune_s": tune_s, "generate_s": gen_s,
     }
 
@@ -267,6 +366,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--timesteps", type=int, default=100,
+                    help="update_data()+solve() timesteps of the batched form (the workload defines 100)")
     ap.add_argument("--no-forms", action="store_true",
                     help="skip the batched, differentiation and precision forms")
     ap.add_argument("--form", choices=["batched", "differentiation", "precision"],
@@ -281,9 +382,16 @@ def main():
         sys.exit(2)
 
     if args.form:
-        fn = {"batched": batched_form, "differentiation": differentiation_form, "precision": precision_form}
-        print(json.dumps({"form": args.form, **fn[args.form]()}))
+        if args.form == "batched":
+            cpu = None if args.no_cpu_baseline else batched_cpu_baseline()
+            res = batched_form(timesteps=args.timesteps, cpu=cpu)
+        else:
+            res = {"differentiation": differentiation_form, "precision": precision_form}[args.form]()
+        print(json.dumps({"form": args.form, **res}))
         return
+    # the host-only oracle baseline of the batched form forks a process pool:
+    # it runs before this process touches the GPU
+    batch_cpu = None if (args.no_forms or args.no_cpu_baseline) else batched_cpu_baseline()
 
     import numpy as np  # noqa: F401
     from conicxx_amd import ConeSpec, Settings, Solver, Status
@@ -372,7 +480,7 @@ def main():
         "trail_tflops": trail_tflops,
     }
     if not args.no_forms:
-        out["batched"] = batched_form()
+        out["batched"] = batched_form(timesteps=args.timesteps, cpu=batch_cpu)
         out["differentiation"] = differentiation_form()
         out["precision"] = precision_form()
     if not args.no_cpu_baseline:

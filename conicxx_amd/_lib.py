@@ -75,6 +75,7 @@ SIGNATURES = [
     ("cx_get_profile", None, [ctypes.c_void_p, _f64p]),
     ("cx_get_kernel_stats", ctypes.c_int, [ctypes.c_void_p, _f64p, ctypes.c_int]),
     ("cx_backward", ctypes.c_int, [ctypes.c_void_p, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p]),
+    ("cx_kkt_factor_solve", ctypes.c_double, [ctypes.c_void_p, _f64p, _f64p, _f64p, _f64p, ctypes.c_int, _i32p]),
     ("cx_batch_create", ctypes.c_void_p, []),
     ("cx_batch_destroy", None, [ctypes.c_void_p]),
     ("cx_batch_setup", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _i32p, _i32p, ctypes.POINTER(_i32p),

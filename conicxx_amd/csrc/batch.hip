@@ -277,6 +277,45 @@ __device__ void unit_shift(const PV& p, double* x, double alpha) {
   __syncthreads();
 }
 
+// Entry (a, c), a >= c, of SOC block k's Hs as the factored matrix holds it,
+// with the adjoint's tangent correction when p.tcorr is set
+__device__ inline double hs_stored(const PV& p, int k, int off, bool id, double be, double vvk, const double* v,
+                                   int a, int c) {
+  if (id) return (a == c) ? 1.0 : 0.0;
+  double h = soc_hs_entry(be, vvk, v, a, c);
+  if (p.tcorr && a > 0 && c > 0) {
+    const double* u = p.tdir + off;
+    h += p.tcorr[k] * ((a == c ? 1.0 : 0.0) - u[a] * u[c]);
+  }
+  return h;
+}
+// out = -(H x + y) with H the (2,2) block's cone part exactly as assemble_M
+// stores it (diagonal orthant, stored SOC entries incl. the tangent correction):
+// the product the refinement residual needs, so that refinement converges to
+// the solution of the matrix that was factored (kkt_system.cpp:296-311 refines
+// against the assembled K)
+__device__ void neg_hs_plus_stored(const PV& p, const double* x, const double* y, double* out) {
+  for (int i = threadIdx.x; i < p.nz; i += BG) out[i] = -y[i];
+  for (int i = threadIdx.x; i < p.nn; i += BG) {
+    const int r = p.nz + i;
+    out[r] = -(p.C.hsd[r] * x[r] + y[r]);
+  }
+  for (int k = threadIdx.x; k < p.nsoc; k += BG) {
+    const int off = p.C.soc_off[k], d = p.C.soc_dim[k];
+    const double* v = p.C.v + off;
+    const double be = p.C.beta[k], vvk = p.C.vv[k];
+    const bool id = p.C.ident[k] != 0;
+    for (int a = 0; a < d; ++a) {
+      double acc = 0.0;
+      for (int c = 0; c < d; ++c)
+        acc += (a >= c ? hs_stored(p, k, off, id, be, vvk, v, a, c) : hs_stored(p, k, off, id, be, vvk, v, c, a)) *
+               x[off + c];
+      out[off + a] = -(acc + y[off + a]);
+    }
+  }
+  __syncthreads();
+}
+
 // ---------------------------------------------------------------------------
 // KKT: M assembly (bound rows folded into the P block, every other row kept)
 // ---------------------------------------------------------------------------
@@ -322,17 +361,7 @@ __device__ void assemble_M(const BatchArgs& A, const PV& p, double dp, double da
     for (int c = 0; c < d; ++c) {
       const long cb = colbase(k0 + c, nr);
       for (int a = c; a < d; ++a) {
-        double h;
-        if (id)
-          h = (a == c) ? 1.0 : 0.0;
-        else
-          h = be * be * (4.0 * vvk * v[a] * v[c] - 2.0 * v[a] * (jsign(c) * v[c]) -
-                         2.0 * (jsign(a) * v[a]) * v[c] + (a == c ? 1.0 : 0.0));
-        if (p.tcorr && !id && a > 0 && c > 0) {
-          const double* u = p.tdir + off;
-          h += p.tcorr[k] * ((a == c ? 1.0 : 0.0) - u[a] * u[c]);
-        }
-        M[cb + (a - c)] = -(h + (a == c ? da : 0.0));
+        M[cb + (a - c)] = -(hs_stored(p, k, off, id, be, vvk, v, a, c) + (a == c ? da : 0.0));
       }
     }
   }
@@ -660,8 +689,7 @@ __device__ void kkt_residual(const BatchArgs& A, PV& p, const double* rhs, const
   atmul(A, p, x + n, p.wn2);
   amul(A, p, x, p.wm);
   vfill(p.m, 0.0, p.wm2);
-  neg_hs_plus(p, x + n, p.wm2, p.wm2);  // -Hs y (each entry read and written by one thread)
-  if (p.tcorr) soc_tangent_apply(p, x + n, p.wm2);
+  neg_hs_plus_stored(p, x + n, p.wm2, p.wm2);  // -H y (each block read and written by one thread)
   for (int j = threadIdx.x; j < n; j += BG) res[j] = rhs[j] - p.wn[j] - A.sreg_p * x[j] - p.wn2[j];
   for (int i = threadIdx.x; i < p.m; i += BG) res[n + i] = rhs[n + i] - p.wm[i] - p.wm2[i] + A.sreg_a * x[n + i];
   __syncthreads();

@@ -2,6 +2,7 @@
 // crosses this boundary: HIP or allocation failures become return codes or
 // a NumericalError status, as the reference's Solver never throws
 // (include/conicxx/solver.h:43-47, 86-89).
+#include <cmath>
 #include <cstdio>
 #include <exception>
 #include <new>
@@ -169,6 +170,18 @@ int cx_backward(cx_solver* h, const double* dl_dx, const double* dl_ds, const do
   } catch (const std::exception& e) {
     std::fprintf(stderr, "[conicxx_amd] backward failed: %s\n", e.what());
     return 0;
+  }
+}
+
+double cx_kkt_factor_solve(cx_solver* h, const double* s, const double* z, const double* rhs, double* x,
+                           int refine_max_iter, int* factorizations) {
+  if (factorizations) *factorizations = 0;
+  if (!h || !rhs || !x) return NAN;
+  try {
+    return h->impl->kkt_factor_solve(s, z, rhs, x, refine_max_iter, factorizations);
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "[conicxx_amd] kkt_factor_solve failed: %s\n", e.what());
+    return NAN;
   }
 }
 

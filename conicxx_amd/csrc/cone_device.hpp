@@ -135,6 +135,18 @@ __device__ inline void soc_scaling(int lane, int off, int d, const double* s, co
   }
 }
 
+// Entry (a, b), a >= b, of one SOC block's Hs = W W = beta^2 (2 v v' - J)^2 as
+// the KKT matrix stores it (the reference writes the dense Hs block into K,
+// kkt_system.cpp:139-150).  Every product that must agree with the factored
+// matrix (the refinement residual) uses these entries: near the cone boundary
+// |v| is large and this entry-wise form and the rank-one product below differ
+// by far more than a rounding of the solution.
+__device__ inline double soc_hs_entry(double beta, double vv, const double* v, int a, int b) {
+  const double va = v[a], vb = v[b];
+  return beta * beta *
+         (4.0 * vv * va * vb - 2.0 * va * (jsign(b) * vb) - 2.0 * (jsign(a) * va) * vb + (a == b ? 1.0 : 0.0));
+}
+
 // out = -(Hs x + y)
 template <int T>
 __device__ inline void soc_neg_hs_plus(int lane, const ConeDev& C, int k, const double* x, const double* y,

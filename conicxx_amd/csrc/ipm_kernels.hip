@@ -294,12 +294,7 @@ __global__ void k_hs_scatter(ConeDev C, int nhs, const int* slot, const int* kin
     if (C.ident[k]) {
       h = (a == b) ? 1.0 : 0.0;
     } else {
-      const double* v = C.v + C.soc_off[k];
-      const double be = C.beta[k];
-      const double va = v[a], vb = v[b];
-      h = be * be *
-          (4.0 * C.vv[k] * va * vb - 2.0 * va * (jsign(b) * vb) - 2.0 * (jsign(a) * va) * vb +
-           (a == b ? 1.0 : 0.0));
+      h = soc_hs_entry(C.beta[k], C.vv[k], C.v + C.soc_off[k], a, b);
       if (tcorr && a > 0 && b > 0) {
         const double* u = tdir + C.soc_off[k];
         h += tcorr[k] * ((a == b ? 1.0 : 0.0) - u[a] * u[b]);

@@ -683,6 +683,41 @@ int Solver::backward(const double* wx, const double* ws, const double* wz, doubl
   return 1;
 }
 
+// KktSystem::updateScalingAndFactorize + KktSystem::solve on their own
+// (kkt_system.cpp:139-150, 296-311): the NT scaling at (s, z) (the identity of
+// construction when NULL) is written into K, K is factorized with the retry
+// ladder, and rhs is solved with up to refine_max_iter refinement steps
+// (negative: the settings' value).  Vectors are in the solver's internal
+// (equilibrated) space.  Returns the final relative residual |rhs - K x| / |rhs|
+// against the statically regularized K, NaN on a failed factorization.
+double Solver::kkt_factor_solve(const double* s, const double* z, const double* rhs, double* x,
+                                int refine_max_iter, int* factorizations) {
+  if (!setup_done_) return NAN;
+  bw_valid_ = false;
+  fp32_ = st_.factor_precision == CX_FACTOR_FP32_REFINED;
+  prof_nfact_ = prof_nsolve_ = prof_nfact32_ = prof_fp64_fallback_ = 0;
+  if (s && z && m_ > 0) {
+    s_.upload(s, m_, stream_);
+    z_.upload(z, m_, stream_);
+    launch_cone_scaling(cone_, s_.p, z_.p, stream_);
+  } else {
+    launch_cone_scaling(cone_, identity_.p, identity_.p, stream_);
+  }
+  double rel = NAN;
+  if (kkt_update_scaling()) {
+    rhs_.upload(rhs, N_, stream_);
+    const int keep = st_.refine_max_iter;
+    if (refine_max_iter >= 0) st_.refine_max_iter = refine_max_iter;
+    rel = kkt_solve(rhs_.p, sol_.p);
+    st_.refine_max_iter = keep;
+    sol_.download(x, N_, stream_);
+    CX_HIP(hipStreamSynchronize(stream_));
+  }
+  if (factorizations) *factorizations = prof_nfact_;
+  fp32_ = false;
+  return rel;
+}
+
 // ---------------------------------------------------------------------------
 // IPM pieces: solver_impl.cpp
 // ---------------------------------------------------------------------------

@@ -38,6 +38,10 @@ class Solver {
   int kernel_stats(double* out, int max_kinds) const;
   int backward(const double* wx, const double* ws, const double* wz, double* gP, double* gq,
                double* gA, double* gb);
+  // One KKT factorization (NT scaling at (s, z), NULL = identity) and one
+  // refined solve on the current data (kkt_system.cpp:139-150, 296-311)
+  double kkt_factor_solve(const double* s, const double* z, const double* rhs, double* x, int refine_max_iter,
+                          int* factorizations);
 
  private:
   // KKT (kkt_system.cpp)

@@ -295,6 +295,32 @@ class Solver:
         return {self.KERNEL_CLASSES[k]: dict(ms=out[3 * k], launches=int(out[3 * k + 1]), work=out[3 * k + 2])
                 for k in range(n)}
 
+    def kkt_factor_solve(self, rhs, s=None, z=None, refine_max_iter=-1):
+        """KktSystem::updateScalingAndFactorize + KktSystem::solve
+        (kkt_system.cpp:139-150, 296-311) on the current data: the NT scaling at
+        (s, z) (None: the identity scaling), one factorization with the retry
+        ladder, one solve of K x = rhs with up to refine_max_iter refinement
+        steps (negative: the settings' value).  Vectors are in the solver's
+        internal space (the original units with equilibrate=False).  Returns
+        (x, relative residual, numeric factorizations run)."""
+        n, m = self._n, self._m
+        r, rp = _lib.f64(np.asarray(rhs, dtype=np.float64).reshape(-1))
+        if r.size != n + m:
+            raise ValueError("rhs must have n + m entries")
+        keep = [r]
+        sp_ = zp_ = _lib.NULL_F
+        if s is not None:
+            a, sp_ = _lib.f64(np.asarray(s, dtype=np.float64).reshape(-1))
+            b, zp_ = _lib.f64(np.asarray(z, dtype=np.float64).reshape(-1))
+            if a.size != m or b.size != m:
+                raise ValueError("s and z must have m entries")
+            keep += [a, b]
+        x = np.zeros(n + m)
+        nf = ctypes.c_int(0)
+        rel = _lib.lib().cx_kkt_factor_solve(self._h, sp_, zp_, rp, x.ctypes.data_as(_lib._f64p),
+                                             int(refine_max_iter), ctypes.byref(nf))
+        return x, float(rel), int(nf.value)
+
     def backward(self, dl_dx=None, dl_ds=None, dl_dz=None):
         """Adjoint pass of the last Solved solve: gradients of a loss L given
         dL/dx, dL/ds, dL/dz (None = zero).  Returns dict(P=csc on P's pattern,

@@ -157,6 +157,23 @@ int cx_get_kernel_stats(const cx_solver* h, double* out, int max_kinds);
 int cx_backward(cx_solver* h, const double* dl_dx, const double* dl_ds, const double* dl_dz,
                 double* dP, double* dq, double* dA, double* db);
 
+/* KktSystem::updateScalingAndFactorize followed by KktSystem::solve
+ * (src/conicxx/kkt/kkt_system.cpp:139-150, 296-311), as the reference's own KKT
+ * tests drive them (test/kkt/test_kkt_system.cpp:32-60): on the data of the last
+ * setup()/updateData(), writes the NT scaling at (s, z) into the KKT matrix
+ * (s or z NULL: the identity scaling of construction), factorizes it with the
+ * x10 retry ladder and solves K x = rhs (N = n + m entries) with up to
+ * refine_max_iter refinement steps (negative: settings.refine_max_iter), with
+ * the x1000 escalation on a non-finite solve.  K is the statically regularized
+ * [P + static_reg_P I, A'; A, -(Hs + static_reg_A I)] in the solver's internal
+ * space (the original units when settings.equilibrate = 0).  factor_precision
+ * applies as in solve().  Writes the number of numeric factorizations it ran
+ * (retries and escalations included) to *factorizations when non-NULL.  Returns
+ * |rhs - K x| / |rhs|, NaN if no factorization was accepted or the handle has
+ * no setup. */
+double cx_kkt_factor_solve(cx_solver* h, const double* s, const double* z, const double* rhs, double* x,
+                           int refine_max_iter, int* factorizations);
+
 /* ---- Batched form: many independent problems in one persistent launch ----
  * No counterpart in the reference (callers there loop over Solver objects,
  * README.md:11-14).  Problem k has the same arguments as cx_setup, passed as
