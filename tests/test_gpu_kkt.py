@@ -175,8 +175,10 @@ def test_update_data_reproduces_fresh_setup(oracle):
     the same KKT solve as a fresh setup with those values."""
     p = PR.config0(seed=0)
     rng = np.random.default_rng(11)
-    P2 = p.P.copy()
-    P2.data = P2.data * (1.0 + 0.1 * rng.uniform(size=P2.nnz))
+    P2 = p.P.copy()  # 1.3 P + 0.05 I: positive definite on the same pattern
+    P2.data = P2.data * 1.3
+    col = np.repeat(np.arange(p.n), np.diff(P2.indptr))
+    P2.data[P2.indices == col] += 0.05
     A2 = p.A.copy()
     A2.data = A2.data * (1.0 + 0.2 * rng.standard_normal(A2.nnz))
     p2 = PR.Problem(P2, p.q, A2, p.b, p.zero_dim, p.nonneg_dim, p.soc_dims)
