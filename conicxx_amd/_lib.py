@@ -63,6 +63,8 @@ SIGNATURES = [
                                 ctypes.c_int, _i32p, ctypes.POINTER(CxSettings)]),
     ("cx_update_data", ctypes.c_int, [ctypes.c_void_p, _i32p, _i32p, _f64p, _f64p, _i32p, _i32p,
                                       _f64p, _f64p]),
+    ("cx_update_data_noalloc", ctypes.c_int, [ctypes.c_void_p, _i32p, _i32p, _f64p, _f64p, _i32p, _i32p,
+                                              _f64p, _f64p]),
     ("cx_set_warm_start", None, [ctypes.c_void_p, _f64p, _f64p, _f64p]),
     ("cx_set_settings", None, [ctypes.c_void_p, ctypes.POINTER(CxSettings)]),
     ("cx_get_settings", None, [ctypes.c_void_p, ctypes.POINTER(CxSettings)]),

@@ -100,6 +100,18 @@ int cx_update_data(cx_solver* h, const int* P_colptr, const int* P_rowind, const
   }
 }
 
+int cx_update_data_noalloc(cx_solver* h, const int* P_colptr, const int* P_rowind, const double* P_val,
+                           const double* q, const int* A_colptr, const int* A_rowind, const double* A_val,
+                           const double* b) {
+  if (!h) return 0;
+  try {
+    return h->impl->update_data_noalloc(P_colptr, P_rowind, P_val, q, A_colptr, A_rowind, A_val, b) ? 1 : 0;
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "[conicxx_amd] updateData (no allocation) failed: %s\n", e.what());
+    return 0;
+  }
+}
+
 void cx_set_warm_start(cx_solver* h, const double* x, const double* s, const double* z) {
   if (!h) return;
   try {

@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <numeric>
 
 #include "solver.hpp"
@@ -41,7 +42,41 @@ RedOp nonfin_op(const double* a, int len) { return RedOp{RED_NONFIN, len, a, nul
 
 }  // namespace
 
+// Staging for update_data_noalloc: one array per GPU for the whole process,
+// shared by every handle on that GPU.  Host code never uses the name itself
+// (a host-side stand-in): it takes the device address from scratch_address().
+static __device__ __align__(256) double scratch[Solver::kScratchDoubles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[kMaxDevices] = {};
+
+// Device address of `scratch` on GPU dev, looked up once per device with that
+// device current; nullptr if the lookup fails.
+double* scratch_address(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    int cur = -1;
+    if (hipGetDevice(&cur) != hipSuccess) return nullptr;
+    if (cur != dev && hipSetDevice(dev) != hipSuccess) return nullptr;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != dev) (void)hipSetDevice(cur);
+    if (e != hipSuccess || !p) {
+      std::fprintf(stderr, "[conicxx_amd] hipGetSymbolAddress(scratch) failed on device %d: %s\n", dev,
+                   hipGetErrorString(e));
+      return nullptr;
+    }
+    g_scratch[dev] = static_cast<double*>(p);
+  }
+  return g_scratch[dev];
+}
+}  // namespace
+
 Solver::Solver() {
+  CX_HIP(hipGetDevice(&device_));
   CX_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   CX_HIP(hipHostMalloc((void**)&host_red_, 64 * sizeof(double), hipHostMallocDefault));
   red_partial_.alloc(RED_MAX_OPS * RED_BLOCKS * 3);
@@ -596,6 +631,80 @@ bool Solver::update_data(const int* Pp, const int* Pi, const double* Px, const d
       for (int i = 0; i < m_; ++i) bv[i] *= ee_[i];
     db_.upload(bv, stream_);
   }
+  CX_HIP(hipStreamSynchronize(stream_));
+  return true;
+}
+
+// update_data with every staged value in the per-GPU scratch array instead of
+// freshly allocated device buffers, and the q / b rescaling on the device (the
+// same products the host does in update_data, so results are identical).  The
+// same checks and the same order of K updates as update_data; rejects any
+// piece longer than kScratchDoubles before touching the device.
+bool Solver::update_data_noalloc(const int* Pp, const int* Pi, const double* Px, const double* q,
+                                 const int* Ap, const int* Ai, const double* Ax, const double* b) {
+  if (!setup_done_) return false;
+  const bool hasP = Px != nullptr, hasA = Ax != nullptr;
+  const long nP = (long)Pi_ref_.size(), nA = (long)Ai_ref_.size();
+  if ((hasP && nP > kScratchDoubles) || (hasA && nA > kScratchDoubles) || (q && n_ > kScratchDoubles) ||
+      (b && m_ > kScratchDoubles)) {
+    std::fprintf(stderr,
+                 "[conicxx_amd] update_data_noalloc: nnz(P) %ld, nnz(A) %ld, n %d, m %d; each staged piece is "
+                 "limited to %ld values\n",
+                 nP, nA, n_, m_, kScratchDoubles);
+    return false;
+  }
+  double* stage = scratch_address(device_);
+  if (!stage) return false;
+  bw_valid_ = false;
+  auto same = [](const int* p, const int* i, int cols, const std::vector<int>& pr, const std::vector<int>& ir) {
+    if (!p || (!i && pr[cols] > 0)) return false;
+    for (int c = 0; c <= cols; ++c)
+      if (p[c] != pr[c]) return false;
+    for (int k = 0; k < pr[cols]; ++k)
+      if (i[k] != ir[k]) return false;
+    return true;
+  };
+  if (hasP) {
+    if (!same(Pp, Pi, n_, Pp_ref_, Pi_ref_)) return false;
+    if (nP) CX_HIP(hipMemcpyAsync(stage, Px, nP * sizeof(double), hipMemcpyHostToDevice, stream_));
+    if (st_.equilibrate)
+      launch_scale_csc(n_, dPp_.p, dPi_.p, stage, ec_, d_ed_.p, d_ed_.p, dPx_new_.p, stream_);
+    else
+      launch_copy((int)nP, stage, dPx_new_.p, stream_);
+    launch_scatter(npd_, pd_slot_.p, pd_src_.p, dPx_new_.p, st_.static_reg_P, Kx_.p, stream_);
+    launch_scatter(npo_, po_slot_.p, po_src_.p, dPx_new_.p, 0.0, Kx_.p, stream_);
+  }
+  if (hasA) {
+    if (!same(Ap, Ai, n_, Ap_ref_, Ai_ref_)) {
+      CX_HIP(hipStreamSynchronize(stream_));
+      return false;
+    }
+    // the stream orders this copy after the kernels that read P's values from the array
+    if (nA) CX_HIP(hipMemcpyAsync(stage, Ax, nA * sizeof(double), hipMemcpyHostToDevice, stream_));
+    if (st_.equilibrate)
+      launch_scale_csc(n_, dAp_.p, dAi_.p, stage, 1.0, d_ee_.p, d_ed_.p, dAx_new_.p, stream_);
+    else
+      launch_copy((int)nA, stage, dAx_new_.p, stream_);
+    launch_scatter(na_, a_slot_.p, a_src_.p, dAx_new_.p, 0.0, Kx_.p, stream_);
+  }
+  if ((hasP || hasA || !factorized_) && !factor_retry()) return false;
+  if (hasP) launch_copy((int)nP, dPx_new_.p, dPx_.p, stream_);
+  if (hasA) launch_copy((int)nA, dAx_new_.p, dAx_.p, stream_);
+  if (q && n_) {
+    CX_HIP(hipMemcpyAsync(stage, q, n_ * sizeof(double), hipMemcpyHostToDevice, stream_));
+    if (st_.equilibrate)
+      launch_vscale(n_, d_ed_.p, ec_, stage, dq_.p, stream_);
+    else
+      launch_copy(n_, stage, dq_.p, stream_);
+  }
+  if (b && m_) {
+    CX_HIP(hipMemcpyAsync(stage, b, m_ * sizeof(double), hipMemcpyHostToDevice, stream_));
+    if (st_.equilibrate)
+      launch_vscale(m_, d_ee_.p, 1.0, stage, db_.p, stream_);
+    else
+      launch_copy(m_, stage, db_.p, stream_);
+  }
+  // the array is free for the next call once this returns
   CX_HIP(hipStreamSynchronize(stream_));
   return true;
 }

@@ -27,6 +27,12 @@ class Solver {
              int nonneg_dim, int nsoc, const int* soc_dims, const cx_settings& st);
   bool update_data(const int* Pp, const int* Pi, const double* Px, const double* q, const int* Ap,
                    const int* Ai, const double* Ax, const double* b);
+  // update_data without any device or host allocation: values are staged in
+  // the library's per-GPU static scratch array (at most kScratchDoubles values
+  // per piece); calls that overlap on one GPU race on that array
+  bool update_data_noalloc(const int* Pp, const int* Pi, const double* Px, const double* q, const int* Ap,
+                           const int* Ai, const double* Ax, const double* b);
+  static constexpr long kScratchDoubles = 1L << 20;
   void set_warm_start(const double* x, const double* s, const double* z);
   void set_settings(const cx_settings& st) { st_ = st; }
   const cx_settings& settings() const { return st_; }
@@ -76,6 +82,7 @@ class Solver {
                       const double* c = nullptr, int lc = 0);
 
   hipStream_t stream_ = nullptr;
+  int device_ = 0;  // the GPU current at construction (owner of stream_)
   cx_settings st_{};
   bool setup_done_ = false;
   int n_ = 0, m_ = 0, N_ = 0;

@@ -97,6 +97,21 @@ int cx_update_data(cx_solver* h, const int* P_colptr, const int* P_rowind, const
                    const double* q, const int* A_colptr, const int* A_rowind,
                    const double* A_val, const double* b);
 
+/* cx_update_data without any allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * checks, K updates and results as cx_update_data, but every staged value
+ * goes through one static device array per GPU (1,048,576 doubles = 8 MiB,
+ * held by each GPU the library runs on until the process ends) instead of
+ * freshly allocated buffers, and it calls no hipMalloc/hipFree.  Each of
+ * nnz(P), nnz(A), n and m (of the pieces passed) must be at most 1,048,576:
+ * a larger piece is rejected (returns 0, stderr message) before anything
+ * changes.  Every call on a GPU shares that GPU's array: calls that overlap
+ * on one GPU (other threads, other handles) race; sequential calls are safe. */
+int cx_update_data_noalloc(cx_solver* h, const int* P_colptr, const int* P_rowind, const double* P_val,
+                           const double* q, const int* A_colptr, const int* A_rowind,
+                           const double* A_val, const double* b);
+
 /* Solver::setWarmStart(x, s, z) in original units (solver.h:81; solver_impl.cpp:93-100). */
 void cx_set_warm_start(cx_solver* h, const double* x, const double* s, const double* z);
 
