@@ -97,6 +97,9 @@ SIGNATURES = [
     ("cx_batch_backward", ctypes.c_int, [ctypes.c_void_p] + [ctypes.POINTER(_f64p)] * 7),
     ("cx_batch_last_backward_ms", ctypes.c_double, [ctypes.c_void_p]),
     ("cx_batch_last_solve_ms", ctypes.c_double, [ctypes.c_void_p]),
+    ("cx_equilibrate", ctypes.c_int, [ctypes.c_int, ctypes.c_int, _i32p, _i32p, _f64p, _i32p, _i32p, _f64p,
+                                      ctypes.c_int, ctypes.c_int, ctypes.c_int, _i32p,
+                                      ctypes.POINTER(CxSettings), _f64p, _f64p, _f64p]),
     ("cx_analyze_pattern", ctypes.c_int, [ctypes.c_int, _i32p, _i32p, _i32p, _i32p, _f64p]),
 ]
 

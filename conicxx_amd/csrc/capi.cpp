@@ -2,13 +2,16 @@
 // crosses this boundary: HIP or allocation failures become return codes or
 // a NumericalError status, as the reference's Solver never throws
 // (include/conicxx/solver.h:43-47, 86-89).
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <exception>
 #include <new>
+#include <vector>
 
 #include "../../include/conicxx_c.h"
 #include "batch.hpp"
+#include "equilibrate.hpp"
 #include "solver.hpp"
 #include "symbolic.hpp"
 
@@ -324,6 +327,31 @@ int cx_batch_backward(cx_batch* h, const double* const* dl_dx, const double* con
 double cx_batch_last_backward_ms(const cx_batch* h) { return h ? h->impl->last_backward_ms() : 0.0; }
 
 double cx_batch_last_solve_ms(const cx_batch* h) { return h ? h->impl->last_solve_ms() : 0.0; }
+
+int cx_equilibrate(int n, int m, const int* P_colptr, const int* P_rowind, const double* P_val,
+                   const int* A_colptr, const int* A_rowind, const double* A_val, int zero_dim,
+                   int nonneg_dim, int n_soc, const int* soc_dims, const cx_settings* settings, double* d,
+                   double* e, double* c) {
+  if (n < 0 || m < 0 || n_soc < 0 || !P_colptr || !A_colptr) return 0;
+  try {
+    cx_settings st;
+    cx_default_settings(&st);
+    if (settings) st = *settings;
+    std::vector<int> soc_off(n_soc), sd(soc_dims, soc_dims + n_soc);
+    int off = zero_dim + nonneg_dim;
+    for (int k = 0; k < n_soc; ++k) soc_off[k] = off, off += sd[k];
+    if (off != m) return 0;
+    std::vector<double> Pv(P_val, P_val + P_colptr[n]), Av(A_val, A_val + A_colptr[n]), ed, ee;
+    double ec = 1.0;
+    cx::ruiz_equilibrate(n, m, P_colptr, P_rowind, Pv, A_colptr, A_rowind, Av, soc_off, sd, st, ed, ee, ec);
+    if (d) std::copy(ed.begin(), ed.end(), d);
+    if (e) std::copy(ee.begin(), ee.end(), e);
+    if (c) *c = ec;
+    return 1;
+  } catch (const std::exception&) {
+    return 0;
+  }
+}
 
 int cx_analyze_pattern(int N, const int* colptr, const int* rowind, int* perm, int* colcount,
                        double* stats8) {

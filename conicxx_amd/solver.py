@@ -369,3 +369,23 @@ def analyze_pattern(K_lower):
         raise RuntimeError("symbolic analysis failed")
     keys = ["N", "nnz_K", "supernodes", "levels", "nnz_L", "flops", "max_front", "nnz_L_exact"]
     return perm, cc, dict(zip(keys, st.tolist()))
+
+
+def equilibrate(P, A, cone_spec, settings=None):
+    """Host Ruiz equilibration exactly as setup() computes it: (d, e, c)."""
+    P = sp.csc_matrix(P); P.sort_indices()
+    A = sp.csc_matrix(A); A.sort_indices()
+    n, m = P.shape[0], A.shape[0]
+    cs = cone_spec if isinstance(cone_spec, ConeSpec) else ConeSpec(*cone_spec)
+    keep = [_lib.i32(P.indptr), _lib.i32(P.indices), _lib.f64(P.data),
+            _lib.i32(A.indptr), _lib.i32(A.indices), _lib.f64(A.data),
+            _lib.i32(np.asarray(cs.soc_dims, dtype=np.int32).reshape(-1))]
+    st = (settings or Settings()).to_c()
+    d = np.zeros(n); e = np.zeros(m); c = ctypes.c_double(0.0)
+    ok = _lib.lib().cx_equilibrate(n, m, keep[0][1], keep[1][1], keep[2][1], keep[3][1], keep[4][1],
+                                   keep[5][1], cs.zero_dim, cs.nonneg_dim, len(cs.soc_dims), keep[6][1],
+                                   ctypes.byref(st), d.ctypes.data_as(_lib._f64p), e.ctypes.data_as(_lib._f64p),
+                                   ctypes.byref(c))
+    if not ok:
+        raise ValueError("invalid sizes")
+    return d, e, c.value

@@ -247,6 +247,16 @@ double cx_batch_last_backward_ms(const cx_batch* h);
 /* Device time of the last cx_batch_solve (ms, HIP events around the graph). */
 double cx_batch_last_solve_ms(const cx_batch* h);
 
+/* Host-only Ruiz equilibration of one problem as setup() computes it
+ * (Equilibration::compute, src/conicxx/kkt/equilibration.cpp:8-99), for
+ * testing: writes the column scale d (n), the row scale e (m) and the cost
+ * scale c for the given data and settings (NULL = defaults).  Returns 1 on
+ * success, 0 on invalid sizes. */
+int cx_equilibrate(int n, int m, const int* P_colptr, const int* P_rowind, const double* P_val,
+                   const int* A_colptr, const int* A_rowind, const double* A_val, int zero_dim,
+                   int nonneg_dim, int n_soc, const int* soc_dims, const cx_settings* settings, double* d,
+                   double* e, double* c);
+
 /* Host-only symbolic analysis of a symmetric pattern (lower CSC, diagonal
  * included), for testing: writes the final permutation (new->old, N ints) and
  * the column counts of L (N ints) when non-NULL, and stats8 as above.

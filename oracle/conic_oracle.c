@@ -10,7 +10,7 @@
 #include "conic_oracle.h"
 
 #include <float.h>
-#include <math.h>
+#include <tgmath.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -21,8 +21,8 @@ enum { LS_EIGEN = 0, LS_QDLDL = 1, LS_REG = 2 };
 
 #define OC_INF (1.0 / 0.0)
 
-static double dmax(double a, double b) { return a > b ? a : b; }
-static double dmin(double a, double b) { return a < b ? a : b; }
+static oc_real dmax(oc_real a, oc_real b) { return a > b ? a : b; }
+static oc_real dmin(oc_real a, oc_real b) { return a < b ? a : b; }
 static void* xcalloc(size_t n, size_t sz) { return calloc(n ? n : 1, sz); }
 
 /* ------------------------------------------------------------------ */
@@ -55,13 +55,13 @@ void oc_default_settings(oc_settings* s) {
 /* ------------------------------------------------------------------ */
 /* Small dense helpers                                                 */
 /* ------------------------------------------------------------------ */
-static double vdot(int n, const double* a, const double* b) {
-  double s = 0.0;
+static oc_real vdot(int n, const oc_real* a, const oc_real* b) {
+  oc_real s = 0.0;
   for (int i = 0; i < n; ++i) s += a[i] * b[i];
   return s;
 }
-static double vnorm(int n, const double* a) { return sqrt(vdot(n, a, a)); }
-static int vfinite(int n, const double* a) {
+static oc_real vnorm(int n, const oc_real* a) { return sqrt(vdot(n, a, a)); }
+static int vfinite(int n, const oc_real* a) {
   for (int i = 0; i < n; ++i)
     if (!isfinite(a[i])) return 0;
   return 1;
@@ -70,35 +70,35 @@ static int vfinite(int n, const double* a) {
 /* LU with partial pivoting of a dense d x d column-major matrix (the role
  * Eigen::PartialPivLU plays for W at include/conicxx/cones/second_order_cone.h:59,
  * src/conicxx/cones/second_order_cone.cpp:158,165-167). */
-static void lu_factor(int d, double* LU, int* piv) {
+static void lu_factor(int d, oc_real* LU, int* piv) {
   for (int k = 0; k < d; ++k) {
     int p = k;
-    double best = fabs(LU[k + k * d]);
+    oc_real best = fabs(LU[k + k * d]);
     for (int i = k + 1; i < d; ++i)
       if (fabs(LU[i + k * d]) > best) best = fabs(LU[i + k * d]), p = i;
     piv[k] = p;
     if (p != k)
       for (int j = 0; j < d; ++j) {
-        double t = LU[k + j * d];
+        oc_real t = LU[k + j * d];
         LU[k + j * d] = LU[p + j * d];
         LU[p + j * d] = t;
       }
-    double pv = LU[k + k * d];
+    oc_real pv = LU[k + k * d];
     if (pv != 0.0)
       for (int i = k + 1; i < d; ++i) LU[i + k * d] /= pv;
     for (int j = k + 1; j < d; ++j) {
-      double f = LU[k + j * d];
+      oc_real f = LU[k + j * d];
       if (f != 0.0)
         for (int i = k + 1; i < d; ++i) LU[i + j * d] -= LU[i + k * d] * f;
     }
   }
 }
-static void lu_solve(int d, const double* LU, const int* piv, const double* b, double* x) {
+static void lu_solve(int d, const oc_real* LU, const int* piv, const oc_real* b, oc_real* x) {
   for (int i = 0; i < d; ++i) x[i] = b[i];
   for (int k = 0; k < d; ++k) {
     int p = piv[k];
     if (p != k) {
-      double t = x[k];
+      oc_real t = x[k];
       x[k] = x[p];
       x[p] = t;
     }
@@ -116,17 +116,17 @@ static void lu_solve(int d, const double* LU, const int* piv, const double* b, d
 /* ------------------------------------------------------------------ */
 typedef struct {
   int type, off, dim;
-  double* W;   /* SOC: d*d */
-  double* Hs;  /* SOC: d*d ; NONNEG: d (diagonal) */
-  double* LU;  /* SOC: d*d */
+  oc_real* W;   /* SOC: d*d */
+  oc_real* Hs;  /* SOC: d*d ; NONNEG: d (diagonal) */
+  oc_real* LU;  /* SOC: d*d */
   int* piv;    /* SOC: d */
-  double* w;   /* NONNEG: d */
+  oc_real* w;   /* NONNEG: d */
 } oc_block;
 
 typedef struct {
   int nblocks, total, degree;
   oc_block* b;
-  double* identity;
+  oc_real* identity;
 } oc_cones;
 
 /* ConeSet ctor: src/conicxx/cones/cone_set.cpp:12-32 (fixed order Zero, Nonneg, SOC...). */
@@ -141,8 +141,8 @@ static void cones_init(oc_cones* C, int zero_dim, int nonneg_dim, int n_soc, con
   if (nonneg_dim > 0) {
     oc_block* B = &C->b[nb];
     B->type = CT_NONNEG, B->off = off, B->dim = nonneg_dim;
-    B->w = (double*)xcalloc((size_t)nonneg_dim, sizeof(double));
-    B->Hs = (double*)xcalloc((size_t)nonneg_dim, sizeof(double));
+    B->w = (oc_real*)xcalloc((size_t)nonneg_dim, sizeof(oc_real));
+    B->Hs = (oc_real*)xcalloc((size_t)nonneg_dim, sizeof(oc_real));
     for (int i = 0; i < nonneg_dim; ++i) B->w[i] = 1.0, B->Hs[i] = 1.0;
     C->degree += nonneg_dim;
     off += nonneg_dim, ++nb;
@@ -151,9 +151,9 @@ static void cones_init(oc_cones* C, int zero_dim, int nonneg_dim, int n_soc, con
     oc_block* B = &C->b[nb];
     int d = soc_dims[k];
     B->type = CT_SOC, B->off = off, B->dim = d;
-    B->W = (double*)xcalloc((size_t)d * d, sizeof(double));
-    B->Hs = (double*)xcalloc((size_t)d * d, sizeof(double));
-    B->LU = (double*)xcalloc((size_t)d * d, sizeof(double));
+    B->W = (oc_real*)xcalloc((size_t)d * d, sizeof(oc_real));
+    B->Hs = (oc_real*)xcalloc((size_t)d * d, sizeof(oc_real));
+    B->LU = (oc_real*)xcalloc((size_t)d * d, sizeof(oc_real));
     B->piv = (int*)xcalloc((size_t)d, sizeof(int));
     for (int i = 0; i < d; ++i) B->W[i + i * d] = 1.0, B->Hs[i + i * d] = 1.0, B->LU[i + i * d] = 1.0, B->piv[i] = i;
     C->degree += 1;
@@ -161,7 +161,7 @@ static void cones_init(oc_cones* C, int zero_dim, int nonneg_dim, int n_soc, con
   }
   C->nblocks = nb;
   C->total = off;
-  C->identity = (double*)xcalloc((size_t)off, sizeof(double));
+  C->identity = (oc_real*)xcalloc((size_t)off, sizeof(oc_real));
   for (int i = 0; i < nb; ++i) {
     oc_block* B = &C->b[i];
     if (B->type == CT_NONNEG)
@@ -180,36 +180,36 @@ static void cones_free(oc_cones* C) {
 }
 
 /* SecondOrderCone::product: second_order_cone.cpp:43-48 */
-void oc_soc_product(int d, const double* u, const double* v, double* out) {
-  double o0 = vdot(d, u, v);
+void oc_soc_product(int d, const oc_real* u, const oc_real* v, oc_real* out) {
+  oc_real o0 = vdot(d, u, v);
   for (int i = 1; i < d; ++i) out[i] = u[0] * v[i] + v[0] * u[i];
   out[0] = o0;
 }
 
 /* SecondOrderCone::inverseProduct: second_order_cone.cpp:50-68 (kTiny = 1e-12 floors) */
-void oc_soc_inverse_product(int d, const double* u, const double* w, double* out) {
-  const double kTiny = 1e-12;
-  double u0 = u[0];
-  double rho = u0 * u0 - vdot(d - 1, u + 1, u + 1);
+void oc_soc_inverse_product(int d, const oc_real* u, const oc_real* w, oc_real* out) {
+  const oc_real kTiny = 1e-12;
+  oc_real u0 = u[0];
+  oc_real rho = u0 * u0 - vdot(d - 1, u + 1, u + 1);
   if (fabs(rho) < kTiny) rho = (rho >= 0 ? kTiny : -kTiny);
   if (fabs(u0) < kTiny) u0 = kTiny;
-  double nu = vdot(d - 1, u + 1, w + 1);
-  double w0 = w[0];
+  oc_real nu = vdot(d - 1, u + 1, w + 1);
+  oc_real w0 = w[0];
   out[0] = (u0 * w0 - nu) / rho;
   for (int i = 1; i < d; ++i) out[i] = ((nu / u0 - w0) * u[i] + (rho / u0) * w[i]) / rho;
 }
 
 /* SecondOrderCone::updateScaling: second_order_cone.cpp:70-159 */
-static void soc_scaling_into(int d, const double* s, const double* z, double* W, double* Hs) {
-  const double kTiny = 1e-12, kTinyRel = 1e-12, kAbsFloor = DBL_MIN * 1e4;
-  const double kBetaMax = 1e8, kWMax = 1e8;
-  double s0 = s[0], z0 = z[0];
-  double s1n = vdot(d - 1, s + 1, s + 1), z1n = vdot(d - 1, z + 1, z + 1);
-  double a = sqrt(dmax(dmax(s0 * s0 - s1n, kTinyRel * s0 * s0), kAbsFloor));
-  double b = sqrt(dmax(dmax(z0 * z0 - z1n, kTinyRel * z0 * z0), kAbsFloor));
-  double beta = dmin(dmax(sqrt(a / b), 1.0 / kBetaMax), kBetaMax);
-  double c = sqrt(dmax((vdot(d, s, z) / (a * b) + 1.0) / 2.0, kTiny));
-  double* v = (double*)xcalloc((size_t)d, sizeof(double));
+static void soc_scaling_into(int d, const oc_real* s, const oc_real* z, oc_real* W, oc_real* Hs) {
+  const oc_real kTiny = 1e-12, kTinyRel = 1e-12, kAbsFloor = DBL_MIN * 1e4;
+  const oc_real kBetaMax = 1e8, kWMax = 1e8;
+  oc_real s0 = s[0], z0 = z[0];
+  oc_real s1n = vdot(d - 1, s + 1, s + 1), z1n = vdot(d - 1, z + 1, z + 1);
+  oc_real a = sqrt(dmax(dmax(s0 * s0 - s1n, kTinyRel * s0 * s0), kAbsFloor));
+  oc_real b = sqrt(dmax(dmax(z0 * z0 - z1n, kTinyRel * z0 * z0), kAbsFloor));
+  oc_real beta = dmin(dmax(sqrt(a / b), 1.0 / kBetaMax), kBetaMax);
+  oc_real c = sqrt(dmax((vdot(d, s, z) / (a * b) + 1.0) / 2.0, kTiny));
+  oc_real* v = (oc_real*)xcalloc((size_t)d, sizeof(oc_real));
   v[0] = z0 / b;
   for (int i = 1; i < d; ++i) v[i] = -z[i] / b;
   v[0] += s0 / a;
@@ -217,7 +217,7 @@ static void soc_scaling_into(int d, const double* s, const double* z, double* W,
   for (int i = 0; i < d; ++i) v[i] /= (2.0 * c);
   v[0] += 1.0;
   v[0] = dmax(v[0], kTiny);
-  double sc = sqrt(2.0 * v[0]);
+  oc_real sc = sqrt(2.0 * v[0]);
   for (int i = 0; i < d; ++i) v[i] /= sc;
   for (int j = 0; j < d; ++j)
     for (int i = 0; i < d; ++i) W[i + j * d] = 2.0 * (v[i] * v[j]);
@@ -226,12 +226,12 @@ static void soc_scaling_into(int d, const double* s, const double* z, double* W,
   for (int k = 0; k < d * d; ++k) W[k] *= beta;
   for (int j = 0; j < d; ++j)
     for (int i = 0; i < d; ++i) {
-      double acc = 0.0;
+      oc_real acc = 0.0;
       for (int k = 0; k < d; ++k) acc += W[i + k * d] * W[k + j * d];
       Hs[i + j * d] = acc;
     }
   int bad = !vfinite(d * d, W) || !vfinite(d * d, Hs);
-  double wmax = 0.0;
+  oc_real wmax = 0.0;
   for (int k = 0; k < d * d; ++k) wmax = dmax(wmax, fabs(W[k]));
   if (bad || wmax > kWMax) {
     for (int k = 0; k < d * d; ++k) W[k] = 0.0, Hs[k] = 0.0;
@@ -240,31 +240,31 @@ static void soc_scaling_into(int d, const double* s, const double* z, double* W,
   free(v);
 }
 
-void oc_soc_scaling(int d, const double* s, const double* z, double* W, double* Hs) {
+void oc_soc_scaling(int d, const oc_real* s, const oc_real* z, oc_real* W, oc_real* Hs) {
   soc_scaling_into(d, s, z, W, Hs);
 }
 
 /* SecondOrderCone::maxStep: second_order_cone.cpp:174-215 */
-double oc_soc_max_step(int d, const double* x, const double* dx, double alpha_max) {
-  const double kEps = DBL_MIN * 1e4;
-  double p0 = x[0], d0 = dx[0];
-  double a = d0 * d0 - vdot(d - 1, dx + 1, dx + 1);
-  double b = 2.0 * (p0 * d0 - vdot(d - 1, x + 1, dx + 1));
-  double c = p0 * p0 - vdot(d - 1, x + 1, x + 1);
-  double alpha = alpha_max;
+oc_real oc_soc_max_step(int d, const oc_real* x, const oc_real* dx, oc_real alpha_max) {
+  const oc_real kEps = DBL_MIN * 1e4;
+  oc_real p0 = x[0], d0 = dx[0];
+  oc_real a = d0 * d0 - vdot(d - 1, dx + 1, dx + 1);
+  oc_real b = 2.0 * (p0 * d0 - vdot(d - 1, x + 1, dx + 1));
+  oc_real c = p0 * p0 - vdot(d - 1, x + 1, x + 1);
+  oc_real alpha = alpha_max;
 #define FEAS(r) ((r) > 0 && (p0 + (r) * d0) >= 0)
   if (fabs(a) < kEps) {
     if (fabs(b) > kEps) {
-      double r = -c / b;
+      oc_real r = -c / b;
       if (FEAS(r)) alpha = dmin(alpha, r);
     }
     return alpha;
   }
-  double disc = b * b - 4.0 * a * c;
+  oc_real disc = b * b - 4.0 * a * c;
   if (disc < 0) return alpha;
-  double sq = sqrt(disc);
-  double qq = (b >= 0) ? -0.5 * (b + sq) : -0.5 * (b - sq);
-  double r1, r2;
+  oc_real sq = sqrt(disc);
+  oc_real qq = (b >= 0) ? -0.5 * (b + sq) : -0.5 * (b - sq);
+  oc_real r1, r2;
   if (fabs(qq) > kEps) {
     r1 = qq / a;
     r2 = c / qq;
@@ -278,11 +278,11 @@ double oc_soc_max_step(int d, const double* x, const double* dx, double alpha_ma
 }
 
 /* ConeSet::updateScaling: cone_set.cpp:48-54; Nonneg: nonnegative_cone.cpp:22-26 */
-static void cones_update_scaling(oc_cones* C, const double* s, const double* z) {
+static void cones_update_scaling(oc_cones* C, const oc_real* s, const oc_real* z) {
   for (int i = 0; i < C->nblocks; ++i) {
     oc_block* B = &C->b[i];
-    const double* sb = s + B->off;
-    const double* zb = z + B->off;
+    const oc_real* sb = s + B->off;
+    const oc_real* zb = z + B->off;
     if (B->type == CT_NONNEG) {
       for (int j = 0; j < B->dim; ++j) {
         B->w[j] = sqrt(sb[j] / zb[j]);
@@ -291,14 +291,14 @@ static void cones_update_scaling(oc_cones* C, const double* s, const double* z) 
     } else if (B->type == CT_SOC) {
       int d = B->dim;
       soc_scaling_into(d, sb, zb, B->W, B->Hs);
-      memcpy(B->LU, B->W, sizeof(double) * (size_t)d * d);
+      memcpy(B->LU, B->W, sizeof(oc_real) * (size_t)d * d);
       lu_factor(d, B->LU, B->piv);
     }
   }
 }
 
 /* ConeSet::product / inverseProduct: cone_set.cpp:34-46 (Zero -> 0, Nonneg elementwise) */
-static void cones_product(const oc_cones* C, const double* u, const double* v, double* out) {
+static void cones_product(const oc_cones* C, const oc_real* u, const oc_real* v, oc_real* out) {
   for (int i = 0; i < C->nblocks; ++i) {
     const oc_block* B = &C->b[i];
     int o = B->off, d = B->dim;
@@ -310,7 +310,7 @@ static void cones_product(const oc_cones* C, const double* u, const double* v, d
       oc_soc_product(d, u + o, v + o, out + o);
   }
 }
-static void cones_inverse_product(const oc_cones* C, const double* u, const double* w, double* out) {
+static void cones_inverse_product(const oc_cones* C, const oc_real* u, const oc_real* w, oc_real* out) {
   for (int i = 0; i < C->nblocks; ++i) {
     const oc_block* B = &C->b[i];
     int o = B->off, d = B->dim;
@@ -324,7 +324,7 @@ static void cones_inverse_product(const oc_cones* C, const double* u, const doub
 }
 
 /* ConeSet::applyW / applyWInv / mulHs: cone_set.cpp:56-75 */
-static void cones_apply_w(const oc_cones* C, const double* x, double* out) {
+static void cones_apply_w(const oc_cones* C, const oc_real* x, oc_real* out) {
   for (int i = 0; i < C->nblocks; ++i) {
     const oc_block* B = &C->b[i];
     int o = B->off, d = B->dim;
@@ -334,13 +334,13 @@ static void cones_apply_w(const oc_cones* C, const double* x, double* out) {
       for (int j = 0; j < d; ++j) out[o + j] = B->w[j] * x[o + j];
     else
       for (int r = 0; r < d; ++r) {
-        double acc = 0.0;
+        oc_real acc = 0.0;
         for (int k = 0; k < d; ++k) acc += B->W[r + k * d] * x[o + k];
         out[o + r] = acc;
       }
   }
 }
-static void cones_apply_winv(const oc_cones* C, const double* x, double* out) {
+static void cones_apply_winv(const oc_cones* C, const oc_real* x, oc_real* out) {
   for (int i = 0; i < C->nblocks; ++i) {
     const oc_block* B = &C->b[i];
     int o = B->off, d = B->dim;
@@ -352,7 +352,7 @@ static void cones_apply_winv(const oc_cones* C, const double* x, double* out) {
       lu_solve(d, B->LU, B->piv, x + o, out + o);
   }
 }
-static void cones_mul_hs(const oc_cones* C, const double* x, double* out) {
+static void cones_mul_hs(const oc_cones* C, const oc_real* x, oc_real* out) {
   for (int i = 0; i < C->nblocks; ++i) {
     const oc_block* B = &C->b[i];
     int o = B->off, d = B->dim;
@@ -362,7 +362,7 @@ static void cones_mul_hs(const oc_cones* C, const double* x, double* out) {
       for (int j = 0; j < d; ++j) out[o + j] = B->Hs[j] * x[o + j];
     else
       for (int r = 0; r < d; ++r) {
-        double acc = 0.0;
+        oc_real acc = 0.0;
         for (int k = 0; k < d; ++k) acc += B->Hs[r + k * d] * x[o + k];
         out[o + r] = acc;
       }
@@ -371,12 +371,12 @@ static void cones_mul_hs(const oc_cones* C, const double* x, double* out) {
 
 /* ConeSet::margins: cone_set.cpp:77-90; per-block margins: nonnegative_cone.h:28,
  * second_order_cone.cpp:169-172, zero_cone.h (+inf). */
-static void cones_margins(const oc_cones* C, const double* x, double* min_margin, double* pos_sum) {
-  double mn = OC_INF, ps = 0.0;
+static void cones_margins(const oc_cones* C, const oc_real* x, oc_real* min_margin, oc_real* pos_sum) {
+  oc_real mn = OC_INF, ps = 0.0;
   for (int i = 0; i < C->nblocks; ++i) {
     const oc_block* B = &C->b[i];
-    const double* xb = x + B->off;
-    double mg;
+    const oc_real* xb = x + B->off;
+    oc_real mg;
     if (B->type == CT_ZERO) continue;
     if (B->type == CT_NONNEG) {
       mg = xb[0];
@@ -395,7 +395,7 @@ static void cones_margins(const oc_cones* C, const double* x, double* min_margin
 }
 
 /* ConeSet::scaledUnitShift: cone_set.cpp:92-98 */
-static void cones_unit_shift(const oc_cones* C, double* x, double alpha) {
+static void cones_unit_shift(const oc_cones* C, oc_real* x, oc_real alpha) {
   for (int i = 0; i < C->nblocks; ++i) {
     const oc_block* B = &C->b[i];
     if (B->type == CT_NONNEG)
@@ -406,15 +406,15 @@ static void cones_unit_shift(const oc_cones* C, double* x, double alpha) {
 }
 
 /* ConeSet::zeroPrimalZeroConeBlocks: cone_set.cpp:100-106 */
-static void cones_zero_zero_blocks(const oc_cones* C, double* s) {
+static void cones_zero_zero_blocks(const oc_cones* C, oc_real* s) {
   for (int i = 0; i < C->nblocks; ++i)
     if (C->b[i].type == CT_ZERO)
       for (int j = 0; j < C->b[i].dim; ++j) s[C->b[i].off + j] = 0.0;
 }
 
 /* ConeSet::maxStep: cone_set.cpp:108-115; Nonneg: nonnegative_cone.cpp:40-49 */
-static double cones_max_step(const oc_cones* C, const double* x, const double* dx, double amax) {
-  double alpha = amax;
+static oc_real cones_max_step(const oc_cones* C, const oc_real* x, const oc_real* dx, oc_real amax) {
+  oc_real alpha = amax;
   for (int i = 0; i < C->nblocks; ++i) {
     const oc_block* B = &C->b[i];
     int o = B->off;
@@ -435,19 +435,19 @@ typedef struct {
   int rows, cols, nnz;
   int* p;
   int* i;
-  double* x;
+  oc_real* x;
 } oc_csc;
 
-static void csc_copy(oc_csc* M, int rows, int cols, const int* p, const int* i, const double* x) {
+static void csc_copy(oc_csc* M, int rows, int cols, const int* p, const int* i, const oc_real* x) {
   M->rows = rows, M->cols = cols;
   M->nnz = p ? p[cols] : 0;
   M->p = (int*)xcalloc((size_t)cols + 1, sizeof(int));
   M->i = (int*)xcalloc((size_t)M->nnz, sizeof(int));
-  M->x = (double*)xcalloc((size_t)M->nnz, sizeof(double));
+  M->x = (oc_real*)xcalloc((size_t)M->nnz, sizeof(oc_real));
   if (p) {
     memcpy(M->p, p, sizeof(int) * ((size_t)cols + 1));
     memcpy(M->i, i, sizeof(int) * (size_t)M->nnz);
-    memcpy(M->x, x, sizeof(double) * (size_t)M->nnz);
+    memcpy(M->x, x, sizeof(oc_real) * (size_t)M->nnz);
   }
 }
 static void csc_free(oc_csc* M) {
@@ -455,7 +455,7 @@ static void csc_free(oc_csc* M) {
   memset(M, 0, sizeof(*M));
 }
 /* Eigen SparseMatrix::coeff(r, c) on a sorted column (binary search; 0 when absent). */
-static double csc_coeff(const oc_csc* M, int r, int c) {
+static oc_real csc_coeff(const oc_csc* M, int r, int c) {
   int lo = M->p[c], hi = M->p[c + 1];
   while (lo < hi) {
     int mid = (lo + hi) / 2;
@@ -469,7 +469,7 @@ static double csc_coeff(const oc_csc* M, int r, int c) {
 }
 
 /* P (symmetric, upper stored) * v: solver_impl.cpp:102 (selfadjointView<Upper>) */
-static void pmul(const oc_csc* P, const double* v, double* out) {
+static void pmul(const oc_csc* P, const oc_real* v, oc_real* out) {
   for (int i = 0; i < P->rows; ++i) out[i] = 0.0;
   for (int c = 0; c < P->cols; ++c)
     for (int k = P->p[c]; k < P->p[c + 1]; ++k) {
@@ -485,9 +485,9 @@ static void pmul(const oc_csc* P, const double* v, double* out) {
 /* ------------------------------------------------------------------ */
 typedef struct {
   int n, m;
-  double* d;
-  double* e;
-  double c;
+  oc_real* d;
+  oc_real* e;
+  oc_real c;
 } oc_equil;
 
 static void equil_compute(oc_equil* E, const oc_csc* P, const oc_csc* A, const oc_cones* C,
@@ -495,42 +495,42 @@ static void equil_compute(oc_equil* E, const oc_csc* P, const oc_csc* A, const o
   int n = P->rows, m = A->rows;
   E->n = n, E->m = m;
   free(E->d), free(E->e);
-  E->d = (double*)xcalloc((size_t)n, sizeof(double));
-  E->e = (double*)xcalloc((size_t)m, sizeof(double));
+  E->d = (oc_real*)xcalloc((size_t)n, sizeof(oc_real));
+  E->e = (oc_real*)xcalloc((size_t)m, sizeof(oc_real));
   for (int i = 0; i < n; ++i) E->d[i] = 1.0;
   for (int i = 0; i < m; ++i) E->e[i] = 1.0;
   E->c = 1.0;
   if (!st->equilibrate || n == 0) return;
-  double* Pw = (double*)xcalloc((size_t)P->nnz, sizeof(double));
-  double* Aw = (double*)xcalloc((size_t)A->nnz, sizeof(double));
-  memcpy(Pw, P->x, sizeof(double) * (size_t)P->nnz);
-  memcpy(Aw, A->x, sizeof(double) * (size_t)A->nnz);
-  double* nx = (double*)xcalloc((size_t)n, sizeof(double));
-  double* nz = (double*)xcalloc((size_t)m, sizeof(double));
-  double* dx = (double*)xcalloc((size_t)n, sizeof(double));
-  double* dz = (double*)xcalloc((size_t)m, sizeof(double));
-  const double minS = st->equilibrate_min_scale, maxS = st->equilibrate_max_scale;
+  oc_real* Pw = (oc_real*)xcalloc((size_t)P->nnz, sizeof(oc_real));
+  oc_real* Aw = (oc_real*)xcalloc((size_t)A->nnz, sizeof(oc_real));
+  memcpy(Pw, P->x, sizeof(oc_real) * (size_t)P->nnz);
+  memcpy(Aw, A->x, sizeof(oc_real) * (size_t)A->nnz);
+  oc_real* nx = (oc_real*)xcalloc((size_t)n, sizeof(oc_real));
+  oc_real* nz = (oc_real*)xcalloc((size_t)m, sizeof(oc_real));
+  oc_real* dx = (oc_real*)xcalloc((size_t)n, sizeof(oc_real));
+  oc_real* dz = (oc_real*)xcalloc((size_t)m, sizeof(oc_real));
+  const oc_real minS = st->equilibrate_min_scale, maxS = st->equilibrate_max_scale;
   for (int it = 0; it < st->equilibrate_max_iter; ++it) {
     for (int i = 0; i < n; ++i) nx[i] = 0.0;
     for (int i = 0; i < m; ++i) nz[i] = 0.0;
     for (int c = 0; c < P->cols; ++c)
       for (int k = P->p[c]; k < P->p[c + 1]; ++k) {
         int r = P->i[k];
-        double av = fabs(Pw[k]);
+        oc_real av = fabs(Pw[k]);
         nx[r] = dmax(nx[r], av);
         nx[c] = dmax(nx[c], av);
       }
     for (int c = 0; c < A->cols; ++c)
       for (int k = A->p[c]; k < A->p[c + 1]; ++k) {
         int r = A->i[k];
-        double av = fabs(Aw[k]);
+        oc_real av = fabs(Aw[k]);
         nx[c] = dmax(nx[c], av);
         nz[r] = dmax(nz[r], av);
       }
     for (int bi = 0; bi < C->nblocks; ++bi) {
       const oc_block* B = &C->b[bi];
       if (B->type != CT_SOC) continue;
-      double mx = nz[B->off];
+      oc_real mx = nz[B->off];
       for (int j = 1; j < B->dim; ++j) mx = dmax(mx, nz[B->off + j]);
       for (int j = 0; j < B->dim; ++j) nz[B->off + j] = mx;
     }
@@ -545,11 +545,11 @@ static void equil_compute(oc_equil* E, const oc_csc* P, const oc_csc* A, const o
   }
   for (int i = 0; i < n; ++i) E->d[i] = dmin(dmax(E->d[i], minS), maxS);
   for (int i = 0; i < m; ++i) E->e[i] = dmin(dmax(E->e[i], minS), maxS);
-  double sumAbs = 0.0;
+  oc_real sumAbs = 0.0;
   int count = P->nnz;
   for (int k = 0; k < P->nnz; ++k) sumAbs += fabs(Pw[k]);
   if (count > 0) {
-    double meanAbs = sumAbs / (double)count;
+    oc_real meanAbs = sumAbs / (oc_real)count;
     if (meanAbs > 0) E->c = 1.0 / dmax(meanAbs, minS);
   }
   E->c = dmin(dmax(E->c, minS), maxS);
@@ -563,18 +563,18 @@ static void equil_scale_a(const oc_equil* E, oc_csc* A) {
   for (int c = 0; c < A->cols; ++c)
     for (int k = A->p[c]; k < A->p[c + 1]; ++k) A->x[k] *= E->e[A->i[k]] * E->d[c];
 }
-static void equil_scale_q(const oc_equil* E, double* q) {
+static void equil_scale_q(const oc_equil* E, oc_real* q) {
   for (int i = 0; i < E->n; ++i) q[i] *= E->c * E->d[i];
 }
-static void equil_scale_b(const oc_equil* E, double* b) {
+static void equil_scale_b(const oc_equil* E, oc_real* b) {
   for (int i = 0; i < E->m; ++i) b[i] *= E->e[i];
 }
-static void equil_unscale_solution(const oc_equil* E, double* x, double* s, double* z) {
+static void equil_unscale_solution(const oc_equil* E, oc_real* x, oc_real* s, oc_real* z) {
   for (int i = 0; i < E->n; ++i) x[i] *= E->d[i];
   for (int i = 0; i < E->m; ++i) z[i] = z[i] * E->e[i] / E->c;
   for (int i = 0; i < E->m; ++i) s[i] /= E->e[i];
 }
-static void equil_scale_solution(const oc_equil* E, double* x, double* s, double* z) {
+static void equil_scale_solution(const oc_equil* E, oc_real* x, oc_real* s, oc_real* z) {
   for (int i = 0; i < E->n; ++i) x[i] /= E->d[i];
   for (int i = 0; i < E->m; ++i) z[i] = z[i] * E->c / E->e[i];
   for (int i = 0; i < E->m; ++i) s[i] *= E->e[i];
@@ -646,21 +646,21 @@ typedef struct {
   int* Up;    /* permuted upper triangle pattern */
   int* Ui;
   int* umap;  /* K lower value index -> U value index */
-  double* Ux;
+  oc_real* Ux;
   int* etree;
   int* Lnz;
   int* Lp;
   int* Li;
-  double* Lx;
-  double* D;
-  double* Dinv;
-  double* sign; /* expected pivot sign per permuted position */
+  oc_real* Lx;
+  oc_real* D;
+  oc_real* Dinv;
+  oc_real* sign; /* expected pivot sign per permuted position */
   int* yIdx;
   int* stack;
   int* next;
   char* mark;
-  double* yv;
-  double flops;
+  oc_real* yv;
+  oc_real flops;
   int info_ok;
   int num_reg;
 } oc_ldl;
@@ -685,7 +685,7 @@ static int ldl_analyze(oc_ldl* F, int N, const int* Kp, const int* Ki, int nx) {
   F->Up = (int*)xcalloc((size_t)N + 1, sizeof(int));
   F->Ui = (int*)xcalloc((size_t)nnz, sizeof(int));
   F->umap = (int*)xcalloc((size_t)nnz, sizeof(int));
-  F->Ux = (double*)xcalloc((size_t)nnz, sizeof(double));
+  F->Ux = (oc_real*)xcalloc((size_t)nnz, sizeof(oc_real));
   int* cnt = (int*)xcalloc((size_t)N + 1, sizeof(int));
   for (int c = 0; c < N; ++c)
     for (int k = Kp[c]; k < Kp[c + 1]; ++k) {
@@ -724,30 +724,30 @@ static int ldl_analyze(oc_ldl* F, int N, const int* Kp, const int* Ki, int nx) {
   }
   free(flag);
   F->Lp = (int*)xcalloc((size_t)N + 1, sizeof(int));
-  double flops = 0.0;
+  oc_real flops = 0.0;
   for (int i = 0; i < N; ++i) {
     F->Lp[i + 1] = F->Lp[i] + F->Lnz[i];
-    double cj = (double)F->Lnz[i] + 1.0;
+    oc_real cj = (oc_real)F->Lnz[i] + 1.0;
     flops += cj * (cj - 1.0);
   }
   F->flops = flops;
   F->Li = (int*)xcalloc((size_t)F->Lp[N], sizeof(int));
-  F->Lx = (double*)xcalloc((size_t)F->Lp[N], sizeof(double));
-  F->D = (double*)xcalloc((size_t)N, sizeof(double));
-  F->Dinv = (double*)xcalloc((size_t)N, sizeof(double));
-  F->sign = (double*)xcalloc((size_t)N, sizeof(double));
+  F->Lx = (oc_real*)xcalloc((size_t)F->Lp[N], sizeof(oc_real));
+  F->D = (oc_real*)xcalloc((size_t)N, sizeof(oc_real));
+  F->Dinv = (oc_real*)xcalloc((size_t)N, sizeof(oc_real));
+  F->sign = (oc_real*)xcalloc((size_t)N, sizeof(oc_real));
   for (int k = 0; k < N; ++k) F->sign[k] = (F->perm[k] < nx) ? 1.0 : -1.0;
   F->yIdx = (int*)xcalloc((size_t)N, sizeof(int));
   F->stack = (int*)xcalloc((size_t)N, sizeof(int));
   F->next = (int*)xcalloc((size_t)N, sizeof(int));
   F->mark = (char*)xcalloc((size_t)N, 1);
-  F->yv = (double*)xcalloc((size_t)N, sizeof(double));
+  F->yv = (oc_real*)xcalloc((size_t)N, sizeof(oc_real));
   F->info_ok = 1;
   return 1;
 }
 
 /* regularizePivot: regularized_ldlt.cpp:132-139 */
-static int reg_pivot(double* Dk, double sgn, double eps, double delta) {
+static int reg_pivot(oc_real* Dk, oc_real sgn, oc_real eps, oc_real delta) {
   if (*Dk * sgn >= eps) return 0;
   *Dk += sgn * delta;
   if (*Dk * sgn < eps) *Dk = sgn * delta;
@@ -755,7 +755,7 @@ static int reg_pivot(double* Dk, double sgn, double eps, double delta) {
 }
 
 /* Numeric factorization of the K lower values (already bumped by the caller). */
-static void ldl_factor(oc_ldl* F, const double* Kx, int regularize, double eps, double delta) {
+static void ldl_factor(oc_ldl* F, const oc_real* Kx, int regularize, oc_real eps, oc_real delta) {
   int N = F->N;
   int nnz = F->Up[N];
   (void)nnz;
@@ -767,7 +767,7 @@ static void ldl_factor(oc_ldl* F, const double* Kx, int regularize, double eps, 
   int nreg = 0;
   F->info_ok = 1;
   for (int k = 0; k < N; ++k) {
-    double Dk = 0.0;
+    oc_real Dk = 0.0;
     int ny = 0;
     /* nonzero pattern of row k of L: reach of column k's entries in the etree;
        each path is pushed so that descendants are processed before ancestors */
@@ -788,10 +788,10 @@ static void ldl_factor(oc_ldl* F, const double* Kx, int regularize, double eps, 
     }
     for (int t = ny - 1; t >= 0; --t) {
       int c = F->yIdx[t];
-      double yc = F->yv[c];
+      oc_real yc = F->yv[c];
       int end = F->next[c];
       for (int j = F->Lp[c]; j < end; ++j) F->yv[F->Li[j]] -= F->Lx[j] * yc;
-      double lkc = yc * F->Dinv[c];
+      oc_real lkc = yc * F->Dinv[c];
       F->Li[end] = k;
       F->Lx[end] = lkc;
       Dk -= yc * lkc;
@@ -808,20 +808,20 @@ static void ldl_factor(oc_ldl* F, const double* Kx, int regularize, double eps, 
     F->Dinv[k] = 1.0 / Dk;
   }
   F->num_reg = nreg;
-  if (regularize && (double)nreg > 0.05 * (double)N) F->info_ok = 0;
+  if (regularize && (oc_real)nreg > 0.05 * (oc_real)N) F->info_ok = 0;
 }
 
 /* QDLDL_solve with the permutation (qdldl_ldlt.cpp:71-75). */
-static void ldl_solve(const oc_ldl* F, const double* rhs, double* x, double* work) {
+static void ldl_solve(const oc_ldl* F, const oc_real* rhs, oc_real* x, oc_real* work) {
   int N = F->N;
   for (int k = 0; k < N; ++k) work[k] = rhs[F->perm[k]];
   for (int j = 0; j < N; ++j) {
-    double xj = work[j];
+    oc_real xj = work[j];
     for (int p = F->Lp[j]; p < F->Lp[j + 1]; ++p) work[F->Li[p]] -= F->Lx[p] * xj;
   }
   for (int j = 0; j < N; ++j) work[j] *= F->Dinv[j];
   for (int j = N - 1; j >= 0; --j) {
-    double acc = work[j];
+    oc_real acc = work[j];
     for (int p = F->Lp[j]; p < F->Lp[j + 1]; ++p) acc -= F->Lx[p] * work[F->Li[p]];
     work[j] = acc;
   }
@@ -833,12 +833,12 @@ static void ldl_solve(const oc_ldl* F, const double* rhs, double* x, double* wor
 /* ------------------------------------------------------------------ */
 typedef struct {
   int n, m, N;
-  double sreg_p, sreg_a, dyn_eps, dyn_delta, refine_tol;
+  oc_real sreg_p, sreg_a, dyn_eps, dyn_delta, refine_tol;
   int refine_max, backend;
   int* Kp; /* K lower, CSC */
   int* Ki;
-  double* Kx;
-  double* Ktry;
+  oc_real* Kx;
+  oc_real* Ktry;
   int* slot_pdiag; /* n */
   int npoff;
   int* poff_slot; int* poff_r; int* poff_c;
@@ -850,8 +850,8 @@ typedef struct {
   int* pA; int* iA; int annz;
   oc_ldl F;
   int factorized;
-  double last_p, last_a;
-  double* w1; double* w2; double* w3;
+  oc_real last_p, last_a;
+  oc_real* w1; oc_real* w2; oc_real* w3;
 } oc_kkt;
 
 static void kkt_free(oc_kkt* K) {
@@ -882,7 +882,7 @@ static int same_pattern(const int* p, const int* i, int cols, const int* pref, c
   return 1;
 }
 
-static void kkt_write_bump(const oc_kkt* K, double* Kv, double ep, double ea) {
+static void kkt_write_bump(const oc_kkt* K, oc_real* Kv, oc_real ep, oc_real ea) {
   /* writeRegularizedDiagonal: kkt_system.cpp:152-168 */
   if (ep != 0.0)
     for (int i = 0; i < K->n; ++i) Kv[K->slot_pdiag[i]] += ep;
@@ -892,16 +892,16 @@ static void kkt_write_bump(const oc_kkt* K, double* Kv, double ep, double ea) {
 }
 
 /* tryFactorizeWithBump: kkt_system.cpp:222-240 */
-static int kkt_try_factor(oc_kkt* K, double ep, double ea) {
-  const double* Kv = K->Kx;
+static int kkt_try_factor(oc_kkt* K, oc_real ep, oc_real ea) {
+  const oc_real* Kv = K->Kx;
   if (ep != 0.0 || ea != 0.0) {
-    memcpy(K->Ktry, K->Kx, sizeof(double) * (size_t)K->Kp[K->N]);
+    memcpy(K->Ktry, K->Kx, sizeof(oc_real) * (size_t)K->Kp[K->N]);
     kkt_write_bump(K, K->Ktry, ep, ea);
     Kv = K->Ktry;
   }
   ldl_factor(&K->F, Kv, K->backend == LS_REG, K->dyn_eps, K->dyn_delta);
   if (!K->F.info_ok) return 0;
-  double mn = 1.0;
+  oc_real mn = 1.0;
   if (K->N > 0) {
     mn = fabs(K->F.D[0]);
     for (int i = 1; i < K->N; ++i) mn = dmin(mn, fabs(K->F.D[i]));
@@ -913,7 +913,7 @@ static int kkt_try_factor(oc_kkt* K, double ep, double ea) {
 
 /* factorizeWithRetry: kkt_system.cpp:242-264 */
 static int kkt_factor_retry(oc_kkt* K) {
-  double ep = 0.0, ea = 0.0;
+  oc_real ep = 0.0, ea = 0.0;
   for (int att = 0; att < 18; ++att) {
     if (kkt_try_factor(K, ep, ea)) {
       K->factorized = 1;
@@ -928,7 +928,7 @@ static int kkt_factor_retry(oc_kkt* K) {
 }
 
 /* y = K x, K symmetric from lower storage (selfadjointView<Lower>) */
-static void kkt_spmv(const oc_kkt* K, const double* x, double* y) {
+static void kkt_spmv(const oc_kkt* K, const oc_real* x, oc_real* y) {
   for (int i = 0; i < K->N; ++i) y[i] = 0.0;
   for (int c = 0; c < K->N; ++c)
     for (int k = K->Kp[c]; k < K->Kp[c + 1]; ++k) {
@@ -939,8 +939,8 @@ static void kkt_spmv(const oc_kkt* K, const double* x, double* y) {
 }
 
 /* escalateAndResolve: kkt_system.cpp:266-294 */
-static int kkt_escalate(oc_kkt* K, const double* rhs, double* x) {
-  double ep = K->last_p, ea = K->last_a;
+static int kkt_escalate(oc_kkt* K, const oc_real* rhs, oc_real* x) {
+  oc_real ep = K->last_p, ea = K->last_a;
   for (int att = 0; att < 10; ++att) {
     ep = dmax(ep * 1000.0, K->dyn_delta);
     ea = dmax(ea * 1000.0, K->dyn_delta);
@@ -955,16 +955,16 @@ static int kkt_escalate(oc_kkt* K, const double* rhs, double* x) {
 }
 
 /* KktSystem::solve: kkt_system.cpp:296-311 */
-static double kkt_solve(oc_kkt* K, const double* rhs, double* x) {
+static oc_real kkt_solve(oc_kkt* K, const oc_real* rhs, oc_real* x) {
   int N = K->N;
   ldl_solve(&K->F, rhs, x, K->w3);
   if (!vfinite(N, x)) kkt_escalate(K, rhs, x);
-  double rn = dmax(vnorm(N, rhs), 1e-30);
-  double* r = K->w1;
-  double* dx = K->w2;
+  oc_real rn = dmax(vnorm(N, rhs), 1e-30);
+  oc_real* r = K->w1;
+  oc_real* dx = K->w2;
   kkt_spmv(K, x, r);
   for (int i = 0; i < N; ++i) r[i] = rhs[i] - r[i];
-  double rel = vnorm(N, r) / rn;
+  oc_real rel = vnorm(N, r) / rn;
   for (int it = 0; it < K->refine_max && rel > K->refine_tol; ++it) {
     ldl_solve(&K->F, r, dx, K->w3);
     for (int i = 0; i < N; ++i) x[i] += dx[i];
@@ -994,14 +994,14 @@ static int kkt_update_scaling(oc_kkt* K, const oc_cones* C) {
   for (int t = 0; t < K->nhs; ++t) {
     const oc_block* B = &C->b[K->hs_blk[t]];
     int a = K->hs_a[t], b = K->hs_b[t];
-    double h;
+    oc_real h;
     if (B->type == CT_ZERO)
       h = 0.0;
     else if (B->type == CT_NONNEG)
       h = (a == b) ? B->Hs[a] : 0.0;
     else
       h = B->Hs[a + b * B->dim];
-    double v = -h;
+    oc_real v = -h;
     if (a == b) v -= K->sreg_a;
     K->Kx[K->hs_slot[t]] = v;
   }
@@ -1085,16 +1085,16 @@ static int kkt_setup(oc_kkt* K, const oc_csc* P, const oc_csc* A, const oc_cones
   }
   for (int c = 0; c < N; ++c) K->Kp[c + 1] += K->Kp[c];
   free(keys);
-  K->Kx = (double*)xcalloc((size_t)nnz, sizeof(double));
-  K->Ktry = (double*)xcalloc((size_t)nnz, sizeof(double));
+  K->Kx = (oc_real*)xcalloc((size_t)nnz, sizeof(oc_real));
+  K->Ktry = (oc_real*)xcalloc((size_t)nnz, sizeof(oc_real));
   for (int i = 0; i < n; ++i) K->slot_pdiag[i] = slot_off[K->slot_pdiag[i]];
   for (int t = 0; t < K->npoff; ++t) K->poff_slot[t] = slot_off[K->poff_slot[t]];
   for (int t = 0; t < K->na; ++t) K->a_slot[t] = slot_off[K->a_slot[t]];
   for (int t = 0; t < K->nhs; ++t) K->hs_slot[t] = slot_off[K->hs_slot[t]];
   free(slot_off);
-  K->w1 = (double*)xcalloc((size_t)N, sizeof(double));
-  K->w2 = (double*)xcalloc((size_t)N, sizeof(double));
-  K->w3 = (double*)xcalloc((size_t)N, sizeof(double));
+  K->w1 = (oc_real*)xcalloc((size_t)N, sizeof(oc_real));
+  K->w2 = (oc_real*)xcalloc((size_t)N, sizeof(oc_real));
+  K->w3 = (oc_real*)xcalloc((size_t)N, sizeof(oc_real));
   if (!ldl_analyze(&K->F, N, K->Kp, K->Ki, n)) return 0;
   K->pnnz = P->nnz;
   K->pP = (int*)xcalloc((size_t)n + 1, sizeof(int));
@@ -1117,28 +1117,28 @@ struct oc_solver {
   oc_settings st;
   int n, m, setup_done;
   oc_csc P, A;
-  double* q;
-  double* b;
+  oc_real* q;
+  oc_real* b;
   oc_cones C;
   oc_equil E;
   oc_kkt K;
   int zero_dim, nonneg_dim, n_soc;
   int* soc_dims;
-  double *x, *s, *z, tau, kappa;
-  double *Px, *rx, *rz;
-  double rtau, dqx, dbz, dsz, dxPx;
-  double *x1, *z1, *Px1;
-  double *dxa, *dza, *dsa, dtaua, dkappaa;
-  double *dx, *dz, *ds, dtau, dkappa;
+  oc_real *x, *s, *z, tau, kappa;
+  oc_real *Px, *rx, *rz;
+  oc_real rtau, dqx, dbz, dsz, dxPx;
+  oc_real *x1, *z1, *Px1;
+  oc_real *dxa, *dza, *dsa, dtaua, dkappaa;
+  oc_real *dx, *dz, *ds, dtau, dkappa;
   int have_warm;
-  double *wx, *ws, *wz;
+  oc_real *wx, *ws, *wz;
   /* solution */
   int status, iterations;
-  double *xo, *so, *zo, objective, pres, dres, gap, mu_out;
-  double* trace;
+  oc_real *xo, *so, *zo, objective, pres, dres, gap, mu_out;
+  oc_real* trace;
   int ntrace;
   /* scratch */
-  double *rhs, *sol, *t1, *t2, *t3, *t4, *t5;
+  oc_real *rhs, *sol, *t1, *t2, *t3, *t4, *t5;
 };
 
 oc_solver* oc_create(void) {
@@ -1154,7 +1154,7 @@ static void solver_free_data(oc_solver* h) {
   free(h->E.d), free(h->E.e);
   memset(&h->E, 0, sizeof(h->E));
   kkt_free(&h->K);
-  double** vs[] = {&h->q, &h->b, &h->x, &h->s, &h->z, &h->Px, &h->rx, &h->rz, &h->x1, &h->z1,
+  oc_real** vs[] = {&h->q, &h->b, &h->x, &h->s, &h->z, &h->Px, &h->rx, &h->rz, &h->x1, &h->z1,
                    &h->Px1, &h->dxa, &h->dza, &h->dsa, &h->dx, &h->dz, &h->ds, &h->wx, &h->ws,
                    &h->wz, &h->xo, &h->so, &h->zo, &h->trace, &h->rhs, &h->sol, &h->t1, &h->t2,
                    &h->t3, &h->t4, &h->t5};
@@ -1169,11 +1169,11 @@ void oc_destroy(oc_solver* h) {
   free(h);
 }
 
-#define ALLOC_V(ptr, len) (ptr) = (double*)xcalloc((size_t)(len), sizeof(double))
+#define ALLOC_V(ptr, len) (ptr) = (oc_real*)xcalloc((size_t)(len), sizeof(oc_real))
 
 /* SolverImpl::setup: solver_impl.cpp:8-55 */
-int oc_setup(oc_solver* h, int n, int m, const int* Pp, const int* Pi, const double* Px,
-             const double* q, const int* Ap, const int* Ai, const double* Ax, const double* b,
+int oc_setup(oc_solver* h, int n, int m, const int* Pp, const int* Pi, const oc_real* Px,
+             const oc_real* q, const int* Ap, const int* Ai, const oc_real* Ax, const oc_real* b,
              int zero_dim, int nonneg_dim, int n_soc, const int* soc_dims, const oc_settings* st) {
   h->setup_done = 0;
   if (zero_dim < 0 || nonneg_dim < 0 || n < 0) return 0;
@@ -1193,8 +1193,8 @@ int oc_setup(oc_solver* h, int n, int m, const int* Pp, const int* Pi, const dou
   csc_copy(&h->P, n, n, Pp, Pi, Px);
   csc_copy(&h->A, m, n, Ap, Ai, Ax);
   ALLOC_V(h->q, n), ALLOC_V(h->b, m);
-  memcpy(h->q, q, sizeof(double) * (size_t)n);
-  memcpy(h->b, b, sizeof(double) * (size_t)m);
+  memcpy(h->q, q, sizeof(oc_real) * (size_t)n);
+  memcpy(h->b, b, sizeof(oc_real) * (size_t)m);
   equil_compute(&h->E, &h->P, &h->A, &h->C, &h->st);
   if (h->st.equilibrate) {
     equil_scale_p(&h->E, &h->P);
@@ -1221,14 +1221,14 @@ int oc_setup(oc_solver* h, int n, int m, const int* Pp, const int* Pi, const dou
 }
 
 /* SolverImpl::updateData: solver_impl.cpp:57-91 */
-int oc_update_data(oc_solver* h, const int* Pp, const int* Pi, const double* Px, const double* q,
-                   const int* Ap, const int* Ai, const double* Ax, const double* b) {
+int oc_update_data(oc_solver* h, const int* Pp, const int* Pi, const oc_real* Px, const oc_real* q,
+                   const int* Ap, const int* Ai, const oc_real* Ax, const oc_real* b) {
   if (!h->setup_done) return 0;
   oc_csc Ps, As;
   memset(&Ps, 0, sizeof(Ps));
   memset(&As, 0, sizeof(As));
-  double* qs = NULL;
-  double* bs = NULL;
+  oc_real* qs = NULL;
+  oc_real* bs = NULL;
   int n = h->n, m = h->m, ok = 1;
   if (Px) {
     csc_copy(&Ps, n, n, Pp, Pi, Px);
@@ -1240,20 +1240,20 @@ int oc_update_data(oc_solver* h, const int* Pp, const int* Pi, const double* Px,
   }
   if (q) {
     ALLOC_V(qs, n);
-    memcpy(qs, q, sizeof(double) * (size_t)n);
+    memcpy(qs, q, sizeof(oc_real) * (size_t)n);
     if (h->st.equilibrate) equil_scale_q(&h->E, qs);
   }
   if (b) {
     ALLOC_V(bs, m);
-    memcpy(bs, b, sizeof(double) * (size_t)m);
+    memcpy(bs, b, sizeof(oc_real) * (size_t)m);
     if (h->st.equilibrate) equil_scale_b(&h->E, bs);
   }
   if (!kkt_update_data(&h->K, Px ? &Ps : NULL, Ax ? &As : NULL)) ok = 0;
   if (ok) {
     if (Px) csc_free(&h->P), h->P = Ps, memset(&Ps, 0, sizeof(Ps));
     if (Ax) csc_free(&h->A), h->A = As, memset(&As, 0, sizeof(As));
-    if (q) memcpy(h->q, qs, sizeof(double) * (size_t)n);
-    if (b) memcpy(h->b, bs, sizeof(double) * (size_t)m);
+    if (q) memcpy(h->q, qs, sizeof(oc_real) * (size_t)n);
+    if (b) memcpy(h->b, bs, sizeof(oc_real) * (size_t)m);
   }
   csc_free(&Ps), csc_free(&As);
   free(qs), free(bs);
@@ -1261,11 +1261,11 @@ int oc_update_data(oc_solver* h, const int* Pp, const int* Pi, const double* Px,
 }
 
 /* SolverImpl::setWarmStart: solver_impl.cpp:93-100 */
-void oc_set_warm_start(oc_solver* h, const double* x, const double* s, const double* z) {
+void oc_set_warm_start(oc_solver* h, const oc_real* x, const oc_real* s, const oc_real* z) {
   if (!h->setup_done) return;
-  memcpy(h->wx, x, sizeof(double) * (size_t)h->n);
-  memcpy(h->ws, s, sizeof(double) * (size_t)h->m);
-  memcpy(h->wz, z, sizeof(double) * (size_t)h->m);
+  memcpy(h->wx, x, sizeof(oc_real) * (size_t)h->n);
+  memcpy(h->ws, s, sizeof(oc_real) * (size_t)h->m);
+  memcpy(h->wz, z, sizeof(oc_real) * (size_t)h->m);
   if (h->st.equilibrate) equil_scale_solution(&h->E, h->wx, h->ws, h->wz);
   h->have_warm = 1;
 }
@@ -1273,11 +1273,11 @@ void oc_set_warm_start(oc_solver* h, const double* x, const double* s, const dou
 void oc_set_settings(oc_solver* h, const oc_settings* st) { h->st = *st; }
 
 /* shiftToInteriorCold / ensureStrictlyInteriorWarm: solver_impl.cpp:104-124 */
-static void shift_cold(oc_solver* h, double* v) {
-  double mn, ps;
+static void shift_cold(oc_solver* h, oc_real* v) {
+  oc_real mn, ps;
   cones_margins(&h->C, v, &mn, &ps);
-  double deg = (double)(h->C.degree > 1 ? h->C.degree : 1);
-  double target = dmax(1.0, 0.1 * ps / deg);
+  oc_real deg = (oc_real)(h->C.degree > 1 ? h->C.degree : 1);
+  oc_real target = dmax(1.0, 0.1 * ps / deg);
   if (mn <= 0.0) {
     cones_unit_shift(&h->C, v, -mn);
     cones_unit_shift(&h->C, v, target);
@@ -1287,8 +1287,8 @@ static void shift_cold(oc_solver* h, double* v) {
     cones_unit_shift(&h->C, v, 0.0);
   }
 }
-static void ensure_interior_warm(oc_solver* h, double* v) {
-  double mn, ps;
+static void ensure_interior_warm(oc_solver* h, oc_real* v) {
+  oc_real mn, ps;
   cones_margins(&h->C, v, &mn, &ps);
   if (mn <= 1e-8) cones_unit_shift(&h->C, v, -mn + 1e-6);
 }
@@ -1328,8 +1328,8 @@ static int constant_solve(oc_solver* h) {
   for (int i = 0; i < n; ++i) h->rhs[i] = -h->q[i];
   for (int i = 0; i < m; ++i) h->rhs[n + i] = h->b[i];
   kkt_solve(&h->K, h->rhs, h->sol);
-  memcpy(h->x1, h->sol, sizeof(double) * (size_t)n);
-  memcpy(h->z1, h->sol + n, sizeof(double) * (size_t)m);
+  memcpy(h->x1, h->sol, sizeof(oc_real) * (size_t)n);
+  memcpy(h->z1, h->sol + n, sizeof(oc_real) * (size_t)m);
   pmul(&h->P, h->x1, h->Px1);
   return vfinite(n, h->x1) && vfinite(m, h->z1);
 }
@@ -1338,7 +1338,7 @@ static int constant_solve(oc_solver* h) {
 static void residuals(oc_solver* h) {
   int n = h->n, m = h->m;
   pmul(&h->P, h->x, h->Px);
-  double* Atz = h->t1;
+  oc_real* Atz = h->t1;
   for (int i = 0; i < n; ++i) Atz[i] = 0.0;
   for (int i = 0; i < m; ++i) h->rz[i] = 0.0;
   for (int c = 0; c < n; ++c)
@@ -1355,24 +1355,24 @@ static void residuals(oc_solver* h) {
   h->rtau = h->dqx + h->dbz + h->kappa + h->dxPx / h->tau;
 }
 
-static double compute_mu(const oc_solver* h) {
-  return (h->dsz + h->tau * h->kappa) / (double)(h->C.degree + 1);
+static oc_real compute_mu(const oc_solver* h) {
+  return (h->dsz + h->tau * h->kappa) / (oc_real)(h->C.degree + 1);
 }
 
 /* shared tau/kappa elimination of computeAffineStep / computeCombinedStep
  * (solver_impl.cpp:200-231, 255-286) */
-static int tau_elim(oc_solver* h, const double* xv, const double* zv, double dtau_rhs,
-                    double dkappa_rhs, double* dtau_out) {
+static int tau_elim(oc_solver* h, const oc_real* xv, const oc_real* zv, oc_real dtau_rhs,
+                    oc_real dkappa_rhs, oc_real* dtau_out) {
   int n = h->n, m = h->m;
-  double* xi = h->t2;
-  double* Pv = h->t3;
+  oc_real* xi = h->t2;
+  oc_real* Pv = h->t3;
   for (int i = 0; i < n; ++i) xi[i] = h->x[i] / h->tau;
   pmul(&h->P, xv, Pv);
-  double num = dtau_rhs - dkappa_rhs / h->tau + vdot(n, h->q, xv) + vdot(m, h->b, zv) + 2.0 * vdot(n, xi, Pv);
-  double* xm = h->t4;
+  oc_real num = dtau_rhs - dkappa_rhs / h->tau + vdot(n, h->q, xv) + vdot(m, h->b, zv) + 2.0 * vdot(n, xi, Pv);
+  oc_real* xm = h->t4;
   for (int i = 0; i < n; ++i) xm[i] = xi[i] - h->x1[i];
   pmul(&h->P, xm, Pv);
-  double den = h->kappa / h->tau - vdot(n, h->q, h->x1) - vdot(m, h->b, h->z1) + vdot(n, xm, Pv) -
+  oc_real den = h->kappa / h->tau - vdot(n, h->q, h->x1) - vdot(m, h->b, h->z1) + vdot(n, xm, Pv) -
                vdot(n, h->x1, h->Px1);
   if (den == 0.0 || !isfinite(num) || !isfinite(den)) return 0;
   *dtau_out = num / den;
@@ -1385,14 +1385,14 @@ static int affine_step(oc_solver* h) {
   for (int i = 0; i < n; ++i) h->rhs[i] = h->rx[i];
   for (int i = 0; i < m; ++i) h->rhs[n + i] = h->s[i] - h->rz[i];
   kkt_solve(&h->K, h->rhs, h->sol);
-  const double* xv = h->sol;
-  const double* zv = h->sol + n;
+  const oc_real* xv = h->sol;
+  const oc_real* zv = h->sol + n;
   if (!vfinite(n + m, h->sol)) return 0;
-  double dtau_rhs = h->rtau, dkappa_rhs = h->tau * h->kappa;
+  oc_real dtau_rhs = h->rtau, dkappa_rhs = h->tau * h->kappa;
   if (!tau_elim(h, xv, zv, dtau_rhs, dkappa_rhs, &h->dtaua)) return 0;
   for (int i = 0; i < n; ++i) h->dxa[i] = xv[i] + h->dtaua * h->x1[i];
   for (int i = 0; i < m; ++i) h->dza[i] = zv[i] + h->dtaua * h->z1[i];
-  double* Hz = h->t5;
+  oc_real* Hz = h->t5;
   cones_mul_hs(&h->C, h->dza, Hz);
   for (int i = 0; i < m; ++i) h->dsa[i] = -(Hz[i] + h->s[i]);
   h->dkappaa = -(dkappa_rhs + h->kappa * h->dtaua) / h->tau;
@@ -1401,16 +1401,16 @@ static int affine_step(oc_solver* h) {
 }
 
 /* computeCombinedStep: solver_impl.cpp:233-286 */
-static int combined_step(oc_solver* h, double sigma, double mu) {
+static int combined_step(oc_solver* h, oc_real sigma, oc_real mu) {
   int n = h->n, m = h->m;
-  double* lam = (double*)xcalloc((size_t)m, sizeof(double));
-  double* lamp = (double*)xcalloc((size_t)m, sizeof(double));
-  double* wds = (double*)xcalloc((size_t)m, sizeof(double));
-  double* wdz = (double*)xcalloc((size_t)m, sizeof(double));
-  double* corr = (double*)xcalloc((size_t)m, sizeof(double));
-  double* dsc = (double*)xcalloc((size_t)m, sizeof(double));
-  double* lid = (double*)xcalloc((size_t)m, sizeof(double));
-  double* dsk = (double*)xcalloc((size_t)m, sizeof(double));
+  oc_real* lam = (oc_real*)xcalloc((size_t)m, sizeof(oc_real));
+  oc_real* lamp = (oc_real*)xcalloc((size_t)m, sizeof(oc_real));
+  oc_real* wds = (oc_real*)xcalloc((size_t)m, sizeof(oc_real));
+  oc_real* wdz = (oc_real*)xcalloc((size_t)m, sizeof(oc_real));
+  oc_real* corr = (oc_real*)xcalloc((size_t)m, sizeof(oc_real));
+  oc_real* dsc = (oc_real*)xcalloc((size_t)m, sizeof(oc_real));
+  oc_real* lid = (oc_real*)xcalloc((size_t)m, sizeof(oc_real));
+  oc_real* dsk = (oc_real*)xcalloc((size_t)m, sizeof(oc_real));
   int ok = 0;
   cones_apply_w(&h->C, h->z, lam);
   cones_product(&h->C, lam, lam, lamp);
@@ -1423,15 +1423,15 @@ static int combined_step(oc_solver* h, double sigma, double mu) {
   for (int i = 0; i < n; ++i) h->rhs[i] = (1.0 - sigma) * h->rx[i];
   for (int i = 0; i < m; ++i) h->rhs[n + i] = dsk[i] - (1.0 - sigma) * h->rz[i];
   kkt_solve(&h->K, h->rhs, h->sol);
-  const double* xv = h->sol;
-  const double* zv = h->sol + n;
+  const oc_real* xv = h->sol;
+  const oc_real* zv = h->sol + n;
   if (vfinite(n + m, h->sol)) {
-    double dtau_rhs = (1.0 - sigma) * h->rtau;
-    double dkappa_rhs = h->tau * h->kappa + h->dtaua * h->dkappaa - sigma * mu;
+    oc_real dtau_rhs = (1.0 - sigma) * h->rtau;
+    oc_real dkappa_rhs = h->tau * h->kappa + h->dtaua * h->dkappaa - sigma * mu;
     if (tau_elim(h, xv, zv, dtau_rhs, dkappa_rhs, &h->dtau)) {
       for (int i = 0; i < n; ++i) h->dx[i] = xv[i] + h->dtau * h->x1[i];
       for (int i = 0; i < m; ++i) h->dz[i] = zv[i] + h->dtau * h->z1[i];
-      double* Hz = h->t5;
+      oc_real* Hz = h->t5;
       cones_mul_hs(&h->C, h->dz, Hz);
       for (int i = 0; i < m; ++i) h->ds[i] = -(Hz[i] + dsk[i]);
       h->dkappa = -(dkappa_rhs + h->kappa * h->dtau) / h->tau;
@@ -1444,9 +1444,9 @@ static int combined_step(oc_solver* h, double sigma, double mu) {
 }
 
 /* computeStepLength: solver_impl.cpp:288-296 */
-static double step_length(const oc_solver* h, const double* ds, const double* dz, double dtau,
-                          double dkappa) {
-  double alpha = 1.0;
+static oc_real step_length(const oc_solver* h, const oc_real* ds, const oc_real* dz, oc_real dtau,
+                          oc_real dkappa) {
+  oc_real alpha = 1.0;
   if (dtau < 0) alpha = dmin(alpha, -h->tau / dtau);
   if (dkappa < 0) alpha = dmin(alpha, -h->kappa / dkappa);
   alpha = cones_max_step(&h->C, h->z, dz, alpha);
@@ -1458,14 +1458,14 @@ static double step_length(const oc_solver* h, const double* ds, const double* dz
 static void finalize(oc_solver* h) {
   int n = h->n, m = h->m;
   int inf = (h->status == ST_PRIMAL_INF || h->status == ST_DUAL_INF);
-  double scale = inf ? 1.0 / dmax(h->kappa, 1e-30) : 1.0 / dmax(h->tau, 1e-30);
+  oc_real scale = inf ? 1.0 / dmax(h->kappa, 1e-30) : 1.0 / dmax(h->tau, 1e-30);
   for (int i = 0; i < n; ++i) h->xo[i] = h->x[i] * scale;
   for (int i = 0; i < m; ++i) h->so[i] = h->s[i] * scale, h->zo[i] = h->z[i] * scale;
   if (h->st.equilibrate) equil_unscale_solution(&h->E, h->xo, h->so, h->zo);
-  double pobj = (0.5 * h->dxPx / h->tau + h->dqx) / h->tau;
+  oc_real pobj = (0.5 * h->dxPx / h->tau + h->dqx) / h->tau;
   if (h->st.equilibrate) pobj /= h->E.c;
   h->objective = pobj;
-  double gap = h->dsz / (h->tau * h->tau);
+  oc_real gap = h->dsz / (h->tau * h->tau);
   if (h->st.equilibrate) gap /= h->E.c;
   h->gap = gap;
   h->pres = vnorm(m, h->rz) / dmax(h->tau, 1e-30);
@@ -1486,12 +1486,12 @@ int oc_solve(oc_solver* h) {
   int n = h->n, m = h->m;
   free(h->trace);
   int tcap = h->st.max_iter > 0 ? h->st.max_iter : 1;
-  h->trace = (double*)xcalloc((size_t)tcap * 8, sizeof(double));
+  h->trace = (oc_real*)xcalloc((size_t)tcap * 8, sizeof(oc_real));
   int use_warm = h->st.warm_start && h->have_warm;
   if (use_warm) {
-    memcpy(h->x, h->wx, sizeof(double) * (size_t)n);
-    memcpy(h->s, h->ws, sizeof(double) * (size_t)m);
-    memcpy(h->z, h->wz, sizeof(double) * (size_t)m);
+    memcpy(h->x, h->wx, sizeof(oc_real) * (size_t)n);
+    memcpy(h->s, h->ws, sizeof(oc_real) * (size_t)m);
+    memcpy(h->z, h->wz, sizeof(oc_real) * (size_t)m);
     h->tau = 1.0, h->kappa = 1.0;
     cones_zero_zero_blocks(&h->C, h->s);
     ensure_interior_warm(h, h->s);
@@ -1502,32 +1502,32 @@ int oc_solve(oc_solver* h) {
     return h->status;
   }
   int final_status = ST_MAXIT, iterations = 0;
-  double* saff = (double*)xcalloc((size_t)m, sizeof(double));
-  double* zaff = (double*)xcalloc((size_t)m, sizeof(double));
+  oc_real* saff = (oc_real*)xcalloc((size_t)m, sizeof(oc_real));
+  oc_real* zaff = (oc_real*)xcalloc((size_t)m, sizeof(oc_real));
   for (int iter = 0; iter < h->st.max_iter; ++iter) {
     iterations = iter + 1;
-    double* tr = h->trace + 8 * iter;
+    oc_real* tr = h->trace + 8 * iter;
     cones_update_scaling(&h->C, h->s, h->z);
     if (!kkt_update_scaling(&h->K, &h->C) || !constant_solve(h)) {
       final_status = ST_NUMERR;
       break;
     }
     residuals(h);
-    double mu = compute_mu(h);
-    double nrx = vnorm(n, h->rx) / h->tau;
-    double nrz = vnorm(m, h->rz) / h->tau;
-    double pobj = (0.5 * h->dxPx / h->tau + h->dqx) / h->tau;
-    double dobj = (-0.5 * h->dxPx / h->tau - h->dbz) / h->tau;
+    oc_real mu = compute_mu(h);
+    oc_real nrx = vnorm(n, h->rx) / h->tau;
+    oc_real nrz = vnorm(m, h->rz) / h->tau;
+    oc_real pobj = (0.5 * h->dxPx / h->tau + h->dqx) / h->tau;
+    oc_real dobj = (-0.5 * h->dxPx / h->tau - h->dbz) / h->tau;
     tr[0] = mu, tr[1] = nrx, tr[2] = nrz, tr[3] = pobj, tr[4] = dobj;
     h->ntrace = iter + 1;
-    double gap_abs = fabs(pobj - dobj);
-    double gap_tol = h->st.tol_gap_abs + h->st.tol_gap_rel * dmax(fabs(pobj), fabs(dobj));
+    oc_real gap_abs = fabs(pobj - dobj);
+    oc_real gap_tol = h->st.tol_gap_abs + h->st.tol_gap_rel * dmax(fabs(pobj), fabs(dobj));
     if (nrx < h->st.tol_feas && nrz < h->st.tol_feas && gap_abs < gap_tol && h->tau > 1e-6) {
       final_status = ST_SOLVED;
       break;
     }
     {
-      double ratio = h->tau / dmax(h->kappa, 1e-30);
+      oc_real ratio = h->tau / dmax(h->kappa, 1e-30);
       if (ratio <= h->st.tol_infeas) {
         if (h->dbz < -h->st.tol_infeas * dmax(1.0, vnorm(m, h->z))) {
           final_status = ST_PRIMAL_INF;
@@ -1543,14 +1543,14 @@ int oc_solve(oc_solver* h) {
       final_status = ST_NUMERR;
       break;
     }
-    double aa = step_length(h, h->dsa, h->dza, h->dtaua, h->dkappaa);
+    oc_real aa = step_length(h, h->dsa, h->dza, h->dtaua, h->dkappaa);
     for (int i = 0; i < m; ++i) saff[i] = h->s[i] + aa * h->dsa[i], zaff[i] = h->z[i] + aa * h->dza[i];
-    double tau_a = h->tau + aa * h->dtaua;
-    double kap_a = h->kappa + aa * h->dkappaa;
-    double mu_aff = (vdot(m, saff, zaff) + tau_a * kap_a) / (double)(h->C.degree + 1);
-    double sigma;
+    oc_real tau_a = h->tau + aa * h->dtaua;
+    oc_real kap_a = h->kappa + aa * h->dkappaa;
+    oc_real mu_aff = (vdot(m, saff, zaff) + tau_a * kap_a) / (oc_real)(h->C.degree + 1);
+    oc_real sigma;
     if (mu > 0) {
-      double r = dmax(mu_aff / mu, 0.0);
+      oc_real r = dmax(mu_aff / mu, 0.0);
       sigma = dmin(dmax(r * r * r, 0.0), 1.0);
     } else {
       sigma = 1.0;
@@ -1560,7 +1560,7 @@ int oc_solve(oc_solver* h) {
       final_status = ST_NUMERR;
       break;
     }
-    double alpha = step_length(h, h->ds, h->dz, h->dtau, h->dkappa) * h->st.max_step_fraction;
+    oc_real alpha = step_length(h, h->ds, h->dz, h->dtau, h->dkappa) * h->st.max_step_fraction;
     tr[7] = alpha;
     if (!isfinite(alpha) || alpha <= 0.0) {
       final_status = ST_NUMERR;
@@ -1570,9 +1570,9 @@ int oc_solve(oc_solver* h) {
     for (int i = 0; i < m; ++i) h->s[i] += alpha * h->ds[i], h->z[i] += alpha * h->dz[i];
     h->tau += alpha * h->dtau;
     h->kappa += alpha * h->dkappa;
-    double sc = dmax(h->tau, h->kappa);
+    oc_real sc = dmax(h->tau, h->kappa);
     if (sc > 1e8 || sc < 1e-8) {
-      double inv = 1.0 / sc;
+      oc_real inv = 1.0 / sc;
       for (int i = 0; i < n; ++i) h->x[i] *= inv;
       for (int i = 0; i < m; ++i) h->s[i] *= inv, h->z[i] *= inv;
       h->tau *= inv, h->kappa *= inv;
@@ -1583,19 +1583,19 @@ int oc_solve(oc_solver* h) {
   h->iterations = iterations;
   finalize(h);
   if (final_status == ST_SOLVED && h->st.warm_start) {
-    memcpy(h->wx, h->x, sizeof(double) * (size_t)n);
-    memcpy(h->ws, h->s, sizeof(double) * (size_t)m);
-    memcpy(h->wz, h->z, sizeof(double) * (size_t)m);
+    memcpy(h->wx, h->x, sizeof(oc_real) * (size_t)n);
+    memcpy(h->ws, h->s, sizeof(oc_real) * (size_t)m);
+    memcpy(h->wz, h->z, sizeof(oc_real) * (size_t)m);
     h->have_warm = 1;
   }
   return h->status;
 }
 
-int oc_get_solution(const oc_solver* h, double* x, double* s, double* z, double* info,
+int oc_get_solution(const oc_solver* h, oc_real* x, oc_real* s, oc_real* z, oc_real* info,
                     int* iterations) {
-  if (h->xo && x) memcpy(x, h->xo, sizeof(double) * (size_t)h->n);
-  if (h->so && s) memcpy(s, h->so, sizeof(double) * (size_t)h->m);
-  if (h->zo && z) memcpy(z, h->zo, sizeof(double) * (size_t)h->m);
+  if (h->xo && x) memcpy(x, h->xo, sizeof(oc_real) * (size_t)h->n);
+  if (h->so && s) memcpy(s, h->so, sizeof(oc_real) * (size_t)h->m);
+  if (h->zo && z) memcpy(z, h->zo, sizeof(oc_real) * (size_t)h->m);
   if (info) {
     info[0] = h->pres, info[1] = h->dres, info[2] = h->gap, info[3] = h->mu_out;
     info[4] = h->objective;
@@ -1604,40 +1604,40 @@ int oc_get_solution(const oc_solver* h, double* x, double* s, double* z, double*
   return h->status;
 }
 
-int oc_get_trace(const oc_solver* h, double* out, int max_rows) {
+int oc_get_trace(const oc_solver* h, oc_real* out, int max_rows) {
   int r = h->ntrace < max_rows ? h->ntrace : max_rows;
-  if (r > 0) memcpy(out, h->trace, sizeof(double) * 8 * (size_t)r);
+  if (r > 0) memcpy(out, h->trace, sizeof(oc_real) * 8 * (size_t)r);
   return r;
 }
 
-void oc_get_kkt_stats(const oc_solver* h, double* out4) {
+void oc_get_kkt_stats(const oc_solver* h, oc_real* out4) {
   out4[0] = h->K.N;
   out4[1] = h->K.Kp ? h->K.Kp[h->K.N] : 0;
-  out4[2] = h->K.F.Lp ? (double)h->K.F.Lp[h->K.N] + h->K.N : 0;
+  out4[2] = h->K.F.Lp ? (oc_real)h->K.F.Lp[h->K.N] + h->K.N : 0;
   out4[3] = h->K.F.flops;
 }
 
 /* ---- per-kernel entry points ---- */
 void oc_cone_margins(int zero_dim, int nonneg_dim, int n_soc, const int* soc_dims,
-                     const double* x, double* min_margin, double* pos_sum) {
+                     const oc_real* x, oc_real* min_margin, oc_real* pos_sum) {
   oc_cones C;
   cones_init(&C, zero_dim, nonneg_dim, n_soc, soc_dims);
   cones_margins(&C, x, min_margin, pos_sum);
   cones_free(&C);
 }
 
-double oc_cone_max_step(int zero_dim, int nonneg_dim, int n_soc, const int* soc_dims,
-                        const double* x, const double* dx, double alpha_max) {
+oc_real oc_cone_max_step(int zero_dim, int nonneg_dim, int n_soc, const int* soc_dims,
+                        const oc_real* x, const oc_real* dx, oc_real alpha_max) {
   oc_cones C;
   cones_init(&C, zero_dim, nonneg_dim, n_soc, soc_dims);
-  double a = cones_max_step(&C, x, dx, alpha_max);
+  oc_real a = cones_max_step(&C, x, dx, alpha_max);
   cones_free(&C);
   return a;
 }
 
 void oc_cone_scaling_apply(int zero_dim, int nonneg_dim, int n_soc, const int* soc_dims,
-                           const double* s, const double* z, const double* v, double* lambda,
-                           double* Hv, double* Wv, double* Winv_v) {
+                           const oc_real* s, const oc_real* z, const oc_real* v, oc_real* lambda,
+                           oc_real* Hv, oc_real* Wv, oc_real* Winv_v) {
   oc_cones C;
   cones_init(&C, zero_dim, nonneg_dim, n_soc, soc_dims);
   cones_update_scaling(&C, s, z);
@@ -1648,9 +1648,9 @@ void oc_cone_scaling_apply(int zero_dim, int nonneg_dim, int n_soc, const int* s
   cones_free(&C);
 }
 
-void oc_equilibrate(int n, int m, const int* Pp, const int* Pi, const double* Px, const int* Ap,
-                    const int* Ai, const double* Ax, int zero_dim, int nonneg_dim, int n_soc,
-                    const int* soc_dims, const oc_settings* st, double* d, double* e, double* c) {
+void oc_equilibrate(int n, int m, const int* Pp, const int* Pi, const oc_real* Px, const int* Ap,
+                    const int* Ai, const oc_real* Ax, int zero_dim, int nonneg_dim, int n_soc,
+                    const int* soc_dims, const oc_settings* st, oc_real* d, oc_real* e, oc_real* c) {
   oc_cones C;
   oc_csc P, A;
   oc_equil E;
@@ -1659,18 +1659,18 @@ void oc_equilibrate(int n, int m, const int* Pp, const int* Pi, const double* Px
   csc_copy(&P, n, n, Pp, Pi, Px);
   csc_copy(&A, m, n, Ap, Ai, Ax);
   equil_compute(&E, &P, &A, &C, st);
-  memcpy(d, E.d, sizeof(double) * (size_t)n);
-  memcpy(e, E.e, sizeof(double) * (size_t)m);
+  memcpy(d, E.d, sizeof(oc_real) * (size_t)n);
+  memcpy(e, E.e, sizeof(oc_real) * (size_t)m);
   *c = E.c;
   free(E.d), free(E.e);
   csc_free(&P), csc_free(&A);
   cones_free(&C);
 }
 
-double oc_kkt_solve(int n, int m, const int* Pp, const int* Pi, const double* Px, const int* Ap,
-                    const int* Ai, const double* Ax, int zero_dim, int nonneg_dim, int n_soc,
-                    const int* soc_dims, const double* s, const double* z, const oc_settings* st,
-                    const double* rhs, double* x_out, int* ok) {
+oc_real oc_kkt_solve(int n, int m, const int* Pp, const int* Pi, const oc_real* Px, const int* Ap,
+                    const int* Ai, const oc_real* Ax, int zero_dim, int nonneg_dim, int n_soc,
+                    const int* soc_dims, const oc_real* s, const oc_real* z, const oc_settings* st,
+                    const oc_real* rhs, oc_real* x_out, int* ok) {
   oc_cones C;
   oc_csc P, A;
   oc_kkt K;
@@ -1678,7 +1678,7 @@ double oc_kkt_solve(int n, int m, const int* Pp, const int* Pi, const double* Px
   cones_init(&C, zero_dim, nonneg_dim, n_soc, soc_dims);
   csc_copy(&P, n, n, Pp, Pi, Px);
   csc_copy(&A, m, n, Ap, Ai, Ax);
-  double rel = NAN;
+  oc_real rel = NAN;
   *ok = kkt_setup(&K, &P, &A, &C, st);
   if (*ok && s && z) {
     cones_update_scaling(&C, s, z);

@@ -298,3 +298,76 @@ def kkt_solve(P, A, zero_dim, nonneg_dim, soc_dims, rhs, s=None, z=None, **st):
                              sdp, sp, zp, ctypes.byref(stt), rp, x.ctypes.data_as(_f64p),
                              ctypes.byref(ok))
     return x, rel, bool(ok.value)
+
+
+# ---------------------------------------------------------------------------
+# The same oracle in x87 extended precision (liboracle_xp.so: every oc_real is
+# a long double, 64-bit significand).  Used by the tests only, to compute a
+# problem's optimum to far below the FP64 method's rounding-level spread.
+# ---------------------------------------------------------------------------
+_LIB_XP_PATH = os.path.join(_HERE, "liboracle_xp.so")
+_xp = None
+_xpp = ctypes.POINTER(ctypes.c_longdouble)
+
+
+class OcSettingsXP(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_longdouble if t is ctypes.c_double else t) for name, t in OcSettings._fields_]
+
+
+def lib_xp():
+    global _xp
+    if _xp is None:
+        if not os.path.exists(_LIB_XP_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_XP_PATH)
+        L.oc_create.restype = ctypes.c_void_p
+        L.oc_destroy.argtypes = [ctypes.c_void_p]
+        L.oc_default_settings.argtypes = [ctypes.POINTER(OcSettingsXP)]
+        L.oc_setup.restype = ctypes.c_int
+        L.oc_setup.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _i32p, _i32p, _xpp,
+                               _xpp, _i32p, _i32p, _xpp, _xpp, ctypes.c_int, ctypes.c_int,
+                               ctypes.c_int, _i32p, ctypes.POINTER(OcSettingsXP)]
+        L.oc_solve.restype = ctypes.c_int
+        L.oc_solve.argtypes = [ctypes.c_void_p]
+        L.oc_get_solution.restype = ctypes.c_int
+        L.oc_get_solution.argtypes = [ctypes.c_void_p, _xpp, _xpp, _xpp, _xpp, _i32p]
+        _xp = L
+    return _xp
+
+
+def _x(a):
+    a = np.ascontiguousarray(a, dtype=np.longdouble)
+    return a, a.ctypes.data_as(_xpp)
+
+
+def solve_xp(P, q, A, b, zero_dim=0, nonneg_dim=0, soc_dims=(), **st):
+    """The oracle's IPM in extended precision; returns the same dict as solve()
+    with x, s, z as np.longdouble arrays."""
+    assert np.dtype(np.longdouble).itemsize > 8, "long double is no wider than double here"
+    L = lib_xp()
+    P = _csc(P)
+    A = _csc(A)
+    n, m = P.shape[0], A.shape[0]
+    keep = [_i(P.indptr), _i(P.indices), _x(P.data), _i(A.indptr), _i(A.indices), _x(A.data), _x(q), _x(b),
+            _i(np.asarray(soc_dims, dtype=np.int32).reshape(-1))]
+    s = OcSettingsXP()
+    L.oc_default_settings(ctypes.byref(s))
+    for k, v in st.items():
+        setattr(s, k, v)
+    h = ctypes.c_void_p(L.oc_create())
+    try:
+        ok = L.oc_setup(h, n, m, keep[0][1], keep[1][1], keep[2][1], keep[6][1], keep[3][1], keep[4][1],
+                        keep[5][1], keep[7][1], int(zero_dim), int(nonneg_dim), len(keep[8][0]), keep[8][1],
+                        ctypes.byref(s))
+        if not ok:
+            return None
+        status = L.oc_solve(h)
+        x = np.zeros(n, dtype=np.longdouble); sv = np.zeros(m, dtype=np.longdouble)
+        z = np.zeros(m, dtype=np.longdouble); info = np.zeros(5, dtype=np.longdouble)
+        it = ctypes.c_int(0)
+        L.oc_get_solution(h, x.ctypes.data_as(_xpp), sv.ctypes.data_as(_xpp), z.ctypes.data_as(_xpp),
+                          info.ctypes.data_as(_xpp), ctypes.byref(it))
+    finally:
+        L.oc_destroy(h)
+    return dict(status=int(status), x=x, s=sv, z=z, objective=info[4], primal_residual=info[0],
+                dual_residual=info[1], duality_gap=info[2], mu=info[3], iterations=int(it.value))

@@ -97,3 +97,20 @@ def test_ordering_quality_comparable_to_exact_min_degree(oracle):
     assert s.setup(p.P, p.q, p.A, p.b, p.zero_dim, p.nonneg_dim, p.soc_dims)
     nnz_oracle = s.kkt_stats()["nnz_L"]
     assert st["nnz_L_exact"] < 1.25 * nnz_oracle
+
+
+@pytest.mark.parametrize("maker", [lambda: PR.config0(seed=2), lambda: PR.config2_problem(7),
+                                   lambda: PR.config3_problem(1), lambda: PR.config1(scale=0.02)],
+                         ids=["config0", "contact", "dense", "config1"])
+@pytest.mark.parametrize("st", [{}, {"equilibrate_max_iter": 25}, {"equilibrate_max_scale": 2.0}],
+                         ids=["default", "iters25", "maxscale2"])
+def test_library_equilibration_matches_oracle(maker, st, oracle):
+    """The library's host Ruiz equilibration (what setup() scales by) against
+    the oracle's restatement of equilibration.cpp:8-99, no GPU needed."""
+    from conicxx_amd import ConeSpec, Settings, equilibrate
+    p = maker()
+    d, e, c = equilibrate(p.P, p.A, ConeSpec(p.zero_dim, p.nonneg_dim, p.soc_dims), Settings(**st))
+    do, eo, co = oracle.equilibrate(p.P, p.A, p.zero_dim, p.nonneg_dim, p.soc_dims, **st)
+    np.testing.assert_allclose(d, do, rtol=1e-14, atol=0)
+    np.testing.assert_allclose(e, eo, rtol=1e-14, atol=0)
+    assert abs(c - co) <= 1e-14 * abs(co)

@@ -144,6 +144,28 @@ def test_autograd_wrapper_batch():
         assert relerr(b.grad[k].numpy(), ref["b"]) < 5e-3
         assert relerr(Av.grad[k].numpy(), ref["A"]) < 5e-3
         assert relerr(Pv.grad[k].numpy(), ref["P"]) < 5e-3
+    # a second call with new values goes through updateData(P, q, A, b) on the
+    # pool's solvers (same patterns); its gradients against the reference at
+    # the new solutions
+    rng = np.random.default_rng(8)
+    Pv2 = (Pv.detach() * 1.2).requires_grad_(True)
+    Av2 = (Av.detach() * (1.0 + 0.02 * torch.tensor(rng.standard_normal(Av.shape)))).requires_grad_(True)
+    q2 = (q.detach() + 0.01 * torch.tensor(rng.standard_normal(q.shape))).requires_grad_(True)
+    b2 = (b.detach() + 0.01 * torch.tensor(np.abs(rng.standard_normal(b.shape)))).requires_grad_(True)
+    x2, s2, z2 = layer(Pv2, q2, Av2, b2)
+    (W * x2).sum().backward()
+    assert relerr(x2.detach().numpy(), x.detach().numpy()) > 1e-4  # a different problem
+    for k, p in enumerate(probs):
+        pk = PR.Problem(sp.csc_matrix((Pv2[k].detach().numpy(), p.P.indices, p.P.indptr), shape=p.P.shape),
+                        q2[k].detach().numpy(),
+                        sp.csc_matrix((Av2[k].detach().numpy(), p.A.indices, p.A.indptr), shape=p.A.shape),
+                        b2[k].detach().numpy(), p.zero_dim, p.nonneg_dim, p.soc_dims)
+        ref = implicit_gradients(pk, x2[k].detach().numpy(), s2[k].detach().numpy(), z2[k].detach().numpy(),
+                                 p.meta["w"])
+        assert relerr(q2.grad[k].numpy(), ref["q"]) < 5e-3
+        assert relerr(b2.grad[k].numpy(), ref["b"]) < 5e-3
+        assert relerr(Av2.grad[k].numpy(), ref["A"]) < 5e-3
+        assert relerr(Pv2.grad[k].numpy(), ref["P"]) < 5e-3
 
 
 def test_batched_backward_matches_reference_and_single_problem_path():

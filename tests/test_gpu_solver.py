@@ -4,18 +4,22 @@ on the same seeded inputs, and against the reference's known answers.
 Parity bar (floating point).  The reference method amplifies rounding: the
 oracle itself, rerun on inputs perturbed by 1e-15 (relative, on q), changes
 its iteration count and moves x, s, z by up to ~5e-6 on these instances, and
-can flip a MaxIterations stall into Solved.  So each comparison measures that
-envelope (the oracle on the exact input plus 3 perturbed copies) and requires:
+can flip a MaxIterations stall into Solved.  Even the same method in x87
+extended precision moves x by ~1e-7 between stopping tolerances 1e-10 and
+1e-11: at the default 1e-8 stopping test x is only determined to ~1e-6.  So
+each comparison measures that envelope (the oracle on the exact input plus 3
+perturbed copies) and requires:
   * the GPU status is one the oracle reaches within the envelope;
-  * x, s, z within max(1e-8, 2 * spread) relative (max|a-b| / max(1, max|b|)),
-    spread = the largest difference among the oracle's own same-status runs;
-  * the objective within max(1e-8, 2 * objective spread) relative;
+  * x, s, z and the objective within max(1e-8, 2 * spread) relative
+    (max|a-b| / max(1, max|b|)) of the nearest same-status oracle run,
+    spread = the largest difference among those runs;
+  * where that fails (the oracle runs cluster on one side of the tolerance
+    ball while the GPU run stops elsewhere inside it), the GPU solution must
+    be as close to the problem's optimum as the oracle's runs are: within
+    max(2e-7, 2 * the oracle runs' largest distance) of x*, s*, z* and the
+    objective of the extended-precision oracle solved to 1e-11 (x* itself is
+    known to ~1e-7 there);
   * for Solved results, the KKT optimality certificate of the GPU solution.
-Where the oracle itself stalls on an instance (its same-input runs disagree on
-the iteration count by more than 25%, or one of them hits max_iter), its
-converged runs cluster on one side of the 1e-8 tolerance ball while a run that
-converges directly stops elsewhere inside it; there a Solved GPU result is
-accepted within 1e-6 relative of the oracle together with the certificate.
 """
 import numpy as np
 import pytest
@@ -57,6 +61,18 @@ def _errs(sol, r):
     return e
 
 
+def optimum_xp(oracle, prob):
+    """x*, s*, z*, objective of the extended-precision oracle (tolerances 1e-11,
+    1e-10 if that stalls); None if neither converges."""
+    for tol in (1e-11, 1e-10):
+        r = oracle.solve_xp(prob.P, prob.q, prob.A, prob.b, prob.zero_dim, prob.nonneg_dim, prob.soc_dims,
+                            tol_feas=tol, tol_gap_abs=tol, tol_gap_rel=tol, refine_tol=1e-18, warm_start=0)
+        if r is not None and r["status"] == 1:
+            return {k: np.asarray(r[k], dtype=np.float64) for k in ("x", "s", "z")} | \
+                {"objective": float(r["objective"]), "tol": tol}
+    return None
+
+
 def assert_parity(sol, runs, prob=None):
     if isinstance(runs, dict):
         runs = [runs]
@@ -72,13 +88,19 @@ def assert_parity(sol, runs, prob=None):
     print("parity", {k: f"{v:.2e}" for k, v in best.items()},
           "oracle spread", {k: f"{v:.1e}" for k, v in spread.items()},
           "iters gpu", sol.info.iterations, "oracle", [r["iterations"] for r in runs])
-    its = [r["iterations"] for r in runs]
-    erratic = max(its) > 1.25 * min(its) or max(its) >= 200
-    for k, v in best.items():
-        bar = max(TOL, 2.0 * spread[k])
-        if erratic and sol.status == Status.Solved:
-            bar = max(bar, 1e-6)
-        assert v <= bar, (k, v, spread[k])
+    miss = [k for k, v in best.items() if v > max(TOL, 2.0 * spread[k])]
+    if miss:
+        # accuracy against the optimum instead of closeness to the oracle's cluster
+        assert sol.status == Status.Solved and prob is not None, (miss, best, spread)
+        import oracle_py
+        opt = optimum_xp(oracle_py, prob)
+        assert opt is not None, "the extended-precision oracle did not converge"
+        e_gpu = _errs(sol, opt)
+        e_orc = {k: max(_errs(r, opt)[k] for r in same) for k in e_gpu}
+        print("  vs optimum (xp tol", opt["tol"], "): gpu", {k: f"{v:.1e}" for k, v in e_gpu.items()},
+              "oracle runs", {k: f"{v:.1e}" for k, v in e_orc.items()})
+        for k in e_gpu:
+            assert e_gpu[k] <= max(2e-7, 2.0 * e_orc[k]), (k, e_gpu[k], e_orc[k])
     if prob is not None and sol.status == Status.Solved:
         c = PR.kkt_certificate(prob, sol.x, sol.s, sol.z)
         scale = 1.0 + np.abs(prob.q).max() + np.abs(prob.b).max()
@@ -231,3 +253,40 @@ def test_large_sparse_certificate_many_panel_tasks():
     scale = 1.0 + np.abs(p.q).max() + np.abs(p.b).max()
     assert c["stationarity"] < 1e-6 * scale and c["feasibility"] < 1e-6 * scale, c
     assert c["margin_s"] > -1e-7 and c["margin_z"] > -1e-7, c
+
+
+def test_update_data_new_P_and_A_values(oracle):
+    """Solver::updateData with new P and A values on the setup's pattern
+    (solver_impl.cpp:57-91, kkt_system.cpp:119-137): device rescaling with the
+    setup's equilibration, scatter into K, refactorization.  The updated solve
+    against the oracle running the same updateData sequence, and against a
+    fresh setup of the new data (same status, certificate)."""
+    p = PR.config0(seed=6)
+    rng = np.random.default_rng(21)
+    P2 = p.P.copy()  # 1.3 P + 0.05 I: positive definite on the same pattern
+    P2.data = P2.data * 1.3
+    col = np.repeat(np.arange(p.n), np.diff(P2.indptr))
+    P2.data[P2.indices == col] += 0.05
+    A2 = p.A.copy()
+    A2.data = A2.data * (1.0 + 0.2 * rng.standard_normal(A2.nnz))
+    q2, b2, *_ = PR.feasible_qb(rng, P2, A2, p.zero_dim, p.nonneg_dim, p.soc_dims)
+    p2 = PR.Problem(P2, q2, A2, b2, p.zero_dim, p.nonneg_dim, p.soc_dims)
+    st = Settings(warm_start=False)
+    s = Solver()
+    assert s.setup(p.P, p.q, p.A, p.b, ConeSpec(p.zero_dim, p.nonneg_dim, p.soc_dims), st)
+    assert s.solve().status == Status.Solved
+    assert s.updateData(P=P2, q=q2, A=A2, b=b2)
+    sol = s.solve()
+    runs = []
+    rng2 = np.random.default_rng(12345)
+    for k in range(NPERT + 1):
+        o = oracle.OracleSolver()
+        assert o.setup(p.P, p.q, p.A, p.b, p.zero_dim, p.nonneg_dim, p.soc_dims, warm_start=0)
+        assert o.solve()["status"] == 1
+        qk = q2 * (1.0 + (1e-15 * rng2.standard_normal(p.n) if k else 0.0))
+        assert o.update_data(P=P2, q=qk, A=A2, b=b2)
+        runs.append(o.solve())
+    assert_parity(sol, runs, p2)
+    s2, fresh = gpu_solve(p2, st)
+    assert fresh.status == sol.status == Status.Solved
+    assert rel(sol.x, fresh.x) < 1e-5

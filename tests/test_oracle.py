@@ -178,6 +178,48 @@ def test_equilibration_soc_block_uniform_and_ruiz_converges(oracle):
     assert 0.5 < blk.min() and blk.max() < 2.0
 
 
+@pytest.mark.parametrize("maker", [lambda: PR.config0(seed=1), lambda: PR.config2_problem(3),
+                                   lambda: PR.config3_problem(2)], ids=["config0", "contact", "dense"])
+def test_equilibration_cost_scale_is_inverse_mean_of_scaled_P(oracle, maker):
+    """equilibration.cpp:87-99: c = 1 / max(mean |P_hat|, min_scale) over the
+    stored entries of the scaled upper triangle, then clamped.  Recomputed here
+    from the oracle's own d (no clamp of d is active on these instances, so
+    P_hat = D P D exactly): a wrong entry set (full P instead of the upper
+    triangle, an unscaled P), max instead of mean, or the clamp order would
+    move c."""
+    p = maker()
+    d, e, c = oracle.equilibrate(p.P, p.A, p.zero_dim, p.nonneg_dim, p.soc_dims)
+    assert 1e-4 < d.min() and d.max() < 1e4
+    U = sp.csc_matrix(p.P)
+    Ph = sp.diags(d) @ U @ sp.diags(d)
+    mean = np.abs(Ph.data).sum() / U.nnz
+    expect = min(max(1.0 / max(mean, 1e-4), 1e-4), 1e4)
+    assert abs(c - expect) <= 1e-12 * expect, (c, expect)
+    # the full symmetric P (both triangles) gives a different mean on these instances
+    Pf = sp.diags(d) @ p.P_full() @ sp.diags(d)
+    assert abs(1.0 / (np.abs(Pf.data).sum() / Pf.nnz) - c) > 1e-6 * c
+
+
+def test_equilibration_cost_scale_special_cases(oracle):
+    """c = 1 when P has no entries or only stored zeros (the mean is 0); the
+    upper clamp equilibrate_max_scale (1e4) when P is tiny; and a narrower
+    max_scale clamps c to it."""
+    A = _sp([(0, 0, 1.0), (1, 1, 2.0)], (2, 2))
+    empty = sp.csc_matrix((2, 2))
+    _, _, c = oracle.equilibrate(empty, A, 0, 2, [])
+    assert c == 1.0
+    zeros = sp.csc_matrix((np.zeros(2), (np.arange(2), np.arange(2))), shape=(2, 2))
+    assert zeros.nnz == 2
+    _, _, c = oracle.equilibrate(zeros, A, 0, 2, [])
+    assert c == 1.0
+    tiny = _sp([(0, 0, 1e-12), (1, 1, 1e-12)], (2, 2))
+    _, _, c = oracle.equilibrate(tiny, A, 0, 2, [])
+    assert c == 1e4
+    unit = _sp([(0, 0, 1e-3), (1, 1, 1e-3)], (2, 2))
+    d, _, c = oracle.equilibrate(unit, A, 0, 2, [], equilibrate_max_scale=2.0)
+    assert c == 2.0
+
+
 # ----------------------------------------------------------------------------
 # KKT: test/kkt/test_kkt_system.cpp (dense FP64 reference of the same matrix)
 # ----------------------------------------------------------------------------
@@ -392,3 +434,37 @@ def test_regularized_backend_matches_default(oracle):
         c = PR.kkt_certificate(p, r["x"], r["s"], r["z"])
         assert c["stationarity"] < 1e-6 and c["feasibility"] < 1e-6
     np.testing.assert_allclose(r1["x"], r2["x"], rtol=1e-4, atol=1e-4)
+
+
+# ----------------------------------------------------------------------------
+# the same oracle in extended precision (liboracle_xp.so), the tests' source of
+# optima: pinned by the reference's known answers, by agreement with the FP64
+# oracle, and by its own optimality certificate at a tight tolerance
+# ----------------------------------------------------------------------------
+@pytest.mark.parametrize("case", [c for c in PR.kat_problems() if c[3] is not None], ids=lambda c: c[0])
+def test_extended_precision_oracle_known_answers(oracle, case):
+    name, p, status, x, obj, tol = case
+    r = oracle.solve_xp(p.P, p.q, p.A, p.b, p.zero_dim, p.nonneg_dim, p.soc_dims)
+    assert r["x"].dtype == np.longdouble
+    assert r["status"] == status
+    np.testing.assert_allclose(np.asarray(r["x"], dtype=np.float64), x, atol=tol)
+    if obj is not None:
+        assert abs(float(r["objective"]) - obj) < tol
+
+
+def test_extended_precision_oracle_tight_optimum(oracle):
+    """config 0: at tolerance 1e-11 the extended-precision run is Solved with a
+    certificate three decades below the FP64 default, and the FP64 oracle's
+    default-tolerance solution lies within its own 1e-5 accuracy of it."""
+    p = PR.config0(seed=0)
+    r = oracle.solve_xp(p.P, p.q, p.A, p.b, p.zero_dim, p.nonneg_dim, p.soc_dims, tol_feas=1e-11,
+                        tol_gap_abs=1e-11, tol_gap_rel=1e-11, refine_tol=1e-18, warm_start=0)
+    assert r["status"] == 1
+    x, s, z = (np.asarray(r[k], dtype=np.float64) for k in ("x", "s", "z"))
+    c = PR.kkt_certificate(p, x, s, z)
+    scale = 1.0 + np.abs(p.q).max() + np.abs(p.b).max()
+    assert c["stationarity"] < 1e-9 * scale and c["feasibility"] < 1e-9 * scale, c
+    assert c["complementarity"] < 1e-9 and c["margin_s"] > -1e-12 and c["margin_z"] > -1e-12, c
+    f = oracle.solve(p.P, p.q, p.A, p.b, p.zero_dim, p.nonneg_dim, p.soc_dims)
+    assert f["status"] == 1
+    assert np.abs(f["x"] - x).max() / max(1.0, np.abs(x).max()) < 1e-5
