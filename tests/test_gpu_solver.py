@@ -15,10 +15,12 @@ perturbed copies) and requires:
     spread = the largest difference among those runs;
   * where that fails (the oracle runs cluster on one side of the tolerance
     ball while the GPU run stops elsewhere inside it), the GPU solution must
-    be as close to the problem's optimum as the oracle's runs are: within
-    max(2e-7, 2 * the oracle runs' largest distance) of x*, s*, z* and the
-    objective of the extended-precision oracle solved to 1e-11 (x* itself is
-    known to ~1e-7 there);
+    be as close to the problem's optimum as the reference method gets: within
+    2x the largest distance of the oracle's runs, or of the extended-precision
+    oracle at the default tolerance (the method without FP64 rounding), from
+    x*, s*, z* and the objective of the extended-precision oracle solved to
+    1e-11 (1e-10 or 1e-9 where the tighter runs stall; x* is known to ~1e-7
+    there), with a floor of 2e-7;
   * for Solved results, the KKT optimality certificate of the GPU solution.
 """
 import numpy as np
@@ -61,15 +63,23 @@ def _errs(sol, r):
     return e
 
 
+def _xp(oracle, prob, **st):
+    r = oracle.solve_xp(prob.P, prob.q, prob.A, prob.b, prob.zero_dim, prob.nonneg_dim, prob.soc_dims,
+                        warm_start=0, **st)
+    if r is None or r["status"] != 1:
+        return None
+    return {k: np.asarray(r[k], dtype=np.float64) for k in ("x", "s", "z")} | {"objective": float(r["objective"])}
+
+
 def optimum_xp(oracle, prob):
-    """x*, s*, z*, objective of the extended-precision oracle (tolerances 1e-11,
-    1e-10 if that stalls); None if neither converges."""
-    for tol in (1e-11, 1e-10):
-        r = oracle.solve_xp(prob.P, prob.q, prob.A, prob.b, prob.zero_dim, prob.nonneg_dim, prob.soc_dims,
-                            tol_feas=tol, tol_gap_abs=tol, tol_gap_rel=tol, refine_tol=1e-18, warm_start=0)
-        if r is not None and r["status"] == 1:
-            return {k: np.asarray(r[k], dtype=np.float64) for k in ("x", "s", "z")} | \
-                {"objective": float(r["objective"]), "tol": tol}
+    """x*, s*, z*, objective of the extended-precision oracle at the tightest
+    tolerance (1e-11, 1e-10, 1e-9) it converges to, with the default settings
+    or with more refinement or smaller static regularization; None if none."""
+    for tol in (1e-11, 1e-10, 1e-9):
+        for extra in ({}, {"refine_max_iter": 10}, {"static_reg_P": 1e-12, "static_reg_A": 1e-12}):
+            r = _xp(oracle, prob, tol_feas=tol, tol_gap_abs=tol, tol_gap_rel=tol, refine_tol=1e-18, **extra)
+            if r is not None:
+                return r | {"tol": tol}
     return None
 
 
@@ -97,10 +107,15 @@ def assert_parity(sol, runs, prob=None):
         assert opt is not None, "the extended-precision oracle did not converge"
         e_gpu = _errs(sol, opt)
         e_orc = {k: max(_errs(r, opt)[k] for r in same) for k in e_gpu}
+        # where the method itself stops at the default tolerance without FP64 rounding
+        dflt = _xp(oracle_py, prob)
+        e_dflt = _errs(dflt, opt) if dflt is not None else {k: 0.0 for k in e_gpu}
         print("  vs optimum (xp tol", opt["tol"], "): gpu", {k: f"{v:.1e}" for k, v in e_gpu.items()},
-              "oracle runs", {k: f"{v:.1e}" for k, v in e_orc.items()})
+              "oracle runs", {k: f"{v:.1e}" for k, v in e_orc.items()},
+              "xp at default tol", {k: f"{v:.1e}" for k, v in e_dflt.items()},
+              "gpu vs xp at default tol", {k: f"{v:.1e}" for k, v in _errs(sol, dflt).items()} if dflt else None)
         for k in e_gpu:
-            assert e_gpu[k] <= max(2e-7, 2.0 * e_orc[k]), (k, e_gpu[k], e_orc[k])
+            assert e_gpu[k] <= max(2e-7, 2.0 * e_orc[k], 2.0 * e_dflt[k]), (k, e_gpu[k], e_orc[k], e_dflt[k])
     if prob is not None and sol.status == Status.Solved:
         c = PR.kkt_certificate(prob, sol.x, sol.s, sol.z)
         scale = 1.0 + np.abs(prob.q).max() + np.abs(prob.b).max()
