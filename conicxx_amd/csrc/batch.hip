@@ -383,126 +383,157 @@ __device__ inline void wave_sync() {
   __builtin_amdgcn_wave_barrier();
 }
 
+typedef double v4d __attribute__((ext_vector_type(4)));
+
 constexpr int PB = 32;  // panel width of the blocked factorization
 constexpr int JC = 64;  // trailing columns staged per chunk
 
-// Blocked right-looking LDL^T of M (packed lower).  Within each 32-column
-// panel the columns are factored left-looking (per column: one dot product per
-// row over the panel's earlier columns, then the pivot), in LDS: when M itself
-// is in global memory the panel is first staged into the LDS region `stage`
-// (row-major, stride 33) and written back after.  The trailing matrix is then
+// Blocked right-looking LDL^T of M (packed lower).  Each 32-column panel is
+// factored in LDS (when M itself is in global memory the panel is first staged
+// into the LDS region `stage`, row-major, stride 33, and written back after):
+// its diagonal block by one wave in registers, the rows below by forward
+// substitution against that block, a row per thread; the trailing update
+// runs on FP64 MFMA tiles.  The trailing matrix is then
 // updated in 64-column chunks: W = D * L(chunk rows, panel) is staged in LDS
 // and every thread owns a row, holding its 32 panel entries in registers and
 // updating 8 entries per batch of independent loads.  Pivot rule of the
 // reference's backends.  Returns acceptance as in try_factor.
 constexpr int PS = PB + 1;  // row stride of a staged panel
 
-__device__ bool factor_M(const BatchArgs& A, const PV& p, double* wbuf, double* pk, double* stage,
-                         long stage_cap, double* red) {
+template <bool STAGED>
+__device__ bool factor_M_t(const BatchArgs& A, const PV& p, double* wbuf, double* pk,
+                                                     double* stage, double* red) {
   const int nr = p.nr;
   double* M = p.M;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   int zero_piv = 0, nreg = 0;
   for (int k0 = 0; k0 < nr; k0 += PB) {
     const int kb = min(PB, nr - k0);
-    const bool staged = stage != nullptr && (long)(nr - k0) * PS <= stage_cap;
+    constexpr bool staged = STAGED;
     // L(i, k0 + c) for i >= k0 + c (address arithmetic: an indexed local
     // array would be placed in scratch memory)
     auto L = [&](int i, int c) -> double& {
-      return staged ? stage[(long)(i - k0) * PS + c] : M[colbase(k0 + c, nr) + (i - k0 - c)];
+      if constexpr (STAGED)
+        return stage[(i - k0) * PS + c];
+      else
+        return M[colbase(k0 + c, nr) + (i - k0 - c)];
     };
-    if (staged) {
+    if constexpr (staged) {
       for (int c = 0; c < kb; ++c) {
         const long cb = colbase(k0 + c, nr);
-        for (int i = k0 + c + tid; i < nr; i += BG) stage[(long)(i - k0) * PS + c] = M[cb + (i - k0 - c)];
+        for (int i = k0 + c + tid; i < nr; i += BG) stage[(i - k0) * PS + c] = M[cb + (i - k0 - c)];
       }
       __syncthreads();
     }
-    // panel columns left-looking: column c takes the updates of panel columns
-    // 0..c-1 as one dot product per row (independent loads; pivots from
-    // pk[0..PB)), the updated diagonal goes through pk[PB]; then every thread
-    // applies the pivot rule to the same value and scales its rows.  Two
-    // barriers per column.
-    for (int c = 0; c < kb; ++c) {
-      const int col = k0 + c;
-      double d;
-      if (c > 0) {
-        for (int i = col + tid; i < nr; i += BG) {
-          double acc = 0.0;
-#pragma unroll 8
-          for (int q = 0; q < c; ++q) acc += L(i, q) * (L(col, q) * pk[q]);
-          const double v = L(i, c) - acc;
-          L(i, c) = v;
-          if (i == col) pk[PB] = v;
+    // panel: wave 0 factors the kb x kb diagonal block in registers (lane i
+    // holds row i; each pivot is a broadcast and a rank-one update of the
+    // rows, no barrier) and publishes D and W(c, q) = d_q L(c, q) in LDS;
+    // then every thread solves its own rows below the block against it
+    // (L(i, c) = (a_ic - sum_q L(i, q) W(c, q)) / d_c, the row in
+    // registers).  Two barriers per panel.
+    if (wv == 0) {
+      double r[PB];
+#pragma unroll
+      for (int c = 0; c < PB; ++c) {
+        const bool ok = c <= lane && lane < kb;
+        r[c] = ok ? L(k0 + (ok ? lane : c), c) : 0.0;
+      }
+#pragma unroll
+      for (int j = 0; j < PB; ++j) {
+        if (j < kb) {
+          double d = bcast(r[j], j);
+          const double sg = k0 + j < p.n ? 1.0 : -1.0;
+          bool reg = false;
+          if (A.mode == PV_REGULARIZE && !(d * sg >= A.dyn_eps)) {  // regularized_ldlt.cpp:132-139
+            d += sg * A.dyn_delta;
+            if (d * sg < A.dyn_eps) d = sg * A.dyn_delta;
+            reg = true;
+          }
+          if (tid == 0) {
+            nreg += reg ? 1 : 0;
+            if (A.mode != PV_REGULARIZE && d == 0.0) zero_piv = 1;
+          }
+          const double l = lane > j ? r[j] * (1.0 / d) : 0.0;
+          r[j] = lane > j ? l : (lane == j ? d : r[j]);
+          const double dl = d * l;
+#pragma unroll
+          for (int k = j + 1; k < PB; ++k) r[k] -= l * bcast(dl, k);
         }
-        __syncthreads();
-        d = pk[PB];
-      } else {
-        d = L(col, 0);
-        __syncthreads();
       }
-      const double sg = col < p.n ? 1.0 : -1.0;
-      bool reg = false;
-      if (A.mode == PV_REGULARIZE) {
-        if (!(d * sg >= A.dyn_eps)) {  // regularized_ldlt.cpp:132-139
-          d += sg * A.dyn_delta;
-          if (d * sg < A.dyn_eps) d = sg * A.dyn_delta;
-          reg = true;
-        }
+#pragma unroll
+      for (int c = 0; c < PB; ++c) {
+        const double dc = bcast(r[c], c);
+        if (c <= lane && lane < kb) L(k0 + lane, c) = r[c];
+        if (c < lane && lane < kb) wbuf[lane * PB + c] = dc * r[c];
+        if (lane == c && c < kb) pk[c] = dc;
       }
-      const double dinv = 1.0 / d;
-      for (int i = col + 1 + tid; i < nr; i += BG) L(i, c) *= dinv;
-      if (tid == 0) {
-        nreg += reg ? 1 : 0;
-        if (A.mode != PV_REGULARIZE && d == 0.0) zero_piv = 1;
-        L(col, c) = d;
-        pk[c] = d;
-      }
-      __syncthreads();
     }
-    if (staged) {
+    __syncthreads();
+    for (int i = k0 + kb + tid; i < nr; i += BG) {
+      // keeps the block's W in LDS instead of hoisting all of it into registers
+      __asm__ volatile("" ::: "memory");
+      double a[PB];
+#pragma unroll
+      for (int c = 0; c < PB; ++c) a[c] = L(i, c < kb ? c : 0) * (c < kb ? 1.0 : 0.0);
+#pragma unroll
+      for (int c = 0; c < PB; ++c) {
+        if (c < kb) {
+          double acc = a[c];
+#pragma unroll
+          for (int q = 0; q < c; ++q) acc -= a[q] * wbuf[c * PB + q];
+          a[c] = acc * (1.0 / pk[c]);
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < PB; ++c)
+        if (c < kb) L(i, c) = a[c];
+    }
+    __syncthreads();
+    if constexpr (staged) {
       for (int c = 0; c < kb; ++c) {
         const long cb = colbase(k0 + c, nr);
-        for (int i = k0 + c + tid; i < nr; i += BG) M[cb + (i - k0 - c)] = stage[(long)(i - k0) * PS + c];
+        for (int i = k0 + c + tid; i < nr; i += BG) M[cb + (i - k0 - c)] = stage[(i - k0) * PS + c];
       }
       __syncthreads();
     }
+    // trailing update C -= L W', W(j, c) = L(j, c) d_c, on v_mfma_f64_16x16x4f64:
+    // 16 x 16 tiles; in each 64-column chunk wave w owns the tile column
+    // [jc0 + 16w, jc0 + 16w + 16) from its diagonal tile down to the last row.
+    // Lane l feeds A(row l & 15, k l >> 4) and B(k l >> 4, column l & 15)
+    // and holds C(rows (l >> 4) + 4r, column l & 15).
     for (int jc0 = k0 + kb; jc0 < nr; jc0 += JC) {
-      const int jn = min(JC, nr - jc0);
-      for (int e = tid; e < JC * PB; e += BG) {
-        const int jj = e / PB, c = e % PB;
-        double w = 0.0;
-        if (jj < jn && c < kb) w = L(jc0 + jj, c) * L(k0 + c, c);
-        wbuf[e] = w;
-      }
-      __syncthreads();
-      for (int i = jc0 + tid; i < nr; i += BG) {
-        double lr[PB];
+      const int tj = jc0 + 16 * wv;
+      if (tj < nr) {
+        const int jl = tj + (lane & 15), kq = lane >> 4;
+        double bf[PB / 4];
 #pragma unroll
-        for (int c = 0; c < PB; ++c) lr[c] = L(i, c < kb ? c : 0) * (c < kb ? 1.0 : 0.0);
-        const int jmax = min(jn, i - jc0 + 1);
-        for (int j0 = 0; j0 < jmax; j0 += 8) {
-          long adr[8];
-          double old[8];
+        for (int k = 0; k < PB / 4; ++k) {
+          const int c = 4 * k + kq;
+          const bool ok = c < kb && jl < nr;
+          bf[k] = ok ? L(ok ? jl : k0 + c, c) * pk[c] : 0.0;
+        }
+        for (int ti = tj; ti < nr; ti += 16) {
+          const int il = ti + (lane & 15);
+          v4d acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-          for (int q = 0; q < 8; ++q) {
-            const int jj = j0 + q < jmax ? j0 + q : j0;
-            adr[q] = colbase(jc0 + jj, nr) + (i - jc0 - jj);
-            old[q] = M[adr[q]];
+          for (int k = 0; k < PB / 4; ++k) {
+            const int c = 4 * k + kq;
+            const bool ok = c < kb && il < nr;
+            const double av = ok ? L(ok ? il : k0 + c, c) : 0.0;
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bf[k], acc, 0, 0, 0);
           }
+          if (jl < nr) {
+            const long cb = colbase(jl, nr);
 #pragma unroll
-          for (int q = 0; q < 8; ++q) {
-            if (j0 + q >= jmax) continue;
-            const double* w = wbuf + (j0 + q) * PB;
-            double acc = 0.0;
-#pragma unroll
-            for (int c = 0; c < PB; ++c) acc += lr[c] * w[c];
-            M[adr[q]] = old[q] - acc;
+            for (int r = 0; r < 4; ++r) {
+              const int i = ti + kq + 4 * r;
+              if (i >= jl && i < nr) M[cb + (i - jl)] -= acc[r];
+            }
           }
         }
       }
-      __syncthreads();
     }
+    __syncthreads();
   }
   double mind = INFINITY, nanf = 0.0;
   for (int k = tid; k < nr; k += BG) {
@@ -518,6 +549,14 @@ __device__ bool factor_M(const BatchArgs& A, const PV& p, double* wbuf, double* 
   if (A.mode == PV_CHECK && zr[0] > 0) return false;
   if (A.mode == PV_REGULARIZE && zr[1] > 0.05 * (double)p.N) return false;
   return isfinite(mind) && mind > A.dyn_eps;
+}
+
+// Every panel staged in LDS when the whole first panel fits the stage (the
+// usual case when M lives in global memory), else direct access to M.
+__device__ bool factor_M(const BatchArgs& A, const PV& p, double* wbuf, double* pk, double* stage, long stage_cap,
+                         double* red) {
+  if (stage != nullptr && (long)p.nr * PS <= stage_cap) return factor_M_t<true>(A, p, wbuf, pk, stage, red);
+  return factor_M_t<false>(A, p, wbuf, pk, stage, red);
 }
 
 __device__ bool try_factor(const BatchArgs& A, PV& p, double ep, double ea, double* colk, double* red) {
