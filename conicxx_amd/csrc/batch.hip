@@ -125,6 +125,7 @@ struct PV {
   double *s, *z, *rz, *z1, *dza, *dsa, *dz, *ds, *dsk, *wm, *wm2;
   double *rhs, *sol, *res, *tmp, *dsol;
   double* M;
+  bool m_lds, v_lds;  // M / the work vectors are in LDS (else in global memory)
   ConeDev C;
   // scalar state
   double tau, kappa, rtau, dqx, dbz, dsz, dxPx, nrx2, nrz2;
@@ -384,27 +385,28 @@ __device__ inline void wave_sync() {
 }
 
 typedef double v4d __attribute__((ext_vector_type(4)));
+// address-space qualified views of M and the work vectors: LDS accesses then
+// compile to ds_read/ds_write and global ones to global_load/global_store with
+// a scalar base, instead of flat accesses that wait on both memory counters
+typedef __attribute__((address_space(3))) double ldsd;
+typedef __attribute__((address_space(1))) double glbd;
 
 constexpr int PB = 32;  // panel width of the blocked factorization
-constexpr int JC = 64;  // trailing columns staged per chunk
+constexpr int JC = 64;  // trailing columns per chunk (16 per wave)
 
 // Blocked right-looking LDL^T of M (packed lower).  Each 32-column panel is
 // factored in LDS (when M itself is in global memory the panel is first staged
-// into the LDS region `stage`, row-major, stride 33, and written back after):
-// its diagonal block by one wave in registers, the rows below by forward
-// substitution against that block, a row per thread; the trailing update
-// runs on FP64 MFMA tiles.  The trailing matrix is then
-// updated in 64-column chunks: W = D * L(chunk rows, panel) is staged in LDS
-// and every thread owns a row, holding its 32 panel entries in registers and
-// updating 8 entries per batch of independent loads.  Pivot rule of the
+// into the LDS region `stage`, row-major, stride 33, and its final values are
+// written to M as they are produced): its diagonal block by one wave in
+// registers, the rows below by forward substitution against that block, a row
+// per thread.  The trailing matrix is then updated on FP64 MFMA 16 x 16 tiles,
+// in 64-column chunks of which each wave owns 16 columns.  Pivot rule of the
 // reference's backends.  Returns acceptance as in try_factor.
 constexpr int PS = PB + 1;  // row stride of a staged panel
 
-template <bool STAGED>
-__device__ bool factor_M_t(const BatchArgs& A, const PV& p, double* wbuf, double* pk,
-                                                     double* stage, double* red) {
+template <typename MT, bool STAGED>
+__device__ bool factor_M_t(const BatchArgs& A, const PV& p, MT* M, ldsd* wbuf, ldsd* pk, ldsd* stage, double* red) {
   const int nr = p.nr;
-  double* M = p.M;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   int zero_piv = 0, nreg = 0;
   for (int k0 = 0; k0 < nr; k0 += PB) {
@@ -412,13 +414,14 @@ __device__ bool factor_M_t(const BatchArgs& A, const PV& p, double* wbuf, double
     constexpr bool staged = STAGED;
     // L(i, k0 + c) for i >= k0 + c (address arithmetic: an indexed local
     // array would be placed in scratch memory)
-    auto L = [&](int i, int c) -> double& {
+    auto L = [&](int i, int c) -> decltype(auto) {
       if constexpr (STAGED)
-        return stage[(i - k0) * PS + c];
+        return (stage[(i - k0) * PS + c]);
       else
-        return M[colbase(k0 + c, nr) + (i - k0 - c)];
+        return (M[colbase(k0 + c, nr) + (i - k0 - c)]);
     };
     if constexpr (staged) {
+#pragma unroll 8
       for (int c = 0; c < kb; ++c) {
         const long cb = colbase(k0 + c, nr);
         for (int i = k0 + c + tid; i < nr; i += BG) stage[(i - k0) * PS + c] = M[cb + (i - k0 - c)];
@@ -436,7 +439,8 @@ __device__ bool factor_M_t(const BatchArgs& A, const PV& p, double* wbuf, double
 #pragma unroll
       for (int c = 0; c < PB; ++c) {
         const bool ok = c <= lane && lane < kb;
-        r[c] = ok ? L(k0 + (ok ? lane : c), c) : 0.0;
+        const double v = L(k0 + (ok ? lane : 0), ok ? c : 0);
+        r[c] = ok ? v : 0.0;
       }
 #pragma unroll
       for (int j = 0; j < PB; ++j) {
@@ -463,7 +467,10 @@ __device__ bool factor_M_t(const BatchArgs& A, const PV& p, double* wbuf, double
 #pragma unroll
       for (int c = 0; c < PB; ++c) {
         const double dc = bcast(r[c], c);
-        if (c <= lane && lane < kb) L(k0 + lane, c) = r[c];
+        if (c <= lane && lane < kb) {
+          L(k0 + lane, c) = r[c];
+          if constexpr (STAGED) M[colbase(k0 + c, nr) + (lane - c)] = r[c];
+        }
         if (c < lane && lane < kb) wbuf[lane * PB + c] = dc * r[c];
         if (lane == c && c < kb) pk[c] = dc;
       }
@@ -486,16 +493,12 @@ __device__ bool factor_M_t(const BatchArgs& A, const PV& p, double* wbuf, double
       }
 #pragma unroll
       for (int c = 0; c < PB; ++c)
-        if (c < kb) L(i, c) = a[c];
+        if (c < kb) {
+          L(i, c) = a[c];
+          if constexpr (STAGED) M[colbase(k0 + c, nr) + (i - k0 - c)] = a[c];
+        }
     }
     __syncthreads();
-    if constexpr (staged) {
-      for (int c = 0; c < kb; ++c) {
-        const long cb = colbase(k0 + c, nr);
-        for (int i = k0 + c + tid; i < nr; i += BG) M[cb + (i - k0 - c)] = stage[(i - k0) * PS + c];
-      }
-      __syncthreads();
-    }
     // trailing update C -= L W', W(j, c) = L(j, c) d_c, on v_mfma_f64_16x16x4f64:
     // 16 x 16 tiles; in each 64-column chunk wave w owns the tile column
     // [jc0 + 16w, jc0 + 16w + 16) from its diagonal tile down to the last row.
@@ -510,25 +513,53 @@ __device__ bool factor_M_t(const BatchArgs& A, const PV& p, double* wbuf, double
         for (int k = 0; k < PB / 4; ++k) {
           const int c = 4 * k + kq;
           const bool ok = c < kb && jl < nr;
-          bf[k] = ok ? L(ok ? jl : k0 + c, c) * pk[c] : 0.0;
+          const double v = L(ok ? jl : k0, ok ? c : 0), d = pk[ok ? c : 0];
+          bf[k] = ok ? v * d : 0.0;
         }
-        for (int ti = tj; ti < nr; ti += 16) {
+        // the tile's entries of M are read one tile ahead (the tiles are
+        // disjoint, so the read may precede the previous tile's store)
+        const long cb = colbase(jl < nr ? jl : tj, nr);
+        auto in_tile = [&](int i) { return jl < nr && i >= jl && i < nr; };
+        double mo[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int i = tj + kq + 4 * r;
+          const double v = M[cb + (in_tile(i) ? i - jl : 0)];
+          mo[r] = in_tile(i) ? v : 0.0;
+        }
+        // A operands are likewise loaded one tile ahead
+        auto load_a = [&](int ti, double (&av)[PB / 4]) {
           const int il = ti + (lane & 15);
-          v4d acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
           for (int k = 0; k < PB / 4; ++k) {
             const int c = 4 * k + kq;
             const bool ok = c < kb && il < nr;
-            const double av = ok ? L(ok ? il : k0 + c, c) : 0.0;
-            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bf[k], acc, 0, 0, 0);
+            const double v = L(ok ? il : k0, ok ? c : 0);
+            av[k] = ok ? v : 0.0;
           }
-          if (jl < nr) {
-            const long cb = colbase(jl, nr);
+        };
+        double an[PB / 4];
+        load_a(tj, an);
+        for (int ti = tj; ti < nr; ti += 16) {
+          double av[PB / 4];
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-              const int i = ti + kq + 4 * r;
-              if (i >= jl && i < nr) M[cb + (i - jl)] -= acc[r];
-            }
+          for (int k = 0; k < PB / 4; ++k) av[k] = an[k];
+          double mn[4];
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int i = ti + 16 + kq + 4 * r;
+            const double v = M[cb + (in_tile(i) ? i - jl : 0)];
+            mn[r] = in_tile(i) ? v : 0.0;
+          }
+          load_a(ti + 16, an);
+          v4d acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+          for (int k = 0; k < PB / 4; ++k) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[k], bf[k], acc, 0, 0, 0);
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int i = ti + kq + 4 * r;
+            if (in_tile(i)) M[cb + (i - jl)] = mo[r] - acc[r];
+            mo[r] = mn[r];
           }
         }
       }
@@ -555,8 +586,11 @@ __device__ bool factor_M_t(const BatchArgs& A, const PV& p, double* wbuf, double
 // usual case when M lives in global memory), else direct access to M.
 __device__ bool factor_M(const BatchArgs& A, const PV& p, double* wbuf, double* pk, double* stage, long stage_cap,
                          double* red) {
-  if (stage != nullptr && (long)p.nr * PS <= stage_cap) return factor_M_t<true>(A, p, wbuf, pk, stage, red);
-  return factor_M_t<false>(A, p, wbuf, pk, stage, red);
+  ldsd *w = (ldsd*)wbuf, *d = (ldsd*)pk;
+  if (p.m_lds) return factor_M_t<ldsd, false>(A, p, (ldsd*)p.M, w, d, nullptr, red);
+  if (stage != nullptr && (long)p.nr * PS <= stage_cap)
+    return factor_M_t<glbd, true>(A, p, (glbd*)p.M, w, d, (ldsd*)stage, red);
+  return factor_M_t<glbd, false>(A, p, (glbd*)p.M, w, d, nullptr, red);
 }
 
 __device__ bool try_factor(const BatchArgs& A, PV& p, double ep, double ea, double* colk, double* red) {
@@ -585,18 +619,23 @@ __device__ bool factor_retry(const BatchArgs& A, PV& p, double* colk, double* re
 
 // x = K^-1 rhs through the factored M (no refinement).  Bound rows:
 // y_i = (a_i x_j - r2_i) / (hs_i + da) after the solve of M.
-__device__ void reduced_solve_impl(const BatchArgs& A, PV& p, const double* rhs, double* x);
+template <typename MT, typename VT>
+__device__ void reduced_solve_t(const BatchArgs& A, const PV& p, const double* rhs, double* x, const MT* M, VT* t);
 __device__ void reduced_solve(const BatchArgs& A, PV& p, const double* rhs, double* x) {
   ++p.nsolve;
   const long long t0 = wall_clock64();
-  reduced_solve_impl(A, p, rhs, x);
+  if (p.m_lds && p.v_lds)
+    reduced_solve_t(A, p, rhs, x, (const ldsd*)p.M, (ldsd*)p.tmp);
+  else if (p.v_lds)
+    reduced_solve_t(A, p, rhs, x, (const glbd*)p.M, (ldsd*)p.tmp);
+  else
+    reduced_solve_t(A, p, rhs, x, (const double*)p.M, p.tmp);
   p.t_solve += wall_clock64() - t0;
 }
-__device__ void reduced_solve_impl(const BatchArgs& A, PV& p, const double* rhs, double* x) {
+template <typename MT, typename VT>
+__device__ void reduced_solve_t(const BatchArgs& A, const PV& p, const double* rhs, double* x, const MT* M, VT* t) {
   const int n = p.n, nr = p.nr, tid = threadIdx.x;
-  const double* M = p.M;
   const double da = p.cur_a;
-  double* t = p.tmp;
   for (int j = tid; j < n; j += BG) {
     double acc = rhs[j];
     for (int e = A.ac_ptr[p.acp + j]; e < A.ac_ptr[p.acp + j + 1]; ++e) {
@@ -644,24 +683,34 @@ __device__ void reduced_solve_impl(const BatchArgs& A, PV& p, const double* rhs,
   for (int k = tid; k < nr; k += BG) t[k] /= M[colbase(k, nr)];
   __syncthreads();
   // L' x = y, blocks from the last: column dot products with the rows below
-  // (one wave per column), then wave 0 solves the diagonal block on registers
+  // (each wave owns 16 columns and keeps their 16 partial sums live, so every
+  // row step issues 16 independent coalesced loads), then wave 0 solves the
+  // diagonal block on registers
   for (int k0 = ((nr - 1) / 64) * 64; k0 >= 0; k0 -= 64) {
     const int kb = min(64, nr - k0);
-    for (int c = wv; c < kb; c += 4) {
-      const long cb = colbase(k0 + c, nr);
-      double acc = 0.0;
-      for (int i0 = k0 + kb; i0 < nr; i0 += 256) {
+    if (k0 + kb < nr) {
+      const int c0 = 16 * wv;
+      double acc[16];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int i = i0 + 64 * u + lane;
-          const bool ok = i < nr;
-          const double lv = M[ok ? cb + (i - k0 - c) : cb];
-          const double xv = t[ok ? i : 0];
-          acc += ok ? lv * xv : 0.0;
+      for (int u = 0; u < 16; ++u) acc[u] = 0.0;
+      for (int i = k0 + kb + lane; i < nr; i += 64) {
+        const double xv = t[i];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+          const int c = c0 + u;
+          const bool ok = c < kb;
+          const double lv = M[ok ? colbase(k0 + c, nr) + (i - k0 - c) : 0];
+          acc[u] += ok ? lv * xv : 0.0;
         }
       }
-      acc = wave_sum(acc);
-      if (lane == 0) t[k0 + c] -= acc;
+#pragma unroll
+      for (int u = 0; u < 16; ++u) acc[u] = wave_sum(acc[u]);
+      if (lane < 16 && c0 + lane < kb) {
+        double a = acc[0];
+#pragma unroll
+        for (int u = 1; u < 16; ++u) a = lane == u ? acc[u] : a;
+        t[k0 + c0 + lane] -= a;
+      }
     }
     __syncthreads();
     if (wv == 0) {
@@ -995,6 +1044,8 @@ __device__ void setup_view(const BatchArgs& A, const BatchDesc& D, PV& p, double
     p.stage = lds;
     p.stage_cap = cap;
   }
+  p.m_lds = p.M == lds;
+  p.v_lds = w == lds || w == lds + PM;
   p.x = w; w += n;
   p.Px = w; w += n;
   p.rx = w; w += n;
