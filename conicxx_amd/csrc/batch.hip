@@ -499,37 +499,46 @@ __device__ bool factor_M_t(const BatchArgs& A, const PV& p, MT* M, ldsd* wbuf, l
         }
     }
     __syncthreads();
-    // trailing update C -= L W', W(j, c) = L(j, c) d_c, on v_mfma_f64_16x16x4f64:
-    // 16 x 16 tiles; in each 64-column chunk wave w owns the tile column
-    // [jc0 + 16w, jc0 + 16w + 16) from its diagonal tile down to the last row.
-    // Lane l feeds A(row l & 15, k l >> 4) and B(k l >> 4, column l & 15)
-    // and holds C(rows (l >> 4) + 4r, column l & 15).
+    // trailing update C -= L W', W(j, c) = L(j, c) d_c, on v_mfma_f64_16x16x4f64
+    // computing transposed 16 x 16 tiles: in each 64-column chunk wave w owns
+    // the tile column [jc0 + 16w, jc0 + 16w + 16) from its diagonal tile down
+    // to the last row.  Lane l feeds A(row l & 15, k l >> 4) = W(tj + (l & 15), c)
+    // and B(k l >> 4, column l & 15) = L(ti + (l & 15), c), c = 4k + (l >> 4),
+    // and holds the updates of M(ti + (l & 15), tj + (l >> 4) + 4r): each of its
+    // four accesses of M covers 4 columns x 16 consecutive rows.
     for (int jc0 = k0 + kb; jc0 < nr; jc0 += JC) {
       const int tj = jc0 + 16 * wv;
       if (tj < nr) {
-        const int jl = tj + (lane & 15), kq = lane >> 4;
+        const int l15 = lane & 15, kq = lane >> 4;
         double bf[PB / 4];
 #pragma unroll
         for (int k = 0; k < PB / 4; ++k) {
-          const int c = 4 * k + kq;
+          const int c = 4 * k + kq, jl = tj + l15;
           const bool ok = c < kb && jl < nr;
           const double v = L(ok ? jl : k0, ok ? c : 0), d = pk[ok ? c : 0];
           bf[k] = ok ? v * d : 0.0;
         }
-        // the tile's entries of M are read one tile ahead (the tiles are
-        // disjoint, so the read may precede the previous tile's store)
-        const long cb = colbase(jl < nr ? jl : tj, nr);
-        auto in_tile = [&](int i) { return jl < nr && i >= jl && i < nr; };
-        double mo[4];
+        // M(i, jr[r]) = M[cb[r] + i]; outside the tile the access is clamped to
+        // the diagonal entry of a valid column and its value discarded
+        long cb[4];
+        int jr[4], jd[4];
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int i = tj + kq + 4 * r;
-          const double v = M[cb + (in_tile(i) ? i - jl : 0)];
-          mo[r] = in_tile(i) ? v : 0.0;
+          jr[r] = tj + kq + 4 * r;
+          jd[r] = min(jr[r], nr - 1);
+          cb[r] = colbase(jd[r], nr) - jd[r];
         }
-        // A operands are likewise loaded one tile ahead
+        auto load_m = [&](int ti, double (&mv)[4]) {
+          const int i = ti + l15;
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const bool ok = i >= jr[r] && i < nr;
+            const double v = M[cb[r] + (ok ? i : jd[r])];
+            mv[r] = ok ? v : 0.0;
+          }
+        };
         auto load_a = [&](int ti, double (&av)[PB / 4]) {
-          const int il = ti + (lane & 15);
+          const int il = ti + l15;
 #pragma unroll
           for (int k = 0; k < PB / 4; ++k) {
             const int c = 4 * k + kq;
@@ -538,28 +547,27 @@ __device__ bool factor_M_t(const BatchArgs& A, const PV& p, MT* M, ldsd* wbuf, l
             av[k] = ok ? v : 0.0;
           }
         };
-        double an[PB / 4];
+        // the tiles of a column are disjoint: M is read two tiles ahead of
+        // the stores and the A operands one tile ahead
+        double m0[4], m1[4], an[PB / 4];
+        load_m(tj, m0);
+        load_m(tj + 16, m1);
         load_a(tj, an);
         for (int ti = tj; ti < nr; ti += 16) {
-          double av[PB / 4];
+          double av[PB / 4], m2[4];
 #pragma unroll
           for (int k = 0; k < PB / 4; ++k) av[k] = an[k];
-          double mn[4];
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const int i = ti + 16 + kq + 4 * r;
-            const double v = M[cb + (in_tile(i) ? i - jl : 0)];
-            mn[r] = in_tile(i) ? v : 0.0;
-          }
+          load_m(ti + 32, m2);
           load_a(ti + 16, an);
           v4d acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-          for (int k = 0; k < PB / 4; ++k) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[k], bf[k], acc, 0, 0, 0);
+          for (int k = 0; k < PB / 4; ++k) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[k], av[k], acc, 0, 0, 0);
+          const int i = ti + l15;
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
-            const int i = ti + kq + 4 * r;
-            if (in_tile(i)) M[cb + (i - jl)] = mo[r] - acc[r];
-            mo[r] = mn[r];
+            if (i >= jr[r] && i < nr) M[cb[r] + i] = m0[r] - acc[r];
+            m0[r] = m1[r];
+            m1[r] = m2[r];
           }
         }
       }
@@ -624,10 +632,14 @@ __device__ void reduced_solve_t(const BatchArgs& A, const PV& p, const double* r
 __device__ void reduced_solve(const BatchArgs& A, PV& p, const double* rhs, double* x) {
   ++p.nsolve;
   const long long t0 = wall_clock64();
+  // the intermediate t lives in LDS: among the vectors, or else in the panel
+  // stage, which only the factorization uses
   if (p.m_lds && p.v_lds)
     reduced_solve_t(A, p, rhs, x, (const ldsd*)p.M, (ldsd*)p.tmp);
   else if (p.v_lds)
     reduced_solve_t(A, p, rhs, x, (const glbd*)p.M, (ldsd*)p.tmp);
+  else if (!p.m_lds && p.stage != nullptr && p.stage_cap >= p.N)
+    reduced_solve_t(A, p, rhs, x, (const glbd*)p.M, (ldsd*)p.stage);
   else
     reduced_solve_t(A, p, rhs, x, (const double*)p.M, p.tmp);
   p.t_solve += wall_clock64() - t0;
