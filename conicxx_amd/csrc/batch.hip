@@ -659,18 +659,21 @@ __device__ void reduced_solve_t(const BatchArgs& A, const PV& p, const double* r
   for (int kk = tid; kk < p.nk; kk += BG) t[n + kk] = rhs[n + A.kr[p.ko + kk]];
   __syncthreads();
   const int lane = tid & 63, wv = tid >> 6;
+  // 32-bit offsets (the host bounds the order): L(k0 + c + r, k0 + c) is
+  // M[cb0 + c * w - c (c + 1) / 2 + c + r], cb0 = colbase(k0), w = nr - k0
+  auto coff = [&](int cb0, int w, int c) { return cb0 + c * w - (c * (c + 1)) / 2; };
   // L y = t in 64-row blocks.  Wave 0 loads its diagonal block into registers
   // (one row per lane, independent loads) and runs the dependent 64-step
   // recurrence on registers only; then every thread updates one row below the
   // block with a fully unrolled, clamped-address dot product.
   for (int k0 = 0; k0 < nr; k0 += 64) {
-    const int kb = min(64, nr - k0);
+    const int kb = min(64, nr - k0), cb0 = (int)colbase(k0, nr), w = nr - k0;
     if (wv == 0) {
       double Lr[64];
 #pragma unroll
       for (int c = 0; c < 64; ++c) {
         const bool ok = c < lane && lane < kb;
-        const double v = M[ok ? colbase(k0 + c, nr) + (lane - c) : 0];
+        const double v = M[ok ? coff(cb0, w, c) + lane : 0];
         Lr[c] = ok ? v : 0.0;
       }
       double tl = lane < kb ? t[k0 + lane] : 0.0;
@@ -684,7 +687,7 @@ __device__ void reduced_solve_t(const BatchArgs& A, const PV& p, const double* r
 #pragma unroll 16
       for (int c = 0; c < 64; ++c) {
         const bool ok = c < kb;
-        const double lv = M[ok ? colbase(k0 + c, nr) + (i - k0 - c) : 0];
+        const double lv = M[ok ? coff(cb0, w, c) + (i - k0) : 0];
         const double yv = t[ok ? k0 + c : 0];
         acc += ok ? lv * yv : 0.0;
       }
@@ -699,7 +702,7 @@ __device__ void reduced_solve_t(const BatchArgs& A, const PV& p, const double* r
   // row step issues 16 independent coalesced loads), then wave 0 solves the
   // diagonal block on registers
   for (int k0 = ((nr - 1) / 64) * 64; k0 >= 0; k0 -= 64) {
-    const int kb = min(64, nr - k0);
+    const int kb = min(64, nr - k0), cb0 = (int)colbase(k0, nr), w = nr - k0;
     if (k0 + kb < nr) {
       const int c0 = 16 * wv;
       double acc[16];
@@ -711,7 +714,7 @@ __device__ void reduced_solve_t(const BatchArgs& A, const PV& p, const double* r
         for (int u = 0; u < 16; ++u) {
           const int c = c0 + u;
           const bool ok = c < kb;
-          const double lv = M[ok ? colbase(k0 + c, nr) + (i - k0 - c) : 0];
+          const double lv = M[ok ? coff(cb0, w, c) + (i - k0) : 0];
           acc[u] += ok ? lv * xv : 0.0;
         }
       }
@@ -727,10 +730,11 @@ __device__ void reduced_solve_t(const BatchArgs& A, const PV& p, const double* r
     __syncthreads();
     if (wv == 0) {
       double Lc[64];  // Lc[c] = L(k0 + c, k0 + lane) for c > lane
+      const int lb = coff(cb0, w, lane);  // M[lb + c] = L(k0 + c, k0 + lane)
 #pragma unroll
       for (int c = 0; c < 64; ++c) {
         const bool ok = c > lane && c < kb;
-        const double v = M[ok ? colbase(k0 + lane, nr) + (c - lane) : 0];
+        const double v = M[ok ? lb + c : 0];
         Lc[c] = ok ? v : 0.0;
       }
       double tl = lane < kb ? t[k0 + lane] : 0.0;
@@ -1539,6 +1543,8 @@ bool BatchSolver::setup(int nprob, const int* n, const int* m, const int* const*
       D.nk = nkept;
       D.nr = nk + nkept;
     }
+    // the device addresses a problem's packed factor with 32-bit offsets
+    if (D.nr > 65535) throw std::runtime_error("batch: reduced KKT order above 65535 (use the single-problem solver)");
     for (int t = 0; t < nsk; ++t) soc_off.push_back(so_k[t]), soc_dim.push_back(socd_[k][t]);
     // workspace: 9 n-vectors, 14 m-vectors, 5 N-vectors
     D.wofs = wofs;
