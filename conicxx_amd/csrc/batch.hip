@@ -70,6 +70,12 @@ struct BatchArgs {
   double *gP, *gq, *gA, *gb;
   int* counter2;
 };
+// The kernels read their arguments in place from the kernarg segment through
+// a constant-address-space reference: the loads are scalar, and the compiler
+// may hoist them past any store (a by-reference copy of the argument struct
+// would live in scratch memory and be reloaded after every store).
+using CArgs = const __attribute__((address_space(4))) BatchArgs;
+__device__ inline CArgs& kernel_args() { return *(CArgs*)__builtin_amdgcn_kernarg_segment_ptr(); }
 
 // ---------------------------------------------------------------------------
 // workgroup reductions (all threads receive the result)
@@ -137,7 +143,7 @@ __device__ inline long packed_size(int nr) { return (long)nr * (nr + 1) / 2; }
 __device__ inline long colbase(int j, int nr) { return (long)j * nr - (long)j * (j - 1) / 2; }
 
 // y = P x (full symmetric CSR)
-__device__ void pmul(const BatchArgs& A, const PV& p, const double* x, double* y) {
+__device__ void pmul(CArgs& A, const PV& p, const double* x, double* y) {
   for (int i = threadIdx.x; i < p.n; i += BG) {
     double acc = 0.0;
     for (int e = A.pf_ptr[p.pfp + i]; e < A.pf_ptr[p.pfp + i + 1]; ++e) acc += A.Pv[A.pf_src[e]] * x[A.pf_col[e]];
@@ -146,7 +152,7 @@ __device__ void pmul(const BatchArgs& A, const PV& p, const double* x, double* y
   __syncthreads();
 }
 // y = A x (all rows)
-__device__ void amul(const BatchArgs& A, const PV& p, const double* x, double* y) {
+__device__ void amul(CArgs& A, const PV& p, const double* x, double* y) {
   for (int i = threadIdx.x; i < p.m; i += BG) {
     double acc = 0.0;
     for (int e = A.ar_ptr[p.arp + i]; e < A.ar_ptr[p.arp + i + 1]; ++e) acc += A.Av[A.ar_src[e]] * x[A.ar_col[e]];
@@ -155,7 +161,7 @@ __device__ void amul(const BatchArgs& A, const PV& p, const double* x, double* y
   __syncthreads();
 }
 // y = A' z
-__device__ void atmul(const BatchArgs& A, const PV& p, const double* z, double* y) {
+__device__ void atmul(CArgs& A, const PV& p, const double* z, double* y) {
   for (int j = threadIdx.x; j < p.n; j += BG) {
     double acc = 0.0;
     for (int e = A.ac_ptr[p.acp + j]; e < A.ac_ptr[p.acp + j + 1]; ++e) acc += A.Av[e] * z[A.ac_row[e]];
@@ -320,7 +326,7 @@ __device__ void neg_hs_plus_stored(const PV& p, const double* x, const double* y
 // ---------------------------------------------------------------------------
 // KKT: M assembly (bound rows folded into the P block, every other row kept)
 // ---------------------------------------------------------------------------
-__device__ void assemble_M(const BatchArgs& A, const PV& p, double dp, double da) {
+__device__ void assemble_M(CArgs& A, const PV& p, double dp, double da) {
   const int nr = p.nr, n = p.n;
   double* M = p.M;
   const long tot = packed_size(nr);
@@ -393,6 +399,7 @@ typedef __attribute__((address_space(1))) double glbd;
 
 constexpr int PB = 32;  // panel width of the blocked factorization
 constexpr int JC = 64;  // trailing columns per chunk (16 per wave)
+constexpr int COLK = PB * JC + 2 * PB + 8;  // factorization scratch in LDS (doubles)
 
 // Blocked right-looking LDL^T of M (packed lower).  Each 32-column panel is
 // factored in LDS (when M itself is in global memory the panel is first staged
@@ -405,35 +412,45 @@ constexpr int JC = 64;  // trailing columns per chunk (16 per wave)
 constexpr int PS = PB + 1;  // row stride of a staged panel
 
 template <typename MT, bool STAGED>
-__device__ bool factor_M_t(const BatchArgs& A, const PV& p, MT* M, ldsd* wbuf, ldsd* pk, ldsd* stage, double* red) {
-  const int nr = p.nr;
+__device__ bool factor_M_t(CArgs& A, const PV& p, MT* M, ldsd* wbuf, ldsd* pk, ldsd* stage, double* red) {
+  // scalars in registers: A and p live in memory that every store in the
+  // loops below might alias, which would force a reload per use
+  const int nr = p.nr, pn = p.n, mode = A.mode;
+  const double dyn_eps = A.dyn_eps, dyn_delta = A.dyn_delta;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  // LDS scratch: Wt of the diagonal block (PB rows of stride 2 PB) in wbuf;
+  // the pivot reciprocals follow the pivots pk
+  static_assert(JC == 2 * PB, "Wt rows of the diagonal block fill wbuf");
+  ldsd* ipk = pk + PB;
   int zero_piv = 0, nreg = 0;
   for (int k0 = 0; k0 < nr; k0 += PB) {
     const int kb = min(PB, nr - k0);
     constexpr bool staged = STAGED;
+    // offset of L(i, k0 + c) in M (32-bit: the host bounds the order),
+    // colbase(k0 + c) - (k0 + c) + i incrementally from colbase(k0)
+    const int cb0 = (int)colbase(k0, nr), w = nr - k0;
+    auto off = [&](int i, int c) { return cb0 + c * w - (c * (c + 1)) / 2 + (i - k0); };
     // L(i, k0 + c) for i >= k0 + c (address arithmetic: an indexed local
     // array would be placed in scratch memory)
     auto L = [&](int i, int c) -> decltype(auto) {
       if constexpr (STAGED)
         return (stage[(i - k0) * PS + c]);
       else
-        return (M[colbase(k0 + c, nr) + (i - k0 - c)]);
+        return (M[off(i, c)]);
     };
     if constexpr (staged) {
 #pragma unroll 8
-      for (int c = 0; c < kb; ++c) {
-        const long cb = colbase(k0 + c, nr);
-        for (int i = k0 + c + tid; i < nr; i += BG) stage[(i - k0) * PS + c] = M[cb + (i - k0 - c)];
-      }
+      for (int c = 0; c < kb; ++c)
+        for (int i = k0 + c + tid; i < nr; i += BG) stage[(i - k0) * PS + c] = M[off(i, c)];
       __syncthreads();
     }
     // panel: wave 0 factors the kb x kb diagonal block in registers (lane i
-    // holds row i; each pivot is a broadcast and a rank-one update of the
-    // rows, no barrier) and publishes D and W(c, q) = d_q L(c, q) in LDS;
-    // then every thread solves its own rows below the block against it
-    // (L(i, c) = (a_ic - sum_q L(i, q) W(c, q)) / d_c, the row in
-    // registers).  Two barriers per panel.
+    // holds row i; at each pivot every lane publishes its entry of the scaled
+    // column d l in LDS, read back as broadcasts) and leaves D, 1/D and
+    // Wt(q, c) = d_q L(c, q) in LDS (rows of stride 2 PB); then every thread
+    // solves its own rows below the block against it
+    // (L(i, c) = (a_ic - sum_q L(i, q) W(c, q)) / d_c, the row in registers).
+    // Two barriers per panel.
     if (wv == 0) {
       double r[PB];
 #pragma unroll
@@ -446,33 +463,33 @@ __device__ bool factor_M_t(const BatchArgs& A, const PV& p, MT* M, ldsd* wbuf, l
       for (int j = 0; j < PB; ++j) {
         if (j < kb) {
           double d = bcast(r[j], j);
-          const double sg = k0 + j < p.n ? 1.0 : -1.0;
+          const double sg = k0 + j < pn ? 1.0 : -1.0;
           bool reg = false;
-          if (A.mode == PV_REGULARIZE && !(d * sg >= A.dyn_eps)) {  // regularized_ldlt.cpp:132-139
-            d += sg * A.dyn_delta;
-            if (d * sg < A.dyn_eps) d = sg * A.dyn_delta;
+          if (mode == PV_REGULARIZE && !(d * sg >= dyn_eps)) {  // regularized_ldlt.cpp:132-139
+            d += sg * dyn_delta;
+            if (d * sg < dyn_eps) d = sg * dyn_delta;
             reg = true;
           }
           if (tid == 0) {
             nreg += reg ? 1 : 0;
-            if (A.mode != PV_REGULARIZE && d == 0.0) zero_piv = 1;
+            if (mode != PV_REGULARIZE && d == 0.0) zero_piv = 1;
           }
-          const double l = lane > j ? r[j] * (1.0 / d) : 0.0;
+          const double inv = 1.0 / d;
+          const double l = lane > j ? r[j] * inv : 0.0;
           r[j] = lane > j ? l : (lane == j ? d : r[j]);
-          const double dl = d * l;
+          wbuf[j * 2 * PB + lane] = d * l;
+          if (lane == j) pk[j] = d, ipk[j] = inv;
+          wave_sync();
 #pragma unroll
-          for (int k = j + 1; k < PB; ++k) r[k] -= l * bcast(dl, k);
+          for (int k = j + 1; k < PB; ++k) r[k] -= l * wbuf[j * 2 * PB + k];
         }
       }
 #pragma unroll
       for (int c = 0; c < PB; ++c) {
-        const double dc = bcast(r[c], c);
         if (c <= lane && lane < kb) {
           L(k0 + lane, c) = r[c];
-          if constexpr (STAGED) M[colbase(k0 + c, nr) + (lane - c)] = r[c];
+          if constexpr (STAGED) M[off(k0 + lane, c)] = r[c];
         }
-        if (c < lane && lane < kb) wbuf[lane * PB + c] = dc * r[c];
-        if (lane == c && c < kb) pk[c] = dc;
       }
     }
     __syncthreads();
@@ -487,15 +504,15 @@ __device__ bool factor_M_t(const BatchArgs& A, const PV& p, MT* M, ldsd* wbuf, l
         if (c < kb) {
           double acc = a[c];
 #pragma unroll
-          for (int q = 0; q < c; ++q) acc -= a[q] * wbuf[c * PB + q];
-          a[c] = acc * (1.0 / pk[c]);
+          for (int q = 0; q < c; ++q) acc -= a[q] * wbuf[q * 2 * PB + c];
+          a[c] = acc * ipk[c];
         }
       }
 #pragma unroll
       for (int c = 0; c < PB; ++c)
         if (c < kb) {
           L(i, c) = a[c];
-          if constexpr (STAGED) M[colbase(k0 + c, nr) + (i - k0 - c)] = a[c];
+          if constexpr (STAGED) M[off(i, c)] = a[c];
         }
     }
     __syncthreads();
@@ -520,13 +537,12 @@ __device__ bool factor_M_t(const BatchArgs& A, const PV& p, MT* M, ldsd* wbuf, l
         }
         // M(i, jr[r]) = M[cb[r] + i]; outside the tile the access is clamped to
         // the diagonal entry of a valid column and its value discarded
-        long cb[4];
-        int jr[4], jd[4];
+        int cb[4], jr[4], jd[4];
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           jr[r] = tj + kq + 4 * r;
           jd[r] = min(jr[r], nr - 1);
-          cb[r] = colbase(jd[r], nr) - jd[r];
+          cb[r] = (int)colbase(jd[r], nr) - jd[r];
         }
         auto load_m = [&](int ti, double (&mv)[4]) {
           const int i = ti + l15;
@@ -592,7 +608,7 @@ __device__ bool factor_M_t(const BatchArgs& A, const PV& p, MT* M, ldsd* wbuf, l
 
 // Every panel staged in LDS when the whole first panel fits the stage (the
 // usual case when M lives in global memory), else direct access to M.
-__device__ bool factor_M(const BatchArgs& A, const PV& p, double* wbuf, double* pk, double* stage, long stage_cap,
+__device__ bool factor_M(CArgs& A, const PV& p, double* wbuf, double* pk, double* stage, long stage_cap,
                          double* red) {
   ldsd *w = (ldsd*)wbuf, *d = (ldsd*)pk;
   if (p.m_lds) return factor_M_t<ldsd, false>(A, p, (ldsd*)p.M, w, d, nullptr, red);
@@ -601,7 +617,7 @@ __device__ bool factor_M(const BatchArgs& A, const PV& p, double* wbuf, double* 
   return factor_M_t<glbd, false>(A, p, (glbd*)p.M, w, d, nullptr, red);
 }
 
-__device__ bool try_factor(const BatchArgs& A, PV& p, double ep, double ea, double* colk, double* red) {
+__device__ bool try_factor(CArgs& A, PV& p, double ep, double ea, double* colk, double* red) {
   p.cur_a = A.sreg_a + ea;
   ++p.nfact;
   const long long t0 = wall_clock64();
@@ -612,7 +628,7 @@ __device__ bool try_factor(const BatchArgs& A, PV& p, double ep, double ea, doub
 }
 
 // factorizeWithRetry (kkt_system.cpp:242-264)
-__device__ bool factor_retry(const BatchArgs& A, PV& p, double* colk, double* red) {
+__device__ bool factor_retry(CArgs& A, PV& p, double* colk, double* red) {
   double ep = 0.0, ea = 0.0;
   for (int att = 0; att < 18; ++att) {
     if (try_factor(A, p, ep, ea, colk, red)) {
@@ -628,8 +644,8 @@ __device__ bool factor_retry(const BatchArgs& A, PV& p, double* colk, double* re
 // x = K^-1 rhs through the factored M (no refinement).  Bound rows:
 // y_i = (a_i x_j - r2_i) / (hs_i + da) after the solve of M.
 template <typename MT, typename VT>
-__device__ void reduced_solve_t(const BatchArgs& A, const PV& p, const double* rhs, double* x, const MT* M, VT* t);
-__device__ void reduced_solve(const BatchArgs& A, PV& p, const double* rhs, double* x) {
+__device__ void reduced_solve_t(CArgs& A, const PV& p, const double* rhs, double* x, const MT* M, VT* t);
+__device__ void reduced_solve(CArgs& A, PV& p, const double* rhs, double* x) {
   ++p.nsolve;
   const long long t0 = wall_clock64();
   // the intermediate t lives in LDS: among the vectors, or else in the panel
@@ -645,18 +661,26 @@ __device__ void reduced_solve(const BatchArgs& A, PV& p, const double* rhs, doub
   p.t_solve += wall_clock64() - t0;
 }
 template <typename MT, typename VT>
-__device__ void reduced_solve_t(const BatchArgs& A, const PV& p, const double* rhs, double* x, const MT* M, VT* t) {
-  const int n = p.n, nr = p.nr, tid = threadIdx.x;
+__device__ void reduced_solve_t(CArgs& A, const PV& p, const double* rhs, double* x, const MT* M, VT* t) {
+  // scalars and pointers in registers (A and p live in memory that the stores
+  // below might alias, which would force a reload per use)
+  const int n = p.n, m = p.m, nr = p.nr, nk = p.nk, tid = threadIdx.x;
   const double da = p.cur_a;
+  const int* __restrict__ acp = A.ac_ptr + p.acp;
+  const int* __restrict__ acr = A.ac_row;
+  const int* __restrict__ rk = A.rk + p.co;
+  const int* __restrict__ kr = A.kr + p.ko;
+  const double* __restrict__ Av = A.Av;
+  const double* __restrict__ hsd = p.C.hsd;
   for (int j = tid; j < n; j += BG) {
     double acc = rhs[j];
-    for (int e = A.ac_ptr[p.acp + j]; e < A.ac_ptr[p.acp + j + 1]; ++e) {
-      const int i = A.ac_row[e];
-      if (A.rk[p.co + i] < 0) acc += A.Av[e] * rhs[n + i] / (p.C.hsd[i] + da);
+    for (int e = acp[j]; e < acp[j + 1]; ++e) {
+      const int i = acr[e];
+      if (rk[i] < 0) acc += Av[e] * rhs[n + i] / (hsd[i] + da);
     }
     t[j] = acc;
   }
-  for (int kk = tid; kk < p.nk; kk += BG) t[n + kk] = rhs[n + A.kr[p.ko + kk]];
+  for (int kk = tid; kk < nk; kk += BG) t[n + kk] = rhs[n + kr[kk]];
   __syncthreads();
   const int lane = tid & 63, wv = tid >> 6;
   // 32-bit offsets (the host bounds the order): L(k0 + c + r, k0 + c) is
@@ -744,12 +768,15 @@ __device__ void reduced_solve_t(const BatchArgs& A, const PV& p, const double* r
     }
     __syncthreads();
   }
+  const int* __restrict__ arp = A.ar_ptr + p.arp;
+  const int* __restrict__ ars = A.ar_src;
+  const int* __restrict__ arc = A.ar_col;
   for (int j = tid; j < n; j += BG) x[j] = t[j];
-  for (int kk = tid; kk < p.nk; kk += BG) x[n + A.kr[p.ko + kk]] = t[n + kk];
-  for (int i = tid; i < p.m; i += BG) {
-    if (A.rk[p.co + i] >= 0) continue;
-    const int e = A.ar_ptr[p.arp + i];
-    x[n + i] = (A.Av[A.ar_src[e]] * t[A.ar_col[e]] - rhs[n + i]) / (p.C.hsd[i] + da);
+  for (int kk = tid; kk < nk; kk += BG) x[n + kr[kk]] = t[n + kk];
+  for (int i = tid; i < m; i += BG) {
+    if (rk[i] >= 0) continue;
+    const int e = arp[i];
+    x[n + i] = (Av[ars[e]] * t[arc[e]] - rhs[n + i]) / (hsd[i] + da);
   }
   __syncthreads();
 }
@@ -786,7 +813,7 @@ __device__ void soc_tangent_apply(const PV& p, const double* x, double* out) {
 }
 
 // res = rhs - K x with K the statically regularized KKT matrix (the refinement target)
-__device__ void kkt_residual(const BatchArgs& A, PV& p, const double* rhs, const double* x, double* res) {
+__device__ void kkt_residual(CArgs& A, PV& p, const double* rhs, const double* x, double* res) {
   const long long t0 = wall_clock64();
   const int n = p.n;
   pmul(A, p, x, p.wn);
@@ -801,7 +828,7 @@ __device__ void kkt_residual(const BatchArgs& A, PV& p, const double* rhs, const
 }
 
 // escalateAndResolve (kkt_system.cpp:266-294)
-__device__ bool escalate(const BatchArgs& A, PV& p, const double* rhs, double* x, double* colk, double* red) {
+__device__ bool escalate(CArgs& A, PV& p, const double* rhs, double* x, double* colk, double* red) {
   double ep = p.last_p, ea = p.last_a;
   for (int att = 0; att < 10; ++att) {
     ep = dmax(ep * 1000.0, A.dyn_delta);
@@ -819,7 +846,7 @@ __device__ bool escalate(const BatchArgs& A, PV& p, const double* rhs, double* x
 // KktSystem::solve (kkt_system.cpp:296-311)
 // tol: relative residual target; steps past min_steps stop once a step no
 // longer halves the residual
-__device__ double kkt_solve(const BatchArgs& A, PV& p, const double* rhs, double* x, double* colk, double* red,
+__device__ double kkt_solve(CArgs& A, PV& p, const double* rhs, double* x, double* colk, double* red,
                            double tol, int min_steps) {
   const long long t_kkt0 = wall_clock64();
   ++p.nkkt;
@@ -881,7 +908,7 @@ __device__ void ensure_interior_warm(const PV& p, double* v, double* red) {  // 
   if (mn <= 1e-8) unit_shift(p, v, -mn + 1e-6);
 }
 
-__device__ bool initial_point(const BatchArgs& A, PV& p, double* colk, double* red) {  // solver_impl.cpp:126-158
+__device__ bool initial_point(CArgs& A, PV& p, double* colk, double* red) {  // solver_impl.cpp:126-158
   const int n = p.n, m = p.m;
   // scaling at the identity element e
   double* e = p.dsk;
@@ -917,7 +944,7 @@ __device__ bool initial_point(const BatchArgs& A, PV& p, double* colk, double* r
   return count_nonfinite(n, p.x, red) + count_nonfinite(m, p.s, red) + count_nonfinite(m, p.z, red) == 0;
 }
 
-__device__ bool constant_solve(const BatchArgs& A, PV& p, double* colk, double* red) {  // solver_impl.cpp:165-175
+__device__ bool constant_solve(CArgs& A, PV& p, double* colk, double* red) {  // solver_impl.cpp:165-175
   vlin2(p.n, -1.0, p.q, 0.0, p.q, p.rhs);
   vcopy(p.m, p.b, p.rhs + p.n);
   kkt_solve(A, p, p.rhs, p.sol, colk, red, A.refine_tol, A.refine_max_iter);
@@ -927,7 +954,7 @@ __device__ bool constant_solve(const BatchArgs& A, PV& p, double* colk, double* 
   return count_nonfinite(p.n, p.x1, red) + count_nonfinite(p.m, p.z1, red) == 0;
 }
 
-__device__ void residuals(const BatchArgs& A, PV& p, double* red) {  // solver_impl.cpp:177-186
+__device__ void residuals(CArgs& A, PV& p, double* red) {  // solver_impl.cpp:177-186
   const int n = p.n, m = p.m;
   pmul(A, p, p.x, p.Px);
   atmul(A, p, p.z, p.wn);
@@ -950,7 +977,7 @@ __device__ void residuals(const BatchArgs& A, PV& p, double* red) {  // solver_i
   p.rtau = p.dqx + p.dbz + p.kappa + p.dxPx / p.tau;
 }
 
-__device__ bool tau_elim(const BatchArgs& A, PV& p, const double* xv, const double* zv, double dtau_rhs,
+__device__ bool tau_elim(CArgs& A, PV& p, const double* xv, const double* zv, double dtau_rhs,
                          double dkappa_rhs, double* dtau, double* red) {
   const int n = p.n, m = p.m;
   pmul(A, p, xv, p.wn);
@@ -976,7 +1003,7 @@ __device__ bool tau_elim(const BatchArgs& A, PV& p, const double* xv, const doub
   return true;
 }
 
-__device__ bool affine_step(const BatchArgs& A, PV& p, double* colk, double* red) {  // solver_impl.cpp:192-231
+__device__ bool affine_step(CArgs& A, PV& p, double* colk, double* red) {  // solver_impl.cpp:192-231
   const int n = p.n, m = p.m;
   vcopy(n, p.rx, p.rhs);
   vlin2(m, 1.0, p.s, -1.0, p.rz, p.rhs + n);
@@ -992,7 +1019,7 @@ __device__ bool affine_step(const BatchArgs& A, PV& p, double* colk, double* red
          count_nonfinite(n, p.dxa, red) + count_nonfinite(m, p.dza, red) + count_nonfinite(m, p.dsa, red) == 0;
 }
 
-__device__ bool combined_step(const BatchArgs& A, PV& p, double sigma, double mu, double* colk, double* red) {
+__device__ bool combined_step(CArgs& A, PV& p, double sigma, double mu, double* colk, double* red) {
   const int n = p.n, m = p.m;  // solver_impl.cpp:233-286
   combined_rhs(p, p.z, p.dsa, p.dza, p.rz, sigma * mu, 1.0 - sigma, p.dsk, p.rhs + n);
   vlin2(n, 1.0 - sigma, p.rx, 0.0, p.rx, p.rhs);
@@ -1025,7 +1052,7 @@ __device__ double step_length(const PV& p, const double* dsv, const double* dzv,
 // Per-problem view: sizes, bases, LDS/global placement of the factor and of
 // the vectors.  global_M forces the factor into this problem's global slot
 // (the backward pass reads the factor its forward pass left there).
-__device__ void setup_view(const BatchArgs& A, const BatchDesc& D, PV& p, double* lds, bool global_M) {
+__device__ void setup_view(CArgs& A, const BatchDesc& D, PV& p, double* lds, bool global_M) {
   p.n = D.n, p.m = D.m, p.nz = D.nz, p.nn = D.nn, p.nsoc = D.nsoc, p.nk = D.nk;
   p.N = D.n + D.m, p.nr = D.nr, p.pnnz = D.pnnz;
   p.co = D.co, p.so = D.so, p.ko = D.ko, p.pfp = D.pfp, p.acp = D.acp, p.arp = D.arp;
@@ -1109,7 +1136,7 @@ __device__ void setup_view(const BatchArgs& A, const BatchDesc& D, PV& p, double
 }
 
 
-__device__ void solve_problem(const BatchArgs& A, int pid, double* lds, double* colk, double* red) {
+__device__ void solve_problem(CArgs& A, int pid, double* lds, double* colk, double* red) {
   const long long t_start = wall_clock64();
   const BatchDesc D = A.desc[pid];
   PV p;
@@ -1268,9 +1295,10 @@ __device__ void solve_problem(const BatchArgs& A, int pid, double* lds, double* 
   __syncthreads();
 }
 
-__global__ __launch_bounds__(BG) void k_batch_solve(BatchArgs A) {
+__global__ __launch_bounds__(BG) void k_batch_solve(BatchArgs) {
+  CArgs& A = kernel_args();
   extern __shared__ double lds[];
-  __shared__ double colk[PB * JC + PB + 8];  // staged W of the trailing update + panel pivots
+  __shared__ double colk[COLK];  // Wt of the diagonal block, pivots, their reciprocals
   __shared__ double red[64];
   __shared__ int slot;
   for (;;) {
@@ -1290,7 +1318,7 @@ __global__ __launch_bounds__(BG) void k_batch_solve(BatchArgs A) {
 //   dL/dq = -c D u,  dL/db = E v,  dL/dA_ij = -(z_i u_j + v_i x_j) E_i D_j,
 //   dL/dP_ij (upper) = -(u_i x_j + u_j x_i) c D_i D_j, diagonal -u_i x_i c D_i^2,
 // with x, z the normalized optimum in the equilibrated space.
-__device__ void backward_problem(const BatchArgs& A, int pid, double* lds, double* colk, double* red) {
+__device__ void backward_problem(CArgs& A, int pid, double* lds, double* colk, double* red) {
   const long long t_start = wall_clock64();
   const BatchDesc D = A.desc[pid];
   PV p;
@@ -1356,9 +1384,10 @@ __device__ void backward_problem(const BatchArgs& A, int pid, double* lds, doubl
   __syncthreads();
 }
 
-__global__ __launch_bounds__(BG) void k_batch_backward(BatchArgs A) {
+__global__ __launch_bounds__(BG) void k_batch_backward(BatchArgs) {
+  CArgs& A = kernel_args();
   extern __shared__ double lds[];
-  __shared__ double colk[PB * JC + PB + 8];
+  __shared__ double colk[COLK];
   __shared__ double red[64];
   __shared__ int slot;
   for (;;) {
@@ -1397,7 +1426,7 @@ bool valid_csc(int rows, int cols, const int* p, const int* i) {
   return true;
 }
 
-constexpr int LDS_MAX_BYTES = 160 * 1024 - (PB * JC + PB + 8) * 8 - 64 * 8 - 64;  // minus the static LDS of the kernel
+constexpr int LDS_MAX_BYTES = 160 * 1024 - COLK * 8 - 64 * 8 - 64;  // minus the static LDS of the kernel
 
 }  // namespace
 
