@@ -254,9 +254,16 @@ def differentiation_form(K=1024):
     # solve (triangular solves + refinement residuals / KKT solves)
     stats = [b.stats(k) for k in range(K)]
     ok = [s for s in stats if s["us_backward"] > 0 and s["kkt_solves"] > 0]
-    bwd_us = sum(s["us_backward"] for s in ok) / max(1, len(ok))
-    kkt_us = sum((s["us_solve"] + s["us_residual"]) / s["kkt_solves"] for s in ok) / max(1, len(ok))
-    last_us = sum(s["us_last_kkt_solve"] for s in ok) / max(1, len(ok))
+    mean = lambda f: sum(f(s) for s in ok) / max(1, len(ok))
+    bwd_us = mean(lambda s: s["us_backward"])
+    adj_us = mean(lambda s: s["us_adjoint_factor"])
+    kkt_us = mean(lambda s: (s["us_solve"] + s["us_residual"]) / s["kkt_solves"])
+    last_us = mean(lambda s: s["us_last_kkt_solve"])
+    # one forward IPM iteration's linear algebra: its factorization (the forward
+    # factorizations exclude the adjoint one) plus its KKT solves
+    fwd_fact_us = mean(lambda s: (s["us_factor"] - s["us_adjoint_factor"]) / max(1, s["iterations"]))
+    it_us = mean(lambda s: (s["us_factor"] - s["us_adjoint_factor"] + s["us_solve"] + s["us_residual"])
+                 / max(1, s["iterations"]))
     return {
         "workload": f"config3: {K} QPs n=64, m=112 (zero 16, nonneg 64, 8 SOC dim 4), loss w'x",
         "fwd_bwd_us_per_problem": 1e6 * (fwd_s + bwd_s) / K,
@@ -267,11 +274,21 @@ def differentiation_form(K=1024):
         "backward_over_forward_kkt_solve": bwd_us / max(1e-9, kkt_us),
         "workgroup_final_forward_kkt_solve_us": last_us,
         "backward_over_final_forward_kkt_solve": bwd_us / max(1e-9, last_us),
-        "backward_reduced_solves": sum(s["backward_solves"] for s in ok) / max(1, len(ok)),
-        "note": "backward = one adjoint KKT solve (with refinement) per problem on the factor its forward "
-                "pass left, plus the gradient assembly; per-problem times from device clocks; the forward "
-                "KKT solve is given as the mean over the solve and as the final iteration's solve (the "
-                "system the adjoint shares its conditioning with)",
+        "workgroup_adjoint_factorization_us": adj_us,
+        "workgroup_forward_factorization_us_per_iteration": fwd_fact_us,
+        "workgroup_forward_iteration_linear_algebra_us": it_us,
+        "backward_with_adjoint_factorization_over_forward_kkt_solve": (bwd_us + adj_us) / max(1e-9, kkt_us),
+        "backward_with_adjoint_factorization_over_forward_iteration": (bwd_us + adj_us) / max(1e-9, it_us),
+        "backward_reduced_solves": mean(lambda s: s["backward_solves"]),
+        "note": "backward = one adjoint KKT solve (with refinement) per problem plus the gradient assembly, "
+                "on the adjoint factor (final NT scaling with the SOC tangent correction) that the forward "
+                "pass factors after its last iteration; that factorization is the backward's cost and is "
+                "reported with it ('..._with_adjoint_factorization_...'): the correction has rank up to "
+                "sum(d-2) over the SOC blocks, and refinement on the IPM's own last factor does not "
+                "contract on it (spectral radius above 1 on some of these problems). Per-problem times "
+                "from device clocks; the forward KKT solve is the mean over the solve (triangular solves + "
+                "refinement residuals), the final iteration's solve, and one IPM iteration's "
+                "factorization + KKT solves",
     }
 
 
@@ -309,16 +326,28 @@ def precision_form(scale=1.0, accuracy_scale=0.1):
                      "trail_tflops": tr["work"] / max(1e-9, tr["ms"]) / 1e9,
                      "per_factorization_ms": {k: ks[k]["ms"] / nf for k in FACTOR_KINDS if k in ks}}
         print(f"[bench] precision {name}: {res[name]['factor_device_ms']:.0f} ms per factorization", file=sys.stderr, flush=True)
-    del s
-    # accuracy of the whole method: FP32 + refinement vs FP64, same problem
-    q = PR.config4(scale=accuracy_scale)
-    cq = ConeSpec(q.zero_dim, q.nonneg_dim, q.soc_dims)
-    sols, profs = {}, {}
-    for name, prec in (("fp64", FactorPrecision.FP64), ("fp32", FactorPrecision.FP32Refined)):
+    # accuracy of the whole method: full solves, FP32 + refinement vs FP64, same
+    # problem (the timing solver's setup is reused at the same scale)
+    if accuracy_scale == scale:
+        q, sv = p, s
+    else:
+        del s
+        q = PR.config4(scale=accuracy_scale)
         sv = Solver()
-        assert sv.setup(q.P, q.q, q.A, q.b, cq, Settings(warm_start=False, factor_precision=prec))
+        assert sv.setup(q.P, q.q, q.A, q.b, ConeSpec(q.zero_dim, q.nonneg_dim, q.soc_dims),
+                        Settings(warm_start=False))
+    sols, profs, cert = {}, {}, {}
+    for name, prec in (("fp64", FactorPrecision.FP64), ("fp32", FactorPrecision.FP32Refined)):
+        sv.setSettings(Settings(warm_start=False, factor_precision=prec))
         sols[name] = sv.solve()
         profs[name] = sv.profile()
+        r = sols[name]
+        c = PR.kkt_certificate(q, r.x, r.s, r.z)
+        sc = 1.0 + float(np.abs(q.q).max() + np.abs(q.b).max())
+        cert[name] = {"stationarity_rel": c["stationarity"] / sc, "feasibility_rel": c["feasibility"] / sc,
+                      "complementarity": c["complementarity"], "margin_s": c["margin_s"], "margin_z": c["margin_z"]}
+        print(f"[bench] precision full solve {name}: {r.status.name} in {r.info.iterations} iterations, "
+              f"{r.info.solve_time_s:.2f} s", file=sys.stderr, flush=True)
     a, b = sols["fp64"], sols["fp32"]
     rel = lambda u, v: float(np.linalg.norm(u - v) / max(1e-300, np.linalg.norm(v)))
     return {
@@ -338,6 +367,10 @@ def precision_form(scale=1.0, accuracy_scale=0.1):
             "rel_err_vs_fp64": {"x": rel(b.x, a.x), "s": rel(b.s, a.s), "z": rel(b.z, a.z),
                                 "objective": abs(b.objective - a.objective) / max(1e-300, abs(a.objective))},
             "solve_s": {"fp64": a.info.solve_time_s, "fp32": b.info.solve_time_s},
+            "kkt_certificate": cert,
+            "error_note": "the FP64 oracle cannot solve this instance in a bounded time on the host; the "
+                          "FP32 result's error is stated against the GPU FP64 solve of the same problem, "
+                          "and each solve carries its own KKT certificate recomputed on the host",
         },
     }
 
@@ -372,6 +405,9 @@ def main():
                     help="skip the batched, differentiation and precision forms")
     ap.add_argument("--form", choices=["batched", "differentiation", "precision"],
                     help="run only this form and print its JSON object (for profiling)")
+    ap.add_argument("--precision-accuracy-scale", type=float, default=0.1,
+                    help="generator scale of the precision form's full FP64 and FP32 solves (1.0: the "
+                         "full n=100000 instance, about 3.5 minutes)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="end with three 1 GiB device-to-device copies: the reference kernel that "
                          "scales the FETCH_SIZE / WRITE_SIZE counters of a profiled run")
@@ -386,7 +422,8 @@ def main():
             cpu = None if args.no_cpu_baseline else batched_cpu_baseline()
             res = batched_form(timesteps=args.timesteps, cpu=cpu)
         else:
-            res = {"differentiation": differentiation_form, "precision": precision_form}[args.form]()
+            res = (differentiation_form() if args.form == "differentiation"
+                   else precision_form(accuracy_scale=args.precision_accuracy_scale))
         print(json.dumps({"form": args.form, **res}))
         return
     # the host-only oracle baseline of the batched form forks a process pool:
@@ -482,7 +519,7 @@ def main():
     if not args.no_forms:
         out["batched"] = batched_form(timesteps=args.timesteps, cpu=batch_cpu)
         out["differentiation"] = differentiation_form()
-        out["precision"] = precision_form()
+        out["precision"] = precision_form(accuracy_scale=args.precision_accuracy_scale)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(kkt["flops"])
     print(json.dumps(out))

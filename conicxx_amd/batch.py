@@ -159,12 +159,13 @@ class BatchSolver:
     def stats(self, k):
         """Iterations, factorizations, reduced KKT solves (incl. refinement) and the
         device time (us) of problem k in the last solve."""
-        out = np.zeros(12)
+        out = np.zeros(13)
         _lib.lib().cx_batch_get_stats(self._h, k, out.ctypes.data_as(_lib._f64p))
         return dict(iterations=int(out[0]), factorizations=int(out[1]), solves=int(out[2]), us=float(out[3]),
                     us_factor=float(out[4]), us_solve=float(out[5]), us_residual=float(out[6]),
                     kkt_solves=int(out[7]), us_backward=float(out[8]), us_backward_solve=float(out[9]),
-                    backward_solves=int(out[10]), us_last_kkt_solve=float(out[11]))
+                    backward_solves=int(out[10]), us_last_kkt_solve=float(out[11]),
+                    us_adjoint_factor=float(out[12]))
 
     def last_solve_ms(self):
         return float(_lib.lib().cx_batch_last_solve_ms(self._h))

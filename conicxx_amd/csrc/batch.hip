@@ -30,6 +30,12 @@ namespace {
 using namespace conedev;
 
 constexpr int BG = 256;  // threads per problem
+// per-problem record of the last solve / backward in A.info:
+//   0-4 residuals, gap, mu, objective; 5-6 factorizations, reduced solves;
+//   7-10 device ticks (total, factorization, solves, residuals); 11 KKT solves;
+//   12-14 backward ticks, backward solve ticks, backward reduced solves;
+//   15 ticks of the final KKT solve; 16 ticks of the adjoint factorization
+constexpr int INFO_STRIDE = 20;
 constexpr int BATCH_REFINE_MAX = 40;
 
 enum { PV_CHECK = 0, PV_REGULARIZE = 1 };
@@ -1258,12 +1264,15 @@ __device__ void solve_problem(CArgs& A, int pid, double* lds, double* colk, doub
     soc_tangent(p, p.s, p.z, tc, td);
     p.tcorr = tc;
     p.tdir = td;
+    const long long t_adj0 = wall_clock64();
     dok = factor_retry(A, p, colk, red) ? 1 : 0;
     double* Mg = A.Mg + D.mofs;
     if (p.M != Mg) {
       const long tot = packed_size(p.nr);
       for (long e = threadIdx.x; e < tot; e += BG) Mg[e] = p.M[e];
     }
+    __syncthreads();
+    if (threadIdx.x == 0) A.info[(long)INFO_STRIDE * pid + 16] = (double)(wall_clock64() - t_adj0);
     const double it = 1.0 / dmax(p.tau, 1e-30);
     for (int i = threadIdx.x; i < n; i += BG) A.xh[D.xo + i] = p.x[i] * it;
     for (int i = threadIdx.x; i < m; i += BG) A.sh[D.co + i] = p.s[i] * it, A.zh[D.co + i] = p.z[i] * it;
@@ -1274,7 +1283,7 @@ __device__ void solve_problem(CArgs& A, int pid, double* lds, double* colk, doub
       A.diff_a[pid] = p.cur_a;
     }
     const double pobj = (0.5 * p.dxPx / p.tau + p.dqx) / p.tau;
-    double* inf8 = A.info + 16L * pid;
+    double* inf8 = A.info + (long)INFO_STRIDE * pid;
     inf8[0] = sqrt(p.nrz2) / dmax(p.tau, 1e-30);  // primal residual
     inf8[1] = sqrt(p.nrx2) / dmax(p.tau, 1e-30);  // dual residual
     inf8[2] = p.dsz / (p.tau * p.tau) / p.ec;     // gap
@@ -1288,6 +1297,7 @@ __device__ void solve_problem(CArgs& A, int pid, double* lds, double* colk, doub
     inf8[10] = (double)p.t_res;
     inf8[11] = p.nkkt;
     inf8[15] = (double)p.t_kkt_last;  // the final iteration's KKT solve
+    if (!(A.diff && status == CX_SOLVED)) inf8[16] = 0.0;  // no adjoint factorization
     A.status[pid] = status;
     A.iters[pid] = iterations;
     if (status == CX_SOLVED && A.warm_start) A.have_warm[pid] = 1;
@@ -1377,9 +1387,9 @@ __device__ void backward_problem(CArgs& A, int pid, double* lds, double* colk, d
   }
   for (int i = threadIdx.x; i < m; i += BG) A.gb[D.co + i] = ee[i] * v[i];
   if (threadIdx.x == 0) {
-    A.info[16L * pid + 12] = (double)(wall_clock64() - t_start);
-    A.info[16L * pid + 13] = (double)(p.t_solve + p.t_res);
-    A.info[16L * pid + 14] = p.nsolve;
+    A.info[(long)INFO_STRIDE * pid + 12] = (double)(wall_clock64() - t_start);
+    A.info[(long)INFO_STRIDE * pid + 13] = (double)(p.t_solve + p.t_res);
+    A.info[(long)INFO_STRIDE * pid + 14] = p.nsolve;
   }
   __syncthreads();
 }
@@ -1626,7 +1636,7 @@ bool BatchSolver::setup(int nprob, const int* n, const int* m, const int* const*
   beta_.alloc(nsoc_tot), vv_.alloc(nsoc_tot), ident_.alloc(nsoc_tot);
   wxs_.alloc(std::max<long>(1, sum_n_)), wss_.alloc(std::max<long>(1, sum_m_)), wzs_.alloc(std::max<long>(1, sum_m_));
   xo_.alloc(std::max<long>(1, sum_n_)), so_.alloc(std::max<long>(1, sum_m_)), zo_.alloc(std::max<long>(1, sum_m_));
-  info_.alloc(16L * nprob);
+  info_.alloc((long)INFO_STRIDE * nprob);
   status_.alloc(nprob), iters_.alloc(nprob), have_warm_.alloc(nprob);
   have_warm_.zero(st_);
   if (stage_q_) (void)hipHostFree(stage_q_), stage_q_ = nullptr;
@@ -1827,13 +1837,13 @@ void BatchSolver::fetch() {
   hx_.resize(std::max<long>(1, sum_n_));
   hs_.resize(std::max<long>(1, sum_m_));
   hz_.resize(std::max<long>(1, sum_m_));
-  hinfo_.resize(16L * np_);
+  hinfo_.resize((long)INFO_STRIDE * np_);
   hstatus_.resize(np_);
   hiters_.resize(np_);
   xo_.download(hx_.data(), sum_n_, st_);
   so_.download(hs_.data(), sum_m_, st_);
   zo_.download(hz_.data(), sum_m_, st_);
-  info_.download(hinfo_.data(), 16L * np_, st_);
+  info_.download(hinfo_.data(), (long)INFO_STRIDE * np_, st_);
   status_.download(hstatus_.data(), np_, st_);
   iters_.download(hiters_.data(), np_, st_);
   CX_HIP(hipStreamSynchronize(st_));
@@ -1847,13 +1857,13 @@ int BatchSolver::get_solution(int k, double* x, double* s, double* z, double* ob
   if (x) std::memcpy(x, hx_.data() + D.xo, D.n * sizeof(double));
   if (s) std::memcpy(s, hs_.data() + D.co, D.m * sizeof(double));
   if (z) std::memcpy(z, hz_.data() + D.co, D.m * sizeof(double));
-  if (obj) *obj = hinfo_[16L * k + 4];
+  if (obj) *obj = hinfo_[(long)INFO_STRIDE * k + 4];
   if (info) {
     info->iterations = hiters_[k];
-    info->primal_residual = hinfo_[16L * k + 0];
-    info->dual_residual = hinfo_[16L * k + 1];
-    info->duality_gap = hinfo_[16L * k + 2];
-    info->mu = hinfo_[16L * k + 3];
+    info->primal_residual = hinfo_[(long)INFO_STRIDE * k + 0];
+    info->dual_residual = hinfo_[(long)INFO_STRIDE * k + 1];
+    info->duality_gap = hinfo_[(long)INFO_STRIDE * k + 2];
+    info->mu = hinfo_[(long)INFO_STRIDE * k + 3];
     info->setup_time_s = 0;
     info->solve_time_s = last_ms_ * 1e-3;
   }
@@ -1915,22 +1925,23 @@ int BatchSolver::autotune(int reps) {
   return (int)cand.size();
 }
 
-void BatchSolver::stats(int k, double* out12) {
-  for (int i = 0; i < 12; ++i) out12[i] = 0.0;
+void BatchSolver::stats(int k, double* out13) {
+  for (int i = 0; i < 13; ++i) out13[i] = 0.0;
   if (!ready_ || k < 0 || k >= np_) return;
   fetch();
-  out12[0] = hiters_[k];
-  out12[1] = hinfo_[16L * k + 5];
-  out12[2] = hinfo_[16L * k + 6];
-  out12[3] = hinfo_[16L * k + 7] / (clock_khz_ * 1e-3);  // microseconds
-  out12[4] = hinfo_[16L * k + 8] / (clock_khz_ * 1e-3);
-  out12[5] = hinfo_[16L * k + 9] / (clock_khz_ * 1e-3);
-  out12[6] = hinfo_[16L * k + 10] / (clock_khz_ * 1e-3);
-  out12[7] = hinfo_[16L * k + 11];
-  out12[8] = hinfo_[16L * k + 12] / (clock_khz_ * 1e-3);
-  out12[9] = hinfo_[16L * k + 13] / (clock_khz_ * 1e-3);
-  out12[10] = hinfo_[16L * k + 14];
-  out12[11] = hinfo_[16L * k + 15] / (clock_khz_ * 1e-3);
+  out13[0] = hiters_[k];
+  out13[1] = hinfo_[(long)INFO_STRIDE * k + 5];
+  out13[2] = hinfo_[(long)INFO_STRIDE * k + 6];
+  out13[3] = hinfo_[(long)INFO_STRIDE * k + 7] / (clock_khz_ * 1e-3);  // microseconds
+  out13[4] = hinfo_[(long)INFO_STRIDE * k + 8] / (clock_khz_ * 1e-3);
+  out13[5] = hinfo_[(long)INFO_STRIDE * k + 9] / (clock_khz_ * 1e-3);
+  out13[6] = hinfo_[(long)INFO_STRIDE * k + 10] / (clock_khz_ * 1e-3);
+  out13[7] = hinfo_[(long)INFO_STRIDE * k + 11];
+  out13[8] = hinfo_[(long)INFO_STRIDE * k + 12] / (clock_khz_ * 1e-3);
+  out13[9] = hinfo_[(long)INFO_STRIDE * k + 13] / (clock_khz_ * 1e-3);
+  out13[10] = hinfo_[(long)INFO_STRIDE * k + 14];
+  out13[11] = hinfo_[(long)INFO_STRIDE * k + 15] / (clock_khz_ * 1e-3);
+  out13[12] = hinfo_[(long)INFO_STRIDE * k + 16] / (clock_khz_ * 1e-3);
 }
 
 void BatchSolver::tuning(double* out4) const {

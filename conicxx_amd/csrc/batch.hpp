@@ -60,7 +60,7 @@ class BatchSolver {
                double* const* gP, double* const* gq, double* const* gA, double* const* gb);
   double last_backward_ms() const { return last_bw_ms_; }
   void tuning(double* out4) const;
-  void stats(int k, double* out12);
+  void stats(int k, double* out13);  // work counters of problem k (cx_batch_get_stats)
   double last_solve_ms() const { return last_ms_; }
 
  private:

@@ -295,10 +295,10 @@ void cx_batch_get_tuning(const cx_batch* h, double* out4) {
   if (h && out4) h->impl->tuning(out4);
 }
 
-void cx_batch_get_stats(cx_batch* h, int k, double* out12) {
-  if (!h || !out12) return;
+void cx_batch_get_stats(cx_batch* h, int k, double* out13) {
+  if (!h || !out13) return;
   try {
-    h->impl->stats(k, out12);
+    h->impl->stats(k, out13);
   } catch (...) {
   }
 }

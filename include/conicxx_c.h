@@ -224,13 +224,15 @@ int cx_batch_autotune(cx_batch* h, int reps);
 /* out4 = (packed M doubles kept in LDS, persistent workgroups, largest packed M,
  * autotuned flag). */
 void cx_batch_get_tuning(const cx_batch* h, double* out4);
-/* Work counters of problem k in the last solve: out11 = (iterations,
+/* Work counters of problem k in the last solve: out13 = (iterations,
  * factorizations, reduced solves incl. refinement, device microseconds its
  * workgroup spent on it, of which in factorizations, in triangular solves, in
  * refinement residuals, KKT solves, device microseconds of its last backward,
  * of which in its KKT solve, reduced solves of that KKT solve, device
- * microseconds of the final IPM iteration's KKT solve). */
-void cx_batch_get_stats(cx_batch* h, int k, double* out12);
+ * microseconds of the final IPM iteration's KKT solve, device microseconds of
+ * the adjoint factorization at the end of a differentiable forward pass (0
+ * without one; its factorizations are among the forward's)). */
+void cx_batch_get_stats(cx_batch* h, int k, double* out13);
 /* Differentiation of the batch (no counterpart in the reference).  With it on,
  * every Solved problem's forward pass also factors its adjoint KKT matrix (the
  * final scaling with the SOC tangent correction, as cx_backward uses), so that

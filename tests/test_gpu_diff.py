@@ -28,7 +28,14 @@ def relerr(a, b):
     return np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-300)
 
 
-@pytest.mark.parametrize("idx", [0, 4, 5])
+# measured deviation of the adjoint (tangent-corrected NT scaling) from the
+# exact linearization of complementarity at the computed iterate, at the 1e-9
+# stopping tolerances: 4e-5, 2.7e-4 and 7e-6 on these problems; the bars keep
+# a factor of about 4 to 7 above that
+IMPLICIT_BAR = {0: 2e-4, 4: 1e-3, 5: 5e-5}
+
+
+@pytest.mark.parametrize("idx", sorted(IMPLICIT_BAR))
 def test_backward_matches_dense_implicit_reference(idx):
     p = PR.config3_problem(idx)
     s, sol = solved(p)
@@ -37,67 +44,57 @@ def test_backward_matches_dense_implicit_reference(idx):
     ref = implicit_gradients(p, sol.x, sol.s, sol.z, p.meta["w"])
     errs = {k: relerr(g[k].data if k in "PA" else g[k], ref[k]) for k in ("P", "q", "A", "b")}
     print(idx, {k: f"{v:.2e}" for k, v in errs.items()})
-    # tolerance: the reference linearizes complementarity exactly at the computed
-    # iterate, the device at its tangent-corrected NT scaling; both agree to
-    # O(residual) at a 1e-10 solve
     for k, e in errs.items():
-        assert e < 1e-3, (k, e)
+        assert e < IMPLICIT_BAR[idx], (k, e)
 
 
-def test_backward_matches_finite_differences():
-    p = PR.config3_problem(0)
+@pytest.mark.parametrize("idx", [0, 1, 2, 3, 4, 5, 6, 7])
+def test_backward_matches_finite_differences(idx):
+    """Central finite differences on 64 random coordinates of the problem (16
+    each of q, b, A's and P's stored entries), fourth-order stencil, step 1e-3.
+
+    The step: the solution of these problems is determined to about 1e-7
+    relative at the 1e-9 stopping tolerances, so a central quotient at step
+    1e-6 is noise-dominated (measured relative errors 0.1 to 0.9 on these
+    eight problems, all perturbed solves Solved); at step 1e-3 the fourth-order
+    stencil's truncation error stays below that noise floor (measured 2e-4 to
+    2.5e-3).  A coordinate counts when its four perturbed solves end Solved."""
+    p = PR.config3_problem(idx)
     s, sol = solved(p)
     assert sol.status == Status.Solved
     g = s.backward(p.meta["w"])
     h = 1e-3
-    rng = np.random.default_rng(3)
+    rng = np.random.default_rng(100 + idx)
     w = p.meta["w"]
-
-    # The finite differences need loss values accurate to far below h times
-    # the 2e-2 tolerance: a perturbed solve counts when it ends Solved or
-    # stalls within 1e-6 residuals and gap of the optimum.  At the 1e-9
-    # tolerances these degenerate problems can stall a few decades short (the
-    # oracle, rerun on the same perturbed problems with 1e-15 relative noise
-    # on the data, stalls in 1 of 320 solves); a direction with a solve that
-    # neither converges nor stalls close is skipped.
-    def loss(**kw):
-        t = Solver()
-        d = dict(P=p.P, q=p.q, A=p.A, b=p.b)
-        d.update(kw)
-        assert t.setup(d["P"], d["q"], d["A"], d["b"], ConeSpec(p.zero_dim, p.nonneg_dim, p.soc_dims), TIGHT)
-        r = t.solve()
-        close = max(r.info.primal_residual, r.info.dual_residual, abs(r.info.duality_gap)) < 1e-6
-        return w @ r.x if (r.status == Status.Solved or close) else None
-
-    def d4(f):
-        v = [f(t) for t in (-2 * h, -h, h, 2 * h)]
-        if any(x is None for x in v):
-            return None
-        return (v[0] - 8 * v[1] + 8 * v[2] - v[3]) / (12 * h)
-    fd, gv = [], []
-    for i in rng.choice(p.n, 4, replace=False):
-        e = np.zeros(p.n); e[i] = 1.0
-        fd.append(d4(lambda t: loss(q=p.q + t * e))); gv.append(g["q"][i])
-    for i in rng.choice(p.m, 4, replace=False):
-        e = np.zeros(p.m); e[i] = 1.0
-        fd.append(d4(lambda t: loss(b=p.b + t * e))); gv.append(g["b"][i])
+    cs = ConeSpec(p.zero_dim, p.nonneg_dim, p.soc_dims)
     A = sp.csc_matrix(p.A); A.sort_indices()
-    for k in rng.choice(A.nnz, 4, replace=False):
-        def fa(t, k=k):
-            B = A.copy(); B.data[k] += t
-            return loss(A=B)
-        fd.append(d4(fa)); gv.append(g["A"].data[k])
     P = sp.csc_matrix(p.P); P.sort_indices()
-    for k in rng.choice(P.nnz, 4, replace=False):
-        def fp(t, k=k):
-            B = P.copy(); B.data[k] += t
-            return loss(P=B)
-        fd.append(d4(fp)); gv.append(g["P"].data[k])
-    ok = [k for k in range(len(fd)) if fd[k] is not None]
-    assert len(ok) >= 14, f"only {len(ok)} of {len(fd)} directions had four solved perturbations"
-    err = relerr([gv[k] for k in ok], [fd[k] for k in ok])
-    print("fd relerr", err, "directions", len(ok), "of", len(fd))
-    assert err < 2e-2
+
+    def loss(kind, k, t):
+        d = dict(P=P, q=p.q, A=A, b=p.b)
+        if kind in "qb":
+            v = d[kind].copy(); v[k] += t
+        else:
+            v = d[kind].copy(); v.data[k] += t
+        d[kind] = v
+        r = Solver()
+        assert r.setup(d["P"], d["q"], d["A"], d["b"], cs, TIGHT)
+        res = r.solve()
+        return w @ res.x if res.status == Status.Solved else None
+
+    fd, gv = [], []
+    for kind in "qbAP":
+        count = {"q": p.n, "b": p.m, "A": A.nnz, "P": P.nnz}[kind]
+        for k in rng.choice(count, 16, replace=False):
+            v = [loss(kind, k, t) for t in (-2 * h, -h, h, 2 * h)]
+            if any(x is None for x in v):
+                continue
+            fd.append((v[0] - 8 * v[1] + 8 * v[2] - v[3]) / (12 * h))
+            gv.append(g[kind][k] if kind in "qb" else g[kind].data[k])
+    assert len(fd) >= 56, f"only {len(fd)} of 64 coordinates had four solved perturbations"
+    err = relerr(gv, fd)
+    print(f"problem {idx}: fd relerr {err:.2e} over {len(fd)} coordinates")
+    assert err < 1e-2
 
 
 def test_backward_reuse_and_invalidation():

@@ -270,6 +270,25 @@ def test_large_sparse_certificate_many_panel_tasks():
     assert c["margin_s"] > -1e-7 and c["margin_z"] > -1e-7, c
 
 
+def test_config1_full_size_certificate():
+    """config 1 at full size (n = 20000, m = 28500, KKT order 48500, largest
+    front ~15k): status Solved within the reference's iteration range and the
+    KKT optimality certificate of the returned point, recomputed on the host
+    from the problem data."""
+    p = PR.config1()
+    s, sol = gpu_solve(p, Settings(warm_start=False))
+    st = s.kkt_stats()
+    print("kkt", st, "iters", sol.info.iterations, "time", sol.info.solve_time_s)
+    assert st["N"] == 48500 and st["max_front"] > 10000
+    assert sol.status == Status.Solved
+    assert sol.info.iterations <= 40
+    c = PR.kkt_certificate(p, sol.x, sol.s, sol.z)
+    scale = 1.0 + np.abs(p.q).max() + np.abs(p.b).max()
+    assert c["stationarity"] < 1e-6 * scale and c["feasibility"] < 1e-6 * scale, c
+    assert c["margin_s"] > -1e-7 and c["margin_z"] > -1e-7, c
+    assert c["complementarity"] < 1e-6 * (1.0 + abs(sol.objective)), c
+
+
 def test_update_data_new_P_and_A_values(oracle):
     """Solver::updateData with new P and A values on the setup's pattern
     (solver_impl.cpp:57-91, kkt_system.cpp:119-137): device rescaling with the
