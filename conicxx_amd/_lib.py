@@ -101,6 +101,12 @@ SIGNATURES = [
                                       ctypes.c_int, ctypes.c_int, ctypes.c_int, _i32p,
                                       ctypes.POINTER(CxSettings), _f64p, _f64p, _f64p]),
     ("cx_analyze_pattern", ctypes.c_int, [ctypes.c_int, _i32p, _i32p, _i32p, _i32p, _f64p]),
+    ("cx_dense_to_csc", ctypes.c_long, [ctypes.c_int, ctypes.c_int, _f64p, ctypes.c_int, ctypes.c_char_p,
+                                        ctypes.c_int, _i32p, _i32p, _f64p]),
+    ("cx_dense_to_csc_upper", ctypes.c_long, [ctypes.c_int, _f64p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int,
+                                              _i32p, _i32p, _f64p]),
+    ("cx_csr_to_csc", ctypes.c_long, [ctypes.c_int, ctypes.c_int, _i32p, _i32p, _f64p, ctypes.c_char_p,
+                                      ctypes.c_int, _i32p, _i32p, _f64p]),
 ]
 
 _lib = None

@@ -15,7 +15,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(ROOT, "_build")
 LIB = os.path.join(HERE, "libconicxx_hip.so")
 
-SOURCES = ["symbolic.cpp", "kkt_device.hip", "ipm_kernels.hip", "solver.hip", "batch.hip", "capi.cpp"]
+SOURCES = ["symbolic.cpp", "kkt_device.hip", "ipm_kernels.hip", "solver.hip", "batch.hip", "capi.cpp", "convert.cpp"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("CONICXX_OFFLOAD_ARCH", "gfx950")

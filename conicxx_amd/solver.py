@@ -11,7 +11,6 @@ values), cone rows are ordered Zero, Nonnegative, SOC...
 """
 import ctypes
 import enum
-import sys
 from dataclasses import dataclass, field, fields
 
 import numpy as np
@@ -129,24 +128,17 @@ class Solution:  # include/conicxx/solution.h:30-39
         return self.status == Status.Solved
 
 
-def _warn_dense(name):
-    # the reference's dense->sparse conversion warns on stderr (src/conicxx/convert.cpp:11-18)
-    print(f"[conicxx] warning: dense matrix '{name}' converted to sparse; it carries no sparsity "
-          "for the solver to exploit", file=sys.stderr)
-
-
 def _as_csc(M, name, upper=False):
-    if sp.issparse(M):
-        C = sp.csc_matrix(M)
+    """Compressed columns for setup(): dense P / A and CSR matrices go through
+    the library's conversions (src/conicxx/solver.cpp:17-30), which warn on
+    stderr; compressed-column input is taken as it is."""
+    from .convert import to_sparse, to_sparse_upper_triangular
+    if not sp.issparse(M):
+        C = to_sparse_upper_triangular(M, name) if upper else to_sparse(M, name)
+    elif isinstance(M, (sp.csr_matrix, sp.csr_array)):
+        C = to_sparse(M, name)
     else:
-        D = np.asarray(M, dtype=np.float64)
-        _warn_dense(name)
-        if upper:
-            D = np.triu(D)
-            r, c = np.triu_indices(D.shape[0], 0, D.shape[1])
-        else:
-            r, c = np.indices(D.shape).reshape(2, -1)
-        C = sp.csc_matrix((D[r, c], (r, c)), shape=D.shape)  # keeps explicit zeros
+        C = sp.csc_matrix(M)
     C = C.copy()
     C.sort_indices()
     return C

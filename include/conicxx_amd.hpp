@@ -35,6 +35,87 @@ struct CscView {
   const Scalar* valuePtr() const { return values; }
 };
 
+// Owning compressed-column matrix: what the conversions below return.
+struct CscMatrix {
+  Index nrows = 0, ncols = 0;
+  std::vector<Index> colptr, rowind;
+  std::vector<Scalar> values;
+  Index rows() const { return nrows; }
+  Index cols() const { return ncols; }
+  Index nonZeros() const { return ncols > 0 ? colptr[ncols] : 0; }
+  const Index* outerIndexPtr() const { return colptr.data(); }
+  const Index* innerIndexPtr() const { return rowind.data(); }
+  const Scalar* valuePtr() const { return values.data(); }
+  // entry (i, j), 0 where the pattern has none
+  Scalar coeff(Index i, Index j) const {
+    for (Index e = colptr[j]; e < colptr[j + 1]; ++e)
+      if (rowind[e] == i) return values[e];
+    return 0.0;
+  }
+};
+
+// Non-owning dense matrix, column-major unless col_major is false.
+struct DenseView {
+  Index nrows = 0, ncols = 0;
+  const Scalar* data = nullptr;
+  bool col_major = true;
+  Index rows() const { return nrows; }
+  Index cols() const { return ncols; }
+};
+
+// Non-owning row-major (CSR) matrix.
+struct CsrView {
+  Index nrows = 0, ncols = 0;
+  const Index* rowptr = nullptr;  // nrows + 1
+  const Index* colind = nullptr;  // rowptr[nrows]
+  const Scalar* values = nullptr;
+};
+
+// conicxx::toSparse / toSparseUpperTriangular (include/conicxx/convert.h:17-41):
+// every entry of a dense matrix explicit (zeros included, so the pattern does
+// not depend on the values); the upper triangle for P (warning when it is not
+// symmetric); a CSR matrix re-laid out by columns.  Each warns once on stderr
+// unless warn is false.  An invalid input gives an empty 0 x 0 matrix.
+inline CscMatrix toSparse(const DenseView& d, const char* name = nullptr, bool warn = true) {
+  CscMatrix out;
+  out.colptr.assign(static_cast<size_t>(d.ncols) + 1, 0);
+  out.rowind.resize(static_cast<size_t>(d.nrows) * d.ncols);
+  out.values.resize(out.rowind.size());
+  if (cx_dense_to_csc(d.nrows, d.ncols, d.data, d.col_major ? 1 : 0, name, warn ? 1 : 0, out.colptr.data(),
+                      out.rowind.data(), out.values.data()) < 0)
+    return CscMatrix{};
+  out.nrows = d.nrows, out.ncols = d.ncols;
+  return out;
+}
+
+inline CscMatrix toSparseUpperTriangular(const DenseView& d, const char* name = nullptr, bool warn = true) {
+  if (d.nrows != d.ncols) return CscMatrix{};
+  const size_t n = static_cast<size_t>(d.nrows);
+  CscMatrix out;
+  out.colptr.assign(n + 1, 0);
+  out.rowind.resize(n * (n + 1) / 2);
+  out.values.resize(out.rowind.size());
+  if (cx_dense_to_csc_upper(d.nrows, d.data, d.col_major ? 1 : 0, name, warn ? 1 : 0, out.colptr.data(),
+                            out.rowind.data(), out.values.data()) < 0)
+    return CscMatrix{};
+  out.nrows = out.ncols = d.nrows;
+  return out;
+}
+
+inline CscMatrix toSparse(const CsrView& r, const char* name = nullptr, bool warn = true) {
+  if (r.nrows < 0 || r.ncols < 0 || !r.rowptr) return CscMatrix{};
+  const size_t nnz = static_cast<size_t>(r.rowptr[r.nrows]);
+  CscMatrix out;
+  out.colptr.assign(static_cast<size_t>(r.ncols) + 1, 0);
+  out.rowind.resize(nnz);
+  out.values.resize(nnz);
+  if (cx_csr_to_csc(r.nrows, r.ncols, r.rowptr, r.colind, r.values, name, warn ? 1 : 0, out.colptr.data(),
+                    out.rowind.data(), out.values.data()) < 0)
+    return CscMatrix{};
+  out.nrows = r.nrows, out.ncols = r.ncols;
+  return out;
+}
+
 enum class Status {
   Unsolved = CX_UNSOLVED,
   Solved = CX_SOLVED,
@@ -175,6 +256,28 @@ class Solver {
                             cone.soc_dims.empty() ? nullptr : cone.soc_dims.data(), &c);
     if (ok) n_ = n, m_ = m;
     return ok != 0;
+  }
+
+  // dense P and/or A (src/conicxx/solver.cpp:17-30): converted with
+  // toSparseUpperTriangular / toSparse, which warn on stderr.  There is no
+  // dense updateData (solver.h:55-62): convert once and update the values.
+  template <class VecQ, class SpA, class VecB>
+  bool setup(const DenseView& P, const VecQ& q, const SpA& A, const VecB& b, const ConeSpec& cone,
+             const Settings& settings = Settings{}) {
+    const CscMatrix Pc = toSparseUpperTriangular(P, "P");
+    return Pc.ncols == P.ncols && setup(Pc, q, A, b, cone, settings);
+  }
+  template <class SpP, class VecQ, class VecB>
+  bool setup(const SpP& P, const VecQ& q, const DenseView& A, const VecB& b, const ConeSpec& cone,
+             const Settings& settings = Settings{}) {
+    const CscMatrix Ac = toSparse(A, "A");
+    return Ac.ncols == A.ncols && setup(P, q, Ac, b, cone, settings);
+  }
+  template <class VecQ, class VecB>
+  bool setup(const DenseView& P, const VecQ& q, const DenseView& A, const VecB& b, const ConeSpec& cone,
+             const Settings& settings = Settings{}) {
+    const CscMatrix Pc = toSparseUpperTriangular(P, "P"), Ac = toSparse(A, "A");
+    return Pc.ncols == P.ncols && Ac.ncols == A.ncols && setup(Pc, q, Ac, b, cone, settings);
   }
 
   // nullptr = unchanged (solver.h:77); false on pattern mismatch or failure

@@ -266,6 +266,26 @@ int cx_equilibrate(int n, int m, const int* P_colptr, const int* P_rowind, const
 int cx_analyze_pattern(int N, const int* colptr, const int* rowind, int* perm, int* colcount,
                        double* stats8);
 
+/* Input conversions to the compressed-column layout (host only, no GPU
+ * needed; conicxx::toSparse / toSparseUpperTriangular, include/conicxx/
+ * convert.h:17-41, src/conicxx/convert.cpp:22-79).  The caller allocates
+ * colptr (cols + 1) and rowind / values (the returned entry count: rows*cols,
+ * n(n+1)/2, or the CSR's nnz).  Each prints a one-line warning on stderr
+ * unless warn is 0 (name, if not NULL, names the matrix in it).  Return the
+ * number of entries written, or -1 on invalid arguments.
+ *   cx_dense_to_csc: every entry explicit, exact zeros included (a pattern
+ *     independent of the values, as updateData needs);
+ *   cx_dense_to_csc_upper: the entries i <= j of a square matrix (for P), with
+ *     an extra warning when it is not symmetric to 1e-8 of its largest entry;
+ *   cx_csr_to_csc: a row-major matrix re-laid out column by column.
+ * dense is column-major when col_major != 0, else row-major. */
+long cx_dense_to_csc(int rows, int cols, const double* dense, int col_major, const char* name, int warn,
+                     int* colptr, int* rowind, double* values);
+long cx_dense_to_csc_upper(int n, const double* dense, int col_major, const char* name, int warn, int* colptr,
+                           int* rowind, double* values);
+long cx_csr_to_csc(int rows, int cols, const int* rowptr, const int* colind, const double* values_in,
+                   const char* name, int warn, int* colptr, int* rowind, double* values);
+
 #ifdef __cplusplus
 }
 #endif

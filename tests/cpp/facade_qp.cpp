@@ -35,6 +35,26 @@ int main() {
             c.linear_solver == (int)def.linear_solver && c.warm_start == (int)def.warm_start);
   check("toString(Solved)", std::string(cx::toString(cx::Status::Solved)) == "Solved");
 
+  // conversions (test/convert/test_convert.cpp:9-104), host only
+  {
+    const double a[6] = {1.0, 0.0, 0.0, 5.0, 3.0, 0.0};  // 2 x 3, column-major
+    const cx::CscMatrix S = cx::toSparse(cx::DenseView{2, 3, a}, "A", false);
+    bool same = S.rows() == 2 && S.cols() == 3 && S.nonZeros() == 6;
+    for (int j = 0; j < 3; ++j)
+      for (int i = 0; i < 2; ++i) same = same && S.coeff(i, j) == a[j * 2 + i];
+    check("toSparse keeps every dense entry, zeros included", same);
+    const double p[9] = {4.0, 1.0, 2.0, 1.0, 5.0, 3.0, 2.0, 3.0, 6.0};
+    const cx::CscMatrix U = cx::toSparseUpperTriangular(cx::DenseView{3, 3, p}, "P", false);
+    check("toSparseUpperTriangular keeps the 6 entries i <= j",
+          U.nonZeros() == 6 && U.coeff(0, 2) == 2.0 && U.coeff(2, 0) == 0.0 && U.coeff(1, 1) == 5.0);
+    const int rp[3] = {0, 2, 3}, ci[3] = {0, 1, 1};
+    const double cv[3] = {1.0, 2.0, 3.0};
+    const cx::CscMatrix C = cx::toSparse(cx::CsrView{2, 2, rp, ci, cv}, "A", false);
+    check("toSparse(CSR) re-lays out by columns",
+          C.nonZeros() == 3 && C.coeff(0, 0) == 1.0 && C.coeff(0, 1) == 2.0 && C.coeff(1, 1) == 3.0 &&
+              C.coeff(1, 0) == 0.0 && C.colptr[1] == 1);
+  }
+
   // min 0.5 (x0^2 + x1^2)  s.t. x0 + x1 = 1  ->  x = (0.5, 0.5), objective 0.25
   Csc P{{0, 1, 2}, {0, 1}, {1.0, 1.0}, 2, 2};
   Csc A{{0, 1, 2}, {0, 0}, {1.0, 1.0}, 1, 2};
@@ -75,5 +95,13 @@ int main() {
   const std::vector<double>* nv = nullptr;
   const cx::CscView pv = Pbad.view();
   check("updateData rejects a new pattern", !sb.updateData(&pv, nv, (const cx::CscView*)nullptr, nv));
+
+  // dense P and A (test/solver/test_solve_dense_input.cpp:58-78): optimum (0, 2)
+  const double Pd[4] = {1.0, 0.0, 0.0, 1.0}, Ad[4] = {-1.0, 0.0, 0.0, -1.0};
+  std::vector<double> qd{1.0, -2.0}, bd{0.0, 0.0};
+  cx::Solver sd;
+  check("dense setup", sd.setup(cx::DenseView{2, 2, Pd}, qd, cx::DenseView{2, 2, Ad}, bd, box));
+  const cx::Solution& s3 = sd.solve();
+  check("dense-input QP x = (0, 2)", s3.ok() && std::fabs(s3.x[0]) < 1e-6 && std::fabs(s3.x[1] - 2.0) < 1e-6);
   return fails ? 1 : 0;
 }

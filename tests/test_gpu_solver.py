@@ -289,6 +289,41 @@ def test_config1_full_size_certificate():
     assert c["complementarity"] < 1e-6 * (1.0 + abs(sol.objective)), c
 
 
+def _dense_input_problem():
+    # the box-constrained QP of test/solver/test_solve_dense_input.cpp:11-33:
+    # min 0.5 (x0 + 1)^2 + 0.5 (x1 - 2)^2 s.t. x >= 0, optimum (0, 2)
+    return np.eye(2), np.array([1.0, -2.0]), -np.eye(2), np.zeros(2), ConeSpec(0, 2, [])
+
+
+def test_dense_input_matches_sparse(capfd):
+    """test_solve_dense_input.cpp:37-56: dense P and A (converted by the
+    library, with warnings) give the sparse input's solution."""
+    P, q, A, b, cs = _dense_input_problem()
+    s1 = Solver()
+    assert s1.setup(sp.csc_matrix(P), q, sp.csc_matrix(A), b, cs)
+    ref = s1.solve()
+    assert ref.ok()
+    capfd.readouterr()
+    s2 = Solver()
+    assert s2.setup(P, q, A, b, cs)
+    err = capfd.readouterr().err
+    assert "(matrix: P)" in err and "(matrix: A)" in err
+    sol = s2.solve()
+    assert sol.ok()
+    assert np.allclose(sol.x, ref.x, atol=1e-6)
+
+
+def test_mixed_dense_and_sparse_inputs():
+    """test_solve_dense_input.cpp:58-78, and a row-major (CSR) A."""
+    P, q, A, b, cs = _dense_input_problem()
+    for Pin, Ain in ((P, sp.csc_matrix(A)), (sp.csc_matrix(P), A), (P, sp.csr_matrix(A))):
+        s = Solver()
+        assert s.setup(Pin, q, Ain, b, cs)
+        sol = s.solve()
+        assert sol.ok()
+        assert np.allclose(sol.x, [0.0, 2.0], atol=1e-6), sol.x
+
+
 def test_update_data_new_P_and_A_values(oracle):
     """Solver::updateData with new P and A values on the setup's pattern
     (solver_impl.cpp:57-91, kkt_system.cpp:119-137): device rescaling with the
