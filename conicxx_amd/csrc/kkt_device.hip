@@ -55,6 +55,10 @@ __host__ __device__ inline long colofs(int nr, int j) { return (long)j * nr - ((
 // element (i, j), i >= j
 __host__ __device__ inline long fpos(int nr, int i, int j) { return colofs(nr, j) + (i - j); }
 
+// stats[0]: an exact zero pivot; stats[1]: pivots corrected inline;
+// stats[2]: in the checked mode, a pivot with |d| <= eps (or NaN) was met.
+// Such a factorization is rejected whatever follows (its minimum |D| is at
+// most eps), so the remaining launches of it return at once.
 __device__ inline double pivot_fix(double d, double sgn, int mode, double eps, double delta,
                                    int* stats, bool count) {
   if (mode == PIVOT_REGULARIZE) {
@@ -66,7 +70,13 @@ __device__ inline double pivot_fix(double d, double sgn, int mode, double eps, d
     return d;
   }
   if (d == 0.0 && count) atomicOr(&stats[0], 1);  // QDLDL_factor reports an exact zero pivot
+  if (count && !(fabs(d) > eps)) atomicOr(&stats[2], 1);
   return d;
+}
+
+// true once a launch of the current factorization has flagged a failing pivot
+__device__ inline bool factor_stopped(const int* stats) {
+  return __hip_atomic_load(&stats[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
 }
 
 __device__ inline double load_acq(const double* p) {
@@ -324,7 +334,8 @@ __global__ __launch_bounds__(256) void k_assemble(
     const int* __restrict__ nrv, const int* __restrict__ ncv, const int* __restrict__ firstv,
     const int* __restrict__ kcolptr, const int* __restrict__ ksrc, const long* __restrict__ kdst,
     const signed char* __restrict__ kdiag, const double* __restrict__ Kx, double extra_p,
-    double extra_a, const int* __restrict__ rmap_ofs, const int* __restrict__ rmap, T* F) {
+    double extra_a, const int* __restrict__ rmap_ofs, const int* __restrict__ rmap, T* F, const int* stats) {
+  if (factor_stopped(stats)) return;
   const AsmTask Tk = tasks[blockIdx.x];
   const int s = Tk.s, nr = nrv[s], nc = ncv[s];
   const long base = fofs[s];
@@ -378,6 +389,7 @@ __global__ __launch_bounds__(256) void k_factor_small(
     const int* __restrict__ list, const long* __restrict__ fofs, const int* __restrict__ nrv,
     const int* __restrict__ ncv, const int* __restrict__ firstv, const double* __restrict__ sgn,
     T* F, double* Dg, int mode, double eps, double delta, int* stats) {
+  if (factor_stopped(stats)) return;
   __shared__ T A[SMALL_NR * LDSP];
   __shared__ T colk[SMALL_NR];
   const int s = list[blockIdx.x];
@@ -414,6 +426,7 @@ __global__ __launch_bounds__(256) void k_panel(
     const PanelTask* __restrict__ tasks, const long* __restrict__ fofs, const int* __restrict__ nrv,
     const int* __restrict__ firstv, const double* __restrict__ sgn, T* F, T* Fd, double* Dg, int mode,
     double eps, double delta, int* stats) {
+  if (factor_stopped(stats)) return;
   // one block in a fixed order, the broadcast column first: its reads then
   // take immediate LDS offsets (below 64 KB) instead of an address register each
   struct Lds {
@@ -497,7 +510,8 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_diag_store(const PanelTask* __restrict__ slots,
                                                     const long* __restrict__ fofs,
                                                     const int* __restrict__ nrv, const T* __restrict__ Fd,
-                                                    T* F) {
+                                                    T* F, const int* stats) {
+  if (factor_stopped(stats)) return;
   const PanelTask Tk = slots[blockIdx.x];
   const int s = Tk.s, nr = nrv[s], a = Tk.a, w = Tk.w;
   const long base = fofs[s];
@@ -542,7 +556,8 @@ struct Mfma<float> {
 template <typename T, int WN>
 __global__ __launch_bounds__(128 * WN, (WN == 4 || sizeof(T) == 4) ? 4 : 2) void k_trail(
     const TrailTask* __restrict__ tasks, const long* __restrict__ fofs, const int* __restrict__ nrv,
-    const int* __restrict__ firstv, const double* __restrict__ Dg, T* F) {
+    const int* __restrict__ firstv, const double* __restrict__ Dg, T* F, const int* stats) {
+  if (factor_stopped(stats)) return;
   typedef typename Mfma<T>::acc acc_t;
   constexpr int NT = 128 * WN;          // threads
   constexpr int NI = 8 / WN;            // 16-column MFMA blocks per wave
@@ -1181,7 +1196,7 @@ void DeviceKKT::setup(const LowerPattern& K, int nx, hipStream_t stream) {
   rmap_.upload(rmap, st_);
   F_.alloc(fofs[ns] + TM);  // + TM: the trailing update's staging reads run past the last front
   D_.alloc(N);
-  fstat_.alloc(2);
+  fstat_.alloc(3);
   dstat_.alloc(1);
 
   build_schedule();
@@ -1483,7 +1498,7 @@ void DeviceKKT::launch_factor(T* F, const double* Kx, double extra_p, double ext
         tbegin();
         hipLaunchKernelGGL(k_assemble<T>, dim3(n), dim3(256), 0, st_, asm_.p + L.beg, seg_.p, fofs_.p,
                            nr_.p, nc_.p, first_.p, kcolptr_.p, ksrc_.p, kdst_.p, kdiag_.p, Kx,
-                           extra_p, extra_a, rmap_ofs_.p, rmap_.p, F);
+                           extra_p, extra_a, rmap_ofs_.p, rmap_.p, F, fstat_.p);
         tend(T_ASM, 0.0);
         break;
       case L_SMALL:
@@ -1502,10 +1517,10 @@ void DeviceKKT::launch_factor(T* F, const double* Kx, double extra_p, double ext
         tbegin();
         if constexpr (sizeof(T) == 4)
           hipLaunchKernelGGL((k_trail<T, 2>), dim3(n), dim3(256), 0, st_, trail_.p + L.beg, fofs_.p, nr_.p,
-                             first_.p, D_.p, F);
+                             first_.p, D_.p, F, fstat_.p);
         else
           hipLaunchKernelGGL((k_trail<T, 4>), dim3(n), dim3(512), 0, st_, trail_.p + L.beg, fofs_.p, nr_.p,
-                             first_.p, D_.p, F);
+                             first_.p, D_.p, F, fstat_.p);
         tend(T_TRAIL, trail_launch_flops_[tl]);
         ++tl;
         break;
@@ -1515,7 +1530,7 @@ void DeviceKKT::launch_factor(T* F, const double* Kx, double extra_p, double ext
   if (!dslot_h_.empty()) {
     tbegin();
     hipLaunchKernelGGL(k_diag_store<T>, dim3((unsigned)dslot_h_.size()), dim3(256), 0, st_, dslot_.p, fofs_.p, nr_.p,
-                       reinterpret_cast<const T*>(Fd_.p), F);
+                       reinterpret_cast<const T*>(Fd_.p), F, fstat_.p);
     tend(T_PANEL, 0.0);
     CX_LAUNCH_CHECK();
   }
@@ -1524,7 +1539,7 @@ void DeviceKKT::launch_factor(T* F, const double* Kx, double extra_p, double ext
 
 void DeviceKKT::factorize_async(const double* Kx, double extra_p, double extra_a, int mode,
                                 double eps, double delta, int prec) {
-  CX_HIP(hipMemsetAsync(fstat_.p, 0, 2 * sizeof(int), st_));
+  CX_HIP(hipMemsetAsync(fstat_.p, 0, 3 * sizeof(int), st_));
   if (prec == 1) {
     launch_factor<float>(reinterpret_cast<float*>(F_.p), Kx, extra_p, extra_a, mode, eps, delta);
   } else {
@@ -1537,14 +1552,15 @@ void DeviceKKT::factorize_async(const double* Kx, double extra_p, double extra_a
 
 FactorStats DeviceKKT::factor_stats() {
   FactorStats fs;
-  int st[2] = {0, 0};
+  int st[3] = {0, 0, 0};
   double mn = 0;
-  fstat_.download(st, 2, st_);
+  fstat_.download(st, 3, st_);
   dstat_.download(&mn, 1, st_);
   CX_HIP(hipStreamSynchronize(st_));
   collect_timing();
   fs.zero_pivot = st[0];
   fs.nreg = st[1];
+  fs.stopped = st[2];
   fs.min_abs_d = mn;
   return fs;
 }

@@ -18,6 +18,7 @@ struct FactorStats {
   double min_abs_d = 0;  // NaN if any pivot is NaN
   int zero_pivot = 0;    // exact zero pivot met (checked pivot mode)
   int nreg = 0;          // pivots corrected inline (regularizing mode)
+  int stopped = 0;       // checked mode: a pivot |d| <= eps ended the factorization early
 };
 
 struct AsmTask { int s, j0, j1, seg_beg, seg_end; };

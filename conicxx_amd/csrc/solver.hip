@@ -365,7 +365,7 @@ bool Solver::try_factor(double ep, double ea) {
   FactorStats fs = kkt_.factor_stats();
   prof_fact_ms_ += (now_s() - t0) * 1e3;
   prof_nfact_++;
-  if (mode == PIVOT_CHECK && fs.zero_pivot) return false;
+  if (mode == PIVOT_CHECK && (fs.zero_pivot || fs.stopped)) return false;
   if (mode == PIVOT_REGULARIZE && (double)fs.nreg > 0.05 * (double)N_) return false;
   return std::isfinite(fs.min_abs_d) && fs.min_abs_d > st_.dynamic_reg_eps;
 }
