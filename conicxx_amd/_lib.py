@@ -85,7 +85,9 @@ SIGNATURES = [
                                       ctypes.POINTER(_i32p), ctypes.POINTER(_i32p), ctypes.POINTER(_f64p),
                                       ctypes.POINTER(_f64p), _i32p, _i32p, _i32p, ctypes.POINTER(_i32p),
                                       ctypes.POINTER(CxSettings)]),
-    ("cx_batch_update_data", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_f64p), ctypes.POINTER(_f64p)]),
+    ("cx_batch_update_data", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_i32p), ctypes.POINTER(_i32p),
+                                            ctypes.POINTER(_f64p), ctypes.POINTER(_f64p), ctypes.POINTER(_i32p),
+                                            ctypes.POINTER(_i32p), ctypes.POINTER(_f64p), ctypes.POINTER(_f64p)]),
     ("cx_batch_solve", ctypes.c_int, [ctypes.c_void_p, _i32p]),
     ("cx_batch_get_solution", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _f64p, _f64p, _f64p, _f64p,
                                              ctypes.POINTER(CxInfo)]),

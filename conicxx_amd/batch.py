@@ -104,8 +104,11 @@ class BatchSolver:
             ctypes.byref(settings.to_c()))
         return bool(ok)
 
-    def update_data(self, qs=None, bs=None):
-        """New q and/or b for every problem (a list; None entries = unchanged)."""
+    def update_data(self, qs=None, bs=None, Ps=None, As=None):
+        """Solver.updateData for every problem at once: lists of new q, b, P
+        (upper triangle) and A per problem, None (or a None entry) = unchanged.
+        P and A must keep setup's sparsity pattern; returns False, changing
+        nothing, if any of them does not."""
         K = len(self._dims)
         keep = []
 
@@ -123,7 +126,24 @@ class BatchSolver:
                 keep.append(a)
                 out[k] = p
             return out
-        return bool(_lib.lib().cx_batch_update_data(self._h, parr(qs, 0), parr(bs, 1)))
+
+        def marr(mats, name, upper):
+            if mats is None:
+                return None, None, None
+            pp, ii, vv = (_lib._i32p * K)(), (_lib._i32p * K)(), (_lib._f64p * K)()
+            for k, M in enumerate(mats):
+                if M is None:
+                    pp[k], ii[k], vv[k] = _lib.NULL_I, _lib.NULL_I, _lib.NULL_F
+                    continue
+                C = _as_csc(M, name, upper=upper)
+                for dst, a, conv in ((pp, C.indptr, _lib.i32), (ii, C.indices, _lib.i32), (vv, C.data, _lib.f64)):
+                    arr, ptr = conv(a if len(a) else np.zeros(1))
+                    keep.append(arr)
+                    dst[k] = ptr
+            return pp, ii, vv
+        Pp, Pi, Px = marr(Ps, "P", True)
+        Ap, Ai, Ax = marr(As, "A", False)
+        return bool(_lib.lib().cx_batch_update_data(self._h, Pp, Pi, Px, parr(qs, 0), Ap, Ai, Ax, parr(bs, 1)))
 
     def solve(self):
         """Solves every problem; returns the array of statuses."""

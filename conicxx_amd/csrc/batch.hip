@@ -1420,6 +1420,13 @@ __global__ void k_batch_prepare(int total_n, int total_m, const int* xprob, cons
   if (i < total_m) b[i] = braw[i] * ee[i];
 }
 
+// P / A values of an update: scaled = raw * (per-entry equilibration factor)
+__global__ void k_batch_scale_values(long n, const double* __restrict__ raw, const double* __restrict__ scale,
+                                     double* __restrict__ out) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = raw[i] * scale[i];
+}
+
 double now_ms() {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
@@ -1452,6 +1459,8 @@ BatchSolver::~BatchSolver() {
   if (graph_) (void)hipGraphDestroy(graph_);
   if (stage_q_) (void)hipHostFree(stage_q_);
   if (stage_b_) (void)hipHostFree(stage_b_);
+  if (stage_P_) (void)hipHostFree(stage_P_);
+  if (stage_A_) (void)hipHostFree(stage_A_);
   if (ev0_) (void)hipEventDestroy(ev0_);
   if (ev1_) (void)hipEventDestroy(ev1_);
   if (st_) (void)hipStreamDestroy(st_);
@@ -1496,7 +1505,9 @@ bool BatchSolver::setup(int nprob, const int* n, const int* m, const int* const*
   desc_h_.assign(nprob, BatchDesc{});
   std::vector<int> pf_ptr, pf_col, pf_src, ac_ptr, ac_row, ar_ptr, ar_col, ar_src, soc_off, soc_dim;
   std::vector<int> rk, kr, xprob;
-  std::vector<double> Pv, Av, qv, bv;
+  std::vector<double> Pv, Av, qv, bv, sP, sA;
+  Pp_h_.assign(nprob, {}), Pi_h_.assign(nprob, {}), Ap_h_.assign(nprob, {}), Ai_h_.assign(nprob, {});
+  Praw_h_.clear(), Araw_h_.clear();
   ed_h_.clear();
   ee_h_.clear();
   long wofs = 0, mofs = 0;
@@ -1523,6 +1534,15 @@ bool BatchSolver::setup(int nprob, const int* n, const int* m, const int* const*
     double eck = 1.0;
     ruiz_equilibrate(nk, mk, Pp[k], Pi[k], pv, Ap[k], Ai[k], av, so_k, socd_[k], st, edk, eek, eck);
     D.ec = eck;
+    // setup's pattern and the per-entry scale of later value updates
+    // (Equilibration::scaleP / scaleA, equilibration.cpp:102-115)
+    Pp_h_[k].assign(Pp[k], Pp[k] + nk + 1), Pi_h_[k].assign(Pi[k], Pi[k] + nP);
+    Ap_h_[k].assign(Ap[k], Ap[k] + nk + 1), Ai_h_[k].assign(Ai[k], Ai[k] + nA);
+    Praw_h_.insert(Praw_h_.end(), Px[k], Px[k] + nP), Araw_h_.insert(Araw_h_.end(), Ax[k], Ax[k] + nA);
+    for (int c = 0; c < nk; ++c) {
+      for (int e = Pp[k][c]; e < Pp[k][c + 1]; ++e) sP.push_back(eck * edk[Pi[k][e]] * edk[c]);
+      for (int e = Ap[k][c]; e < Ap[k][c + 1]; ++e) sA.push_back(eek[Ai[k][e]] * edk[c]);
+    }
     ed_h_.insert(ed_h_.end(), edk.begin(), edk.end());
     ee_h_.insert(ee_h_.end(), eek.begin(), eek.end());
     const int pbase = (int)Pv.size(), abase = (int)Av.size();
@@ -1617,6 +1637,11 @@ bool BatchSolver::setup(int nprob, const int* n, const int* m, const int* const*
   up_i(soc_off_, soc_off), up_i(soc_dim_, soc_dim);
   up_i(rk_, rk), up_i(kr_, kr);
   up_d(Pv_, Pv), up_d(Av_, Av), up_d(qraw_, qv), up_d(braw_, bv), up_d(ed_, ed_h_), up_d(ee_, ee_h_);
+  if (sP.size() != Pv.size() || sA.size() != Av.size()) throw std::runtime_error("batch: value scales out of step");
+  up_d(sP_, sP), up_d(sA_, sA);
+  P_dirty_ = A_dirty_ = false;
+  if (stage_P_) (void)hipHostFree(stage_P_), stage_P_ = nullptr;
+  if (stage_A_) (void)hipHostFree(stage_A_), stage_A_ = nullptr;
   q_.alloc(std::max<long>(1, sum_n_));
   b_.alloc(std::max<long>(1, sum_m_));
   {
@@ -1667,6 +1692,43 @@ bool BatchSolver::setup(int nprob, const int* n, const int* m, const int* const*
     set_differentiable(true);
   }
   return true;
+}
+
+bool BatchSolver::update_data(const int* const* Pp, const int* const* Pi, const double* const* Px,
+                              const double* const* q, const int* const* Ap, const int* const* Ai,
+                              const double* const* Ax, const double* const* b) {
+  if (!ready_) return false;
+  // every problem's pattern is checked before anything changes
+  // (KktSystem::sameSparsityPattern, kkt_system.cpp:10-19)
+  auto same = [](const std::vector<int>& p, const std::vector<int>& i, int ncol, const int* pp, const int* ii) {
+    if (!pp || !ii) return false;
+    if (!std::equal(p.begin(), p.end(), pp)) return false;
+    return std::equal(i.begin(), i.end(), ii) && pp[ncol] == (int)i.size();
+  };
+  for (int k = 0; k < np_; ++k) {
+    const int nk = desc_h_[k].n;
+    if (Px && Px[k] && !same(Pp_h_[k], Pi_h_[k], nk, Pp ? Pp[k] : nullptr, Pi ? Pi[k] : nullptr)) return false;
+    if (Ax && Ax[k] && !same(Ap_h_[k], Ai_h_[k], nk, Ap ? Ap[k] : nullptr, Ai ? Ai[k] : nullptr)) return false;
+  }
+  bool anyP = false, anyA = false;
+  for (int k = 0; k < np_; ++k) anyP |= Px && Px[k], anyA |= Ax && Ax[k];
+  if (anyP && !stage_P_) {
+    CX_HIP(hipHostMalloc((void**)&stage_P_, std::max<long>(1, nnzP_total_) * sizeof(double), hipHostMallocDefault));
+    Praw_.alloc(std::max<long>(1, nnzP_total_));
+    // problems not passed keep their values: the staging starts from setup's
+    std::memcpy(stage_P_, Praw_h_.data(), Praw_h_.size() * sizeof(double));
+  }
+  if (anyA && !stage_A_) {
+    CX_HIP(hipHostMalloc((void**)&stage_A_, std::max<long>(1, nnzA_total_) * sizeof(double), hipHostMallocDefault));
+    Araw_.alloc(std::max<long>(1, nnzA_total_));
+    std::memcpy(stage_A_, Araw_h_.data(), Araw_h_.size() * sizeof(double));
+  }
+  for (int k = 0; k < np_; ++k) {
+    if (Px && Px[k]) std::memcpy(stage_P_ + pv_base_[k], Px[k], Pi_h_[k].size() * sizeof(double));
+    if (Ax && Ax[k]) std::memcpy(stage_A_ + av_base_[k], Ax[k], Ai_h_[k].size() * sizeof(double));
+  }
+  P_dirty_ |= anyP, A_dirty_ |= anyA;
+  return update_data(q, b);
 }
 
 bool BatchSolver::update_data(const double* const* q, const double* const* b) {
@@ -1807,6 +1869,24 @@ int BatchSolver::backward(const double* const* wx, const double* const* ws, cons
   return nok;
 }
 
+// P / A values staged by update_data: uploaded and scaled on the solve's
+// stream ahead of the graph (q and b travel inside the graph, every solve)
+void BatchSolver::upload_values() {
+  if (P_dirty_ && nnzP_total_ > 0) {
+    CX_HIP(hipMemcpyAsync(Praw_.p, stage_P_, nnzP_total_ * sizeof(double), hipMemcpyHostToDevice, st_));
+    hipLaunchKernelGGL(k_batch_scale_values, dim3(ceil_div(nnzP_total_, 256)), dim3(256), 0, st_, nnzP_total_,
+                       Praw_.p, sP_.p, Pv_.p);
+    CX_LAUNCH_CHECK();
+  }
+  if (A_dirty_ && nnzA_total_ > 0) {
+    CX_HIP(hipMemcpyAsync(Araw_.p, stage_A_, nnzA_total_ * sizeof(double), hipMemcpyHostToDevice, st_));
+    hipLaunchKernelGGL(k_batch_scale_values, dim3(ceil_div(nnzA_total_, 256)), dim3(256), 0, st_, nnzA_total_,
+                       Araw_.p, sA_.p, Av_.p);
+    CX_LAUNCH_CHECK();
+  }
+  P_dirty_ = A_dirty_ = false;
+}
+
 int BatchSolver::solve(int* status_out) {
   if (!ready_) return -1;
   if (!gexec_) {
@@ -1816,6 +1896,7 @@ int BatchSolver::solve(int* status_out) {
     CX_HIP(hipGraphInstantiate(&gexec_, graph_, nullptr, nullptr, 0));
   }
   CX_HIP(hipEventRecord(ev0_, st_));
+  upload_values();
   CX_HIP(hipGraphLaunch(gexec_, st_));
   CX_HIP(hipEventRecord(ev1_, st_));
   CX_HIP(hipStreamSynchronize(st_));

@@ -49,6 +49,11 @@ class BatchSolver {
              const int* nonneg_dim, const int* nsoc, const int* const* soc_dims, const cx_settings& st);
   // new q and/or b values for every problem (NULL = unchanged)
   bool update_data(const double* const* q, const double* const* b);
+  // new P, q, A and/or b values (NULL = unchanged); P and A values must come
+  // with their index arrays, which must equal setup's pattern
+  bool update_data(const int* const* Pp, const int* const* Pi, const double* const* Px,
+                   const double* const* q, const int* const* Ap, const int* const* Ai,
+                   const double* const* Ax, const double* const* b);
   int solve(int* status_out);  // returns the number of Solved problems, -1 on failure
   int get_solution(int k, double* x, double* s, double* z, double* obj, cx_info* info);
   void set_settings(const cx_settings& st);
@@ -89,6 +94,16 @@ class BatchSolver {
   DevBuf<int> ident_, status_, iters_, have_warm_;
   double* stage_q_ = nullptr;  // pinned staging for update_data
   double* stage_b_ = nullptr;
+  // P / A value updates: setup's patterns (host, for the pattern check), the
+  // per-entry equilibration factors (c d_r d_c for P, e_r d_c for A), raw
+  // values staged in pinned memory and uploaded + scaled before the next solve
+  std::vector<std::vector<int>> Pp_h_, Pi_h_, Ap_h_, Ai_h_;
+  std::vector<double> Praw_h_, Araw_h_;  // setup's values, unscaled
+  DevBuf<double> sP_, sA_, Praw_, Araw_;
+  double* stage_P_ = nullptr;
+  double* stage_A_ = nullptr;
+  bool P_dirty_ = false, A_dirty_ = false;
+  void upload_values();
   int max_packed_ = 0;
   long max_lds_need_ = 0;
   BatchTuning tune_{};

@@ -248,10 +248,12 @@ int cx_batch_setup(cx_batch* h, int nprob, const int* n, const int* m, const int
   }
 }
 
-int cx_batch_update_data(cx_batch* h, const double* const* q, const double* const* b) {
+int cx_batch_update_data(cx_batch* h, const int* const* P_colptr, const int* const* P_rowind,
+                         const double* const* P_val, const double* const* q, const int* const* A_colptr,
+                         const int* const* A_rowind, const double* const* A_val, const double* const* b) {
   if (!h) return 0;
   try {
-    return h->impl->update_data(q, b) ? 1 : 0;
+    return h->impl->update_data(P_colptr, P_rowind, P_val, q, A_colptr, A_rowind, A_val, b) ? 1 : 0;
   } catch (...) {
     return 0;
   }

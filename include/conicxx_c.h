@@ -206,10 +206,19 @@ int cx_batch_setup(cx_batch* h, int nprob, const int* n, const int* m, const int
                    const int* const* A_colptr, const int* const* A_rowind, const double* const* A_val,
                    const double* const* b, const int* zero_dim, const int* nonneg_dim, const int* n_soc,
                    const int* const* soc_dims, const cx_settings* settings);
-/* updateData(nullptr, &q, nullptr, &b) for every problem at once; q or b (or an
- * entry of either) NULL = unchanged.  Values are staged on the host and uploaded
- * by the next cx_batch_solve. */
-int cx_batch_update_data(cx_batch* h, const double* const* q, const double* const* b);
+/* Solver::updateData(P*, q*, A*, b*) (solver.h:77; solver_impl.cpp:57-91) for
+ * every problem at once, with per-problem pointer arrays laid out as in
+ * cx_batch_setup.  An array, or an entry of one, that is NULL leaves that piece
+ * of that problem unchanged; P_val[k] (A_val[k]) must come with
+ * P_colptr[k]/P_rowind[k] (A_colptr[k]/A_rowind[k]) equal to setup's pattern
+ * (kkt_system.cpp:10-19).  Every pattern is checked before anything changes:
+ * returns 0 on any mismatch, 1 otherwise.  New values are scaled by setup's
+ * equilibration (equilibration.cpp:102-121), staged on the host and uploaded
+ * by the next cx_batch_solve, whose per-problem factorizations use them (the
+ * reference refactorizes inside updateData; here every solve factors anyway). */
+int cx_batch_update_data(cx_batch* h, const int* const* P_colptr, const int* const* P_rowind,
+                         const double* const* P_val, const double* const* q, const int* const* A_colptr,
+                         const int* const* A_rowind, const double* const* A_val, const double* const* b);
 /* Solves every problem; writes per-problem statuses (nprob) when non-NULL and
  * returns the number of Solved problems (-1 on device failure).  With
  * settings.warm_start each Solved problem seeds its next solve. */

@@ -170,3 +170,65 @@ def test_batch_whole_contact_batch_statuses():
     bad = [i for i in range(len(probs)) if st[i] != int(Status.Solved)]
     print("non-Solved", bad, "device ms", b.last_solve_ms())
     assert len(bad) <= 4, bad
+
+
+def _new_values(p, rng):
+    """P and A with new values on p's pattern (P stays positive definite), and
+    a feasible q, b for them from the seeded construction."""
+    P2 = p.P.copy()
+    P2.data = P2.data * 1.3
+    col = np.repeat(np.arange(p.n), np.diff(P2.indptr))
+    P2.data[P2.indices == col] += 0.05
+    A2 = p.A.copy()
+    A2.data = A2.data * (1.0 + 0.2 * rng.standard_normal(A2.nnz))
+    q2, b2, *_ = PR.feasible_qb(rng, P2, A2, p.zero_dim, p.nonneg_dim, p.soc_dims)
+    return PR.Problem(P2, q2, A2, b2, p.zero_dim, p.nonneg_dim, p.soc_dims)
+
+
+def test_batch_update_data_new_P_and_A_values(oracle):
+    """Solver::updateData(P, q, A, b) on every problem of a batch at once
+    (solver_impl.cpp:57-91): new values on setup's pattern are scaled by the
+    setup's equilibration and used by the next solve.  Problems given new data
+    against the oracle running the same setup + updateData sequence; problems
+    left out keep their old data and results; a pattern mismatch in any
+    problem is rejected before anything changes."""
+    probs = [PR.config2_problem(i) for i in range(6)] + [PR.config3_problem(i) for i in (0, 1)]
+    b = batch_of(probs, Settings(warm_start=False))
+    b.solve()
+    before = [b.solution(k) for k in range(len(probs))]
+    rng = np.random.default_rng(77)
+    upd = {k: _new_values(probs[k], rng) for k in range(0, len(probs), 2)}
+    Ps = [upd[k].P if k in upd else None for k in range(len(probs))]
+    As = [upd[k].A if k in upd else None for k in range(len(probs))]
+    qs = [upd[k].q if k in upd else None for k in range(len(probs))]
+    bs = [upd[k].b if k in upd else None for k in range(len(probs))]
+    # a pattern mismatch in one problem: nothing changes
+    bad = Ps[2].copy().tolil()
+    bad[0, 1] = 0.0 if bad[0, 1] != 0 else 1.0
+    bad = bad.tocsc()
+    bad.eliminate_zeros()
+    assert bad.nnz != Ps[2].nnz
+    assert not b.update_data(qs, bs, Ps[:2] + [bad] + Ps[3:], As)
+    b.solve()
+    for k in range(len(probs)):
+        assert np.array_equal(b.solution(k).x, before[k].x), k
+    assert b.update_data(qs, bs, Ps, As)
+    st = b.solve()
+    for k, p in enumerate(probs):
+        sol = b.solution(k)
+        if k not in upd:
+            assert np.array_equal(sol.x, before[k].x), k
+            continue
+        p2 = upd[k]
+        runs = []
+        rng2 = np.random.default_rng(12345)
+        for r in range(4):
+            o = oracle.OracleSolver()
+            assert o.setup(p.P, p.q, p.A, p.b, p.zero_dim, p.nonneg_dim, p.soc_dims, warm_start=0)
+            o.solve()
+            qk = p2.q * (1.0 + (1e-15 * rng2.standard_normal(p.n) if r else 0.0))
+            assert o.update_data(P=p2.P, q=qk, A=p2.A, b=p2.b)
+            runs.append(o.solve())
+        print(k, "batch", sol.status.name, sol.info.iterations, "oracle", [r["status"] for r in runs])
+        assert int(st[k]) == int(sol.status)
+        assert_parity(sol, runs, p2)
