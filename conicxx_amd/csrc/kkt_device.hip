@@ -781,14 +781,17 @@ __global__ __launch_bounds__(256) void k_fwd(
     const int* __restrict__ rowptr, const int* __restrict__ gptr, const int* __restrict__ gidx,
     const long* __restrict__ uofs, const int* __restrict__ child_ptr, const int* __restrict__ child,
     const int* __restrict__ nblk, const int* __restrict__ flag_ofs, int* flags, int* done, int gen,
-    const T* __restrict__ F, double* xp, double* yh, double* U, int* err) {
+    const T* __restrict__ F, double* xp, double* yh, double* U, int* err, SweepCtx C) {
   __shared__ int tk;
   __shared__ double part[4][SB];
   __shared__ double Ld[SB * LDSP];  // Ld[row][col] of the diagonal block
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   if (tid == 0) tk = atomicAdd(ticket, 1);
   __syncthreads();
-  const SolveTask Tk = tasks[tk];
+  const int ctx = tk % C.nctx;
+  const SolveTask Tk = tasks[tk / C.nctx];
+  xp += ctx * C.vstride, yh += ctx * C.hstride, U += ctx * C.ustride;
+  flags += ctx * C.fstride, done += ctx * C.dstride;
   const int s = Tk.s, rb = Tk.blk, nr = nrv[s], nc = ncv[s], first = firstv[s];
   const long base = fofs[s];
   const int r0 = rb * SB;
@@ -939,7 +942,7 @@ __global__ __launch_bounds__(256) void k_bwd(
     const int* __restrict__ nrv, const int* __restrict__ ncv, const int* __restrict__ firstv,
     const int* __restrict__ rowptr, const int* __restrict__ rows, const int* __restrict__ parent,
     const int* __restrict__ nblkb, const int* __restrict__ flag_ofs, int* flags, int* done, int gen,
-    const T* __restrict__ F, const double* __restrict__ Dg, double* xp, double* xh, int* err) {
+    const T* __restrict__ F, const double* __restrict__ Dg, double* xp, double* xh, int* err, SweepCtx C) {
   // one block in a fixed order: the broadcast vectors below 64 KB, where
   // their reads take immediate offsets
   struct Lds {
@@ -958,7 +961,10 @@ __global__ __launch_bounds__(256) void k_bwd(
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   if (tid == 0) tk = atomicAdd(ticket, 1);
   __syncthreads();
-  const SolveTask Tk = tasks[tk];
+  const int ctx = tk % C.nctx;
+  const SolveTask Tk = tasks[tk / C.nctx];
+  xp += ctx * C.vstride, xh += ctx * C.hstride;
+  flags += ctx * C.fstride, done += ctx * C.dstride;
   const int s = Tk.s, cb = Tk.blk, nr = nrv[s], nc = ncv[s], first = firstv[s];
   const long base = fofs[s];
   const int c0 = cb * SB, kc = min(SB, nc - c0);
@@ -1259,14 +1265,16 @@ void DeviceKKT::setup(const LowerPattern& K, int nx, hipStream_t stream) {
   nblk_.upload(nblk, st_);
   nblkb_.upload(nblkb, st_);
   parent_.upload(P.sn_parent, st_);
-  flags_.alloc((size_t)std::max(1, flag_ofs[ns]) * FSTRIDE);
+  // two sweep contexts (solve2_async): flags, done counters, work vectors,
+  // hand-off buffers and update vectors twice over
+  flags_.alloc(2 * (size_t)std::max(1, flag_ofs[ns]) * FSTRIDE);
   flags_.zero(st_);
-  done_.alloc((size_t)(2 * ns + 2) * FSTRIDE);
+  done_.alloc((size_t)(4 * ns + 2) * FSTRIDE);
   serr_.alloc(1);
   serr_.zero(st_);
-  xp_.alloc(N);
-  hand_.alloc(2 * (size_t)std::max(1, N));
-  U_.alloc(std::max<long>(1, uofs[ns]));
+  xp_.alloc(2 * (size_t)std::max(1, N));
+  hand_.alloc(4 * (size_t)std::max(1, N));
+  U_.alloc(2 * std::max<long>(1, uofs[ns]));
   gen_ = 0;
 
   // --- full symmetric CSR of K for the refinement residual ---
@@ -1566,37 +1574,61 @@ FactorStats DeviceKKT::factor_stats() {
 }
 
 void DeviceKKT::solve_async(const double* rhs, double* x) {
+  const double* r[1] = {rhs};
+  double* xo[1] = {x};
+  sweeps(1, r, xo);
+}
+
+void DeviceKKT::solve2_async(const double* rhs0, double* x0, const double* rhs1, double* x1) {
+  const double* r[2] = {rhs0, rhs1};
+  double* xo[2] = {x0, x1};
+  sweeps(2, r, xo);
+}
+
+// Forward and backward sweeps over nctx right-hand sides.  Counter layout:
+// forward done counters of context c at [c ns, (c+1) ns), backward ones at
+// [(2 + c) ns, (3 + c) ns), then the two tickets (FSTRIDE ints each).
+void DeviceKKT::sweeps(int nctx, const double* const* rhs, double* const* x) {
   const int N = P_.N, ns = P_.ns;
   if (N == 0) return;
-  const int nctr = (2 * ns + 2) * FSTRIDE;
+  const SweepCtx C{nctx, (long)std::max(1, N), 2L * std::max(1, N), (long)(U_.n / 2), (long)(flags_.n / 2),
+                   (long)ns * FSTRIDE};
+  const int nctr = (4 * ns + 2) * FSTRIDE;
   hipLaunchKernelGGL(k_zero_int, dim3(std::min(1024, ceil_div(nctr, 256))), dim3(256), 0, st_, done_.p, nctr);
-  hipLaunchKernelGGL(k_perm_in, dim3(ceil_div(N, 256)), dim3(256), 0, st_, perm_.p, rhs, xp_.p, hand_.p, N);
-  int* tick = done_.p + (size_t)2 * ns * FSTRIDE;
+  for (int c = 0; c < nctx; ++c)
+    hipLaunchKernelGGL(k_perm_in, dim3(ceil_div(N, 256)), dim3(256), 0, st_, perm_.p, rhs[c], xp_.p + c * C.vstride,
+                       hand_.p + c * C.hstride, N);
+  int* tick = done_.p + (size_t)4 * ns * FSTRIDE;
+  int* done_b = done_.p + (size_t)2 * ns * FSTRIDE;
   ++gen_;
   sample_ = timing_ && sweep_seq_++ % timing_every_ == 0;
+  const double bytes = sweep_bytes_ + (nctx - 1) * 24.0 * N;  // L once, the vectors per context
   tbegin();
   if (prec_ == 1)
-    hipLaunchKernelGGL(k_fwd<float>, dim3(nfwd_), dim3(256), 0, st_, fwd_.p, tick, fofs_.p, nr_.p, nc_.p,
+    hipLaunchKernelGGL(k_fwd<float>, dim3(nfwd_ * nctx), dim3(256), 0, st_, fwd_.p, tick, fofs_.p, nr_.p, nc_.p,
                        first_.p, rowptr_.p, gptr_.p, gidx_.p, uofs_.p, child_ptr_.p, child_.p, nblk_.p,
-                       flag_ofs_.p, flags_.p, done_.p, gen_, reinterpret_cast<const float*>(F_.p), xp_.p, hand_.p, U_.p, serr_.p);
+                       flag_ofs_.p, flags_.p, done_.p, gen_, reinterpret_cast<const float*>(F_.p), xp_.p, hand_.p,
+                       U_.p, serr_.p, C);
   else
-    hipLaunchKernelGGL(k_fwd<double>, dim3(nfwd_), dim3(256), 0, st_, fwd_.p, tick, fofs_.p, nr_.p, nc_.p,
+    hipLaunchKernelGGL(k_fwd<double>, dim3(nfwd_ * nctx), dim3(256), 0, st_, fwd_.p, tick, fofs_.p, nr_.p, nc_.p,
                        first_.p, rowptr_.p, gptr_.p, gidx_.p, uofs_.p, child_ptr_.p, child_.p, nblk_.p,
-                       flag_ofs_.p, flags_.p, done_.p, gen_, (const double*)F_.p, xp_.p, hand_.p, U_.p, serr_.p);
-  tend(T_FWD, sweep_bytes_);
+                       flag_ofs_.p, flags_.p, done_.p, gen_, (const double*)F_.p, xp_.p, hand_.p, U_.p, serr_.p, C);
+  tend(T_FWD, bytes);
   ++gen_;
   tbegin();
   if (prec_ == 1)
-    hipLaunchKernelGGL(k_bwd<float>, dim3(nbwd_), dim3(256), 0, st_, bwd_.p, tick + FSTRIDE, fofs_.p, nr_.p,
-                     nc_.p, first_.p, rowptr_.p, rows_.p, parent_.p, nblkb_.p, flag_ofs_.p, flags_.p,
-                     done_.p + (size_t)ns * FSTRIDE, gen_, reinterpret_cast<const float*>(F_.p), D_.p, xp_.p, hand_.p + N, serr_.p);
+    hipLaunchKernelGGL(k_bwd<float>, dim3(nbwd_ * nctx), dim3(256), 0, st_, bwd_.p, tick + FSTRIDE, fofs_.p, nr_.p,
+                       nc_.p, first_.p, rowptr_.p, rows_.p, parent_.p, nblkb_.p, flag_ofs_.p, flags_.p, done_b,
+                       gen_, reinterpret_cast<const float*>(F_.p), D_.p, xp_.p, hand_.p + N, serr_.p, C);
   else
-    hipLaunchKernelGGL(k_bwd<double>, dim3(nbwd_), dim3(256), 0, st_, bwd_.p, tick + FSTRIDE, fofs_.p, nr_.p,
-                     nc_.p, first_.p, rowptr_.p, rows_.p, parent_.p, nblkb_.p, flag_ofs_.p, flags_.p,
-                     done_.p + (size_t)ns * FSTRIDE, gen_, (const double*)F_.p, D_.p, xp_.p, hand_.p + N, serr_.p);
-  tend(T_BWD, sweep_bytes_);
+    hipLaunchKernelGGL(k_bwd<double>, dim3(nbwd_ * nctx), dim3(256), 0, st_, bwd_.p, tick + FSTRIDE, fofs_.p, nr_.p,
+                       nc_.p, first_.p, rowptr_.p, rows_.p, parent_.p, nblkb_.p, flag_ofs_.p, flags_.p, done_b,
+                       gen_, (const double*)F_.p, D_.p, xp_.p, hand_.p + N, serr_.p, C);
+  tend(T_BWD, bytes);
   sample_ = false;
-  hipLaunchKernelGGL(k_perm_out, dim3(ceil_div(N, 256)), dim3(256), 0, st_, perm_.p, xp_.p, x, N);
+  for (int c = 0; c < nctx; ++c)
+    hipLaunchKernelGGL(k_perm_out, dim3(ceil_div(N, 256)), dim3(256), 0, st_, perm_.p, xp_.p + c * C.vstride, x[c],
+                       N);
   CX_LAUNCH_CHECK();
 }
 

@@ -28,6 +28,10 @@ struct AsmSeg { int child, i0, i1; };
 struct PanelTask { int s, a, w, r0, r1, blk; };
 struct TrailTask { int s, ti, tj, k0, k1, rlo, chi; };
 struct SolveTask { int s, blk; };
+// Interleaved right-hand sides of one sweep launch: ticket t runs task t / nctx
+// of context t % nctx, each context with its own work vector, hand-off
+// buffer, update vectors, block flags and done counters at these strides.
+struct SweepCtx { int nctx; long vstride, hstride, ustride, fstride, dstride; };
 
 enum LaunchKind { L_ASSEMBLE = 0, L_SMALL = 1, L_PANEL = 3, L_TRAIL = 4 };
 struct Launch { int kind, beg, end; };
@@ -56,6 +60,9 @@ class DeviceKKT {
 
   // x = K^-1 rhs with the current factorization (device vectors of length N).
   void solve_async(const double* rhs, double* x);
+  // x0 = K^-1 rhs0 and x1 = K^-1 rhs1 in one launch per sweep (the two
+  // contexts' tasks interleaved); each result equals solve_async's
+  void solve2_async(const double* rhs0, double* x0, const double* rhs1, double* x1);
   // y = K x, K symmetric from its lower values Kx.
   void spmv_async(const double* Kx, const double* x, double* y);
   // nonzero if a bounded wait in a solve sweep timed out (synchronizes)
@@ -83,6 +90,7 @@ class DeviceKKT {
 
  private:
   void build_schedule();
+  void sweeps(int nctx, const double* const* rhs, double* const* x);
   template <typename T>
   void launch_factor(T* F, const double* Kx, double extra_p, double extra_a, int mode, double eps,
                      double delta);

@@ -309,7 +309,8 @@ bool Solver::setup(int n, int m, const int* Pp, const int* Pi, const double* Px,
     v->alloc(std::max(1, n)), v->zero(stream_);
   for (DevBuf<double>* v : {&s_, &z_, &rz_, &z1_, &dza_, &dsa_, &dz_, &ds_, &ws_, &wz_, &wm_, &wm2_, &dsk_})
     v->alloc(std::max(1, m)), v->zero(stream_);
-  for (DevBuf<double>* v : {&rhs_, &sol_, &r_, &t_, &dsol_}) v->alloc(std::max(1, N_)), v->zero(stream_);
+  for (DevBuf<double>* v : {&rhs_, &sol_, &r_, &t_, &dsol_, &rhs2_, &sol2_, &r2_, &t2_, &dsol2_})
+    v->alloc(std::max(1, N_)), v->zero(stream_);
 
   // updateData(P, A) then updateScalingAndFactorize with the constructor scaling
   if (!kkt_update_pa(true, true)) return false;
@@ -487,6 +488,83 @@ double Solver::kkt_solve(const double* rhs, double* x) {
     return kkt_solve(rhs, x);
   }
   return rel;
+}
+
+// The constant and affine solves of one iteration (kkt_solve on [-q; b] and
+// on [rx; s - rz]) have independent right-hand sides: both go through one
+// launch per sweep, and their refinement steps are paired while both still
+// refine.  Each side takes exactly the steps kkt_solve takes on it alone, so
+// the results are the same.  The first side gets kkt_solve's escalation; the
+// second is returned (true) only when it came out finite on the factorization
+// the first ended with: otherwise the caller solves it again with kkt_solve,
+// after the convergence tests, where the reference solves it
+// (solver_impl.cpp:165-231, kkt_system.cpp:266-311).  FP64 factorizations only.
+bool Solver::kkt_solve_pair(const double* rhs0, double* x0, const double* rhs1, double* x1) {
+  const double t0 = now_s();
+  kkt_.solve2_async(rhs0, x0, rhs1, x1);
+  kkt_.spmv_async(Kx_.p, x0, t_.p);
+  launch_lin2(N_, 1.0, rhs0, -1.0, t_.p, r_.p, stream_);
+  kkt_.spmv_async(Kx_.p, x1, t2_.p);
+  launch_lin2(N_, 1.0, rhs1, -1.0, t2_.p, r2_.p, stream_);
+  RedBatch B{};
+  B.op[B.nops++] = nonfin_op(x0, N_);
+  B.op[B.nops++] = dot_op(rhs0, rhs0, N_);
+  B.op[B.nops++] = dot_op(r_.p, r_.p, N_);
+  B.op[B.nops++] = nonfin_op(x1, N_);
+  B.op[B.nops++] = dot_op(rhs1, rhs1, N_);
+  B.op[B.nops++] = dot_op(r2_.p, r2_.p, N_);
+  std::vector<double> v = reduce(B);
+  prof_nsolve_ += 2;
+  if (kkt_.take_solve_error()) {
+    launch_fill(x0, N_, NAN, stream_);
+    prof_solve_ms_ += (now_s() - t0) * 1e3;
+    return false;
+  }
+  const double rn0 = std::max(std::sqrt(v[1]), 1e-30), rn1 = std::max(std::sqrt(v[4]), 1e-30);
+  double rr0 = v[2];
+  bool ok1 = v[3] == 0.0;
+  if (v[0] != 0.0) {
+    escalate(rhs0, x0);  // a new factorization: the second side is solved again later
+    ok1 = false;
+    kkt_.spmv_async(Kx_.p, x0, t_.p);
+    launch_lin2(N_, 1.0, rhs0, -1.0, t_.p, r_.p, stream_);
+    RedBatch B2{};
+    B2.op[B2.nops++] = dot_op(r_.p, r_.p, N_);
+    rr0 = reduce(B2)[0];
+  }
+  double rel0 = std::sqrt(rr0) / rn0, rel1 = std::sqrt(v[5]) / rn1;
+  for (int it0 = 0, it1 = 0;;) {
+    const bool a0 = it0 < st_.refine_max_iter && rel0 > st_.refine_tol;
+    const bool a1 = ok1 && it1 < st_.refine_max_iter && rel1 > st_.refine_tol;
+    if (!a0 && !a1) break;
+    if (a0 && a1) {
+      kkt_.solve2_async(r_.p, dsol_.p, r2_.p, dsol2_.p);
+      prof_nsolve_ += 2;
+    } else {
+      kkt_.solve_async(a0 ? r_.p : r2_.p, a0 ? dsol_.p : dsol2_.p);
+      prof_nsolve_++;
+    }
+    RedBatch B3{};
+    if (a0) {
+      launch_axpby(N_, 1.0, dsol_.p, 1.0, x0, stream_);
+      kkt_.spmv_async(Kx_.p, x0, t_.p);
+      launch_lin2(N_, 1.0, rhs0, -1.0, t_.p, r_.p, stream_);
+      B3.op[B3.nops++] = dot_op(r_.p, r_.p, N_);
+    }
+    if (a1) {
+      launch_axpby(N_, 1.0, dsol2_.p, 1.0, x1, stream_);
+      kkt_.spmv_async(Kx_.p, x1, t2_.p);
+      launch_lin2(N_, 1.0, rhs1, -1.0, t2_.p, r2_.p, stream_);
+      B3.op[B3.nops++] = dot_op(r2_.p, r2_.p, N_);
+    }
+    std::vector<double> w = reduce(B3);
+    if (a0) rel0 = std::sqrt(w[0]) / rn0, ++it0;
+    if (a1) rel1 = std::sqrt(w[a0 ? 1 : 0]) / rn1, ++it1;
+  }
+  prof_max_rel_ = std::max(prof_max_rel_, rel0);
+  if (ok1) prof_max_rel_ = std::max(prof_max_rel_, rel1);
+  prof_solve_ms_ += (now_s() - t0) * 1e3;
+  return ok1;
 }
 
 // Restarted GMRES (classical Gram-Schmidt applied twice, Givens rotations) on
@@ -891,11 +969,19 @@ bool Solver::initial_point() {  // solver_impl.cpp:126-158
 }
 
 bool Solver::constant_solve() {  // solver_impl.cpp:165-175
-  launch_lin2(n_, -1.0, dq_.p, 0.0, dq_.p, rhs_.p, stream_);
-  launch_copy(m_, db_.p, rhs_.p + n_, stream_);
+  constant_rhs(rhs_.p);
   kkt_solve(rhs_.p, sol_.p);
-  launch_copy(n_, sol_.p, x1_.p, stream_);
-  launch_copy(m_, sol_.p + n_, z1_.p, stream_);
+  return constant_finish(sol_.p);
+}
+
+void Solver::constant_rhs(double* rhs) {  // [-q; b]
+  launch_lin2(n_, -1.0, dq_.p, 0.0, dq_.p, rhs, stream_);
+  launch_copy(m_, db_.p, rhs + n_, stream_);
+}
+
+bool Solver::constant_finish(const double* sol) {
+  launch_copy(n_, sol, x1_.p, stream_);
+  launch_copy(m_, sol + n_, z1_.p, stream_);
   pmul(x1_.p, Px1_.p);
   return count_nonfinite(x1_.p, n_, z1_.p, m_) == 0;
 }
@@ -941,14 +1027,22 @@ bool Solver::tau_elim(const double* xv, const double* zv, double dtau_rhs, doubl
 }
 
 bool Solver::affine_step() {  // solver_impl.cpp:192-231
-  launch_copy(n_, rx_.p, rhs_.p, stream_);
-  launch_lin2(m_, 1.0, s_.p, -1.0, rz_.p, rhs_.p + n_, stream_);
+  affine_rhs(rhs_.p);
   kkt_solve(rhs_.p, sol_.p);
-  if (count_nonfinite(sol_.p, N_) != 0) return false;
+  return affine_finish(sol_.p);
+}
+
+void Solver::affine_rhs(double* rhs) {  // [rx; s - rz]
+  launch_copy(n_, rx_.p, rhs, stream_);
+  launch_lin2(m_, 1.0, s_.p, -1.0, rz_.p, rhs + n_, stream_);
+}
+
+bool Solver::affine_finish(const double* sol) {
+  if (count_nonfinite(sol, N_) != 0) return false;
   const double dtau_rhs = rtau_, dkappa_rhs = tau_ * kappa_;
-  if (!tau_elim(sol_.p, sol_.p + n_, dtau_rhs, dkappa_rhs, &dtaua_)) return false;
-  launch_lin2(n_, 1.0, sol_.p, dtaua_, x1_.p, dxa_.p, stream_);
-  launch_lin2(m_, 1.0, sol_.p + n_, dtaua_, z1_.p, dza_.p, stream_);
+  if (!tau_elim(sol, sol + n_, dtau_rhs, dkappa_rhs, &dtaua_)) return false;
+  launch_lin2(n_, 1.0, sol, dtaua_, x1_.p, dxa_.p, stream_);
+  launch_lin2(m_, 1.0, sol + n_, dtaua_, z1_.p, dza_.p, stream_);
   launch_neg_hs_plus(cone_, m_, dza_.p, s_.p, dsa_.p, stream_);
   dkappaa_ = -(dkappa_rhs + kappa_ * dtaua_) / tau_;
   return std::isfinite(dtaua_) && std::isfinite(dkappaa_) &&
@@ -1050,11 +1144,28 @@ int Solver::solve() {  // solver_impl.cpp:367-466
       iterations = iter + 1;
       iterations_ = iter;  // current iteration, for the profile
       launch_cone_scaling(cone_, s_.p, z_.p, stream_);
-      if (!kkt_update_scaling() || !constant_solve()) {
+      if (!kkt_update_scaling()) {
         final_status = CX_NUMERICAL_ERROR;
         break;
       }
+      // The residuals do not use the constant solve, so they come first; with
+      // an FP64 factorization the affine solve (whose right-hand side they
+      // give) is then paired with the constant solve
       residuals();
+      bool affine_ready = false;
+      bool const_ok;
+      if (fp32_) {
+        const_ok = constant_solve();
+      } else {
+        constant_rhs(rhs_.p);
+        affine_rhs(rhs2_.p);
+        affine_ready = kkt_solve_pair(rhs_.p, sol_.p, rhs2_.p, sol2_.p);
+        const_ok = constant_finish(sol_.p);
+      }
+      if (!const_ok) {
+        final_status = CX_NUMERICAL_ERROR;
+        break;
+      }
       const double mu_now = mu();
       if (st_.verbose >= 1) {
         std::fprintf(stderr, "[conicxx_amd] iter %3d mu %.3e |rx| %.3e |rz| %.3e t %.3fs (fact %.1f ms, solve %.1f ms)\n",
@@ -1089,7 +1200,7 @@ int Solver::solve() {  // solver_impl.cpp:367-466
           }
         }
       }
-      if (!affine_step()) {
+      if (!(affine_ready ? affine_finish(sol2_.p) : affine_step())) {
         final_status = CX_NUMERICAL_ERROR;
         break;
       }

@@ -58,6 +58,9 @@ class Solver {
   bool escalate(const double* rhs, double* x);
   void leave_fp32(double rel);  // FP64 factorizations for the rest of this solve()
   double kkt_solve(const double* rhs, double* x);
+  // kkt_solve on two independent right-hand sides with their sweeps paired;
+  // returns whether the second result is valid (see solver.hip)
+  bool kkt_solve_pair(const double* rhs0, double* x0, const double* rhs1, double* x1);
   // GMRES on K with the FP32 factorization as right preconditioner, from x
   // with residual r_ (relative residual rel); returns the final relative residual
   double gmres_refine(const double* rhs, double* x, double rn, double rel);
@@ -66,9 +69,13 @@ class Solver {
   void shift_cold(double* v);
   void ensure_interior_warm(double* v);
   bool constant_solve();
+  void constant_rhs(double* rhs);
+  bool constant_finish(const double* sol);
   void residuals();
   bool tau_elim(const double* xv, const double* zv, double dtau_rhs, double dkappa_rhs, double* dtau);
   bool affine_step();
+  void affine_rhs(double* rhs);
+  bool affine_finish(const double* sol);
   bool combined_step(double sigma, double mu);
   double step_length(const double* ds, const double* dz, double dtau, double dkappa);
   void finalize();
@@ -121,6 +128,7 @@ class Solver {
   DevBuf<double> x_, s_, z_, Px_, rx_, rz_, x1_, z1_, Px1_, dxa_, dza_, dsa_, dx_, dz_, ds_;
   DevBuf<double> wx_, ws_, wz_;
   DevBuf<double> rhs_, sol_, r_, t_, dsol_, wn_, wn2_, wm_, wm2_, dsk_;
+  DevBuf<double> rhs2_, sol2_, r2_, t2_, dsol2_;  // second side of kkt_solve_pair
   DevBuf<double> red_partial_, red_out_;
   double* host_red_ = nullptr;
   double tau_ = 1.0, kappa_ = 1.0;
