@@ -246,6 +246,135 @@ __device__ __forceinline__ void ldl_wave_steps(std::integer_sequence<int, Ks...>
   (ldl_wave_step<T, Ks>(a, nc, sg, lane, mode, eps, delta, stats, count, colk), ...);
 }
 
+// pivot_fix with its statistics accumulated in registers (written once, by
+// ldl_wave_full, after the last step): no branch or atomic in a step
+struct PivotAcc {
+  int zero = 0, stop = 0, nreg = 0;
+};
+__device__ inline double pivot_rule(double d, double sgn, int mode, double eps, double delta, PivotAcc& pa) {
+  if (mode == PIVOT_REGULARIZE) {  // regularizePivot: src/conicxx/kkt/regularized_ldlt.cpp:132-139
+    const bool ok = d * sgn >= eps;
+    double dd = d + sgn * delta;
+    dd = (dd * sgn < eps) ? sgn * delta : dd;
+    pa.nreg += ok ? 0 : 1;
+    return ok ? d : dd;
+  }
+  pa.zero |= d == 0.0 ? 1 : 0;
+  pa.stop |= !(fabs(d) > eps) ? 1 : 0;
+  return d;
+}
+
+// The 64 steps of a full block (n = nc = 64) with the same operations, in
+// the same order per entry, as ldl_wave_step, software-pipelined for latency.
+// Step K is split into
+//   A: the pivot d (readlane of lane K, pivot_rule) and the multiplier
+//      l = c / d, with c the lane's entry (lane, K) and the next two columns'
+//      entries c1, c2 taken from lanes K+1, K+2 by readlane;
+//   B: the entries of columns K+1 and K+2 (the next pivots' columns), and c
+//      written to one of two LDS buffers for the far columns;
+//   D: the far columns K+3..63, reading c from LDS in chunks of 16, the next
+//      chunk's reads issued before the current chunk's FMAs.
+// They are emitted A0 B0 | A1 D0 B1 | A2 D1 B2 | ...: step K+1's pivot and
+// division (which need only B of step K) run while step K's far FMAs issue.
+// Every entry still takes its updates in step order (column K+2 takes D of
+// step K-1 before B of step K), so the factor is the same.
+template <typename T>
+struct LStep {
+  T d, c, c1, c2, l;
+};
+
+// An empty asm statement with the value as a register in/out operand: no
+// instruction, but the value must exist at this point, so the compiler cannot
+// sink the update that produced it down to its last use (which would keep
+// every step's multiplier and column values live at once).
+template <typename T>
+__device__ __forceinline__ void pin(T& v) {
+  __asm__ volatile("" : "+v"(v));
+}
+
+template <typename T, int K>
+__device__ __forceinline__ void lstep_a(T (&a)[64], double sg, int mode, double eps, double delta, PivotAcc& pa,
+                                        LStep<T>& s) {
+  s.d = (T)pivot_rule((double)lane_bcast(a[K], K), lane_bcast(sg, K), mode, eps, delta, pa);
+  s.c = a[K];
+  s.c1 = s.c2 = T(0);
+  if constexpr (K + 1 < 64) s.c1 = lane_bcast(s.c, K + 1);
+  if constexpr (K + 2 < 64) s.c2 = lane_bcast(s.c, K + 2);
+  s.l = s.c / s.d;
+  pin(s.l);  // the division stays in the region of the previous step's far FMAs
+}
+
+template <typename T, int K>
+__device__ __forceinline__ void lstep_b(T (&a)[64], int lane, const LStep<T>& s, T* colk) {
+  a[K] = (lane == K) ? s.d : (lane > K ? s.l : a[K]);
+  pin(a[K]);
+  if constexpr (K + 1 < 64) a[K + 1] -= s.l * s.c1, pin(a[K + 1]);
+  if constexpr (K + 2 < 64) a[K + 2] -= s.l * s.c2, pin(a[K + 2]);
+  if constexpr (K + 3 < 64) colk[(K & 1) * 64 + lane] = s.c;
+}
+
+// Group K (1 <= K < 64): A of step K, D of step K-1, B of step K.  The
+// first chunk of D's column values was read at the end of the previous group
+// (v0), so A's division shares its scheduling region with FMAs whose
+// operands are already in registers; the group ends by reading the first
+// chunk of its own step's far columns for the next group.
+template <typename T, int K>
+__device__ __forceinline__ void lstep_group(T (&a)[64], double sg, int lane, int mode, double eps, double delta,
+                                            PivotAcc& pa, T* colk, LStep<T>& prev, T (&v0)[16]) {
+  constexpr int J = K + 2;  // first far column of step K-1
+  const T* cb = colk + ((K - 1) & 1) * 64;
+  LStep<T> cur;
+  lstep_a<T, K>(a, sg, mode, eps, delta, pa, cur);
+  if constexpr (J < 64) {
+    T v[2][16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) v[0][q] = v0[q];
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      const int j0 = J + 16 * g;
+      if (j0 < 64) {
+#pragma unroll
+        for (int q = 0; q < 16; ++q)
+          if (j0 + 16 + q < 64) v[(g + 1) & 1][q] = cb[j0 + 16 + q];
+#pragma unroll
+        for (int q = 0; q < 16; ++q)
+          if (j0 + q < 64) a[j0 + q] -= prev.l * v[g & 1][q], pin(a[j0 + q]);
+        // one chunk in flight at a time
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+  }
+  lstep_b<T, K>(a, lane, cur, colk);
+  prev = cur;
+  if constexpr (K + 3 < 64) {
+    const T* nb = colk + (K & 1) * 64;
+#pragma unroll
+    for (int q = 0; q < 16; ++q)
+      if (K + 3 + q < 64) v0[q] = nb[K + 3 + q];
+  }
+  __builtin_amdgcn_sched_barrier(0);
+}
+
+template <typename T, int... Ks>
+__device__ __forceinline__ void lstep_groups(std::integer_sequence<int, Ks...>, T (&a)[64], double sg, int lane,
+                                             int mode, double eps, double delta, PivotAcc& pa, T* colk,
+                                             LStep<T>& prev, T (&v0)[16]) {
+  (lstep_group<T, Ks + 1>(a, sg, lane, mode, eps, delta, pa, colk, prev, v0), ...);
+}
+
+template <typename T>
+__device__ __forceinline__ void ldl_full(T (&a)[64], double sg, int lane, int mode, double eps, double delta,
+                                         PivotAcc& pa, T* colk) {
+  LStep<T> prev;
+  T v0[16];
+  lstep_a<T, 0>(a, sg, mode, eps, delta, pa, prev);
+  lstep_b<T, 0>(a, lane, prev, colk);
+#pragma unroll
+  for (int q = 0; q < 16; ++q) v0[q] = colk[3 + q];
+  __builtin_amdgcn_sched_barrier(0);
+  lstep_groups<T>(std::make_integer_sequence<int, 63>{}, a, sg, lane, mode, eps, delta, pa, colk, prev, v0);
+}
+
 // The same partial LDL^T as ldl_lds for n <= 64 rows, by one wave with no
 // workgroup barrier: lane i keeps row i in registers, pivot k is read from
 // lane k, and the unscaled column k goes through LDS (colk) so that every lane
@@ -255,14 +384,33 @@ template <typename T>
 __device__ __forceinline__ void ldl_wave(T* A, int n, int nc, const double* sgn, int mode, double eps, double delta,
                          int* stats, bool count, T* colk) {
   const int lane = threadIdx.x & 63;
-  T a[64];
-#pragma unroll
-  for (int j = 0; j < 64; ++j) a[j] = (lane < n && j <= lane) ? A[lane * LDSP + j] : T(0);
   const double sg = lane < nc ? sgn[lane] : 1.0;  // pivot signs, read from lane k at step k
-  ldl_wave_steps<T>(std::make_integer_sequence<int, 64>{}, a, nc, sg, lane, mode, eps, delta, stats, count, colk);
+  // each path loads and stores the block itself: no register array lives
+  // across the branch
+  if (n == 64 && nc == 64) {
+    // a full block: colk holds two 64-entry buffers
+    T a[64];
 #pragma unroll
-  for (int j = 0; j < 64; ++j)
-    if (lane < n && j <= lane) A[lane * LDSP + j] = a[j];
+    for (int j = 0; j < 64; ++j) a[j] = (j <= lane) ? A[lane * LDSP + j] : T(0);
+    PivotAcc pa;
+    ldl_full<T>(a, sg, lane, mode, eps, delta, pa, colk);
+    if (count && lane == 0) {
+      if (pa.zero) atomicOr(&stats[0], 1);
+      if (pa.nreg) atomicAdd(&stats[1], pa.nreg);
+      if (pa.stop) atomicOr(&stats[2], 1);
+    }
+#pragma unroll
+    for (int j = 0; j < 64; ++j)
+      if (j <= lane) A[lane * LDSP + j] = a[j];
+  } else {
+    T a[64];
+#pragma unroll
+    for (int j = 0; j < 64; ++j) a[j] = (lane < n && j <= lane) ? A[lane * LDSP + j] : T(0);
+    ldl_wave_steps<T>(std::make_integer_sequence<int, 64>{}, a, nc, sg, lane, mode, eps, delta, stats, count, colk);
+#pragma unroll
+    for (int j = 0; j < 64; ++j)
+      if (lane < n && j <= lane) A[lane * LDSP + j] = a[j];
+  }
 }
 
 // Forward substitution with the unit-lower part of a 64 x 64 block held
@@ -430,7 +578,7 @@ __global__ __launch_bounds__(256) void k_panel(
   // one block in a fixed order, the broadcast column first: its reads then
   // take immediate LDS offsets (below 64 KB) instead of an address register each
   struct Lds {
-    T colk[NB];
+    T colk[2 * NB];
     T dinv[NB];
     T L11[NB * LDSP];
     T X[PANEL_ROWS * LDSP];
