@@ -32,8 +32,8 @@ os.environ.setdefault("CONICXX_KERNEL_TIMING", "4")
 
 HBM_PEAK_GBS = 8000.0
 # k_trail HBM bytes per launch on config 1 (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0
-# per the copy calibration, averaged over the launches of one solve)
-K_TRAIL_C1_TRAFFIC_BYTES = 3.17e8
+# per the copy calibration, averaged over the launches of setup + one solve)
+K_TRAIL_C1_TRAFFIC_BYTES = 2.41e8
 FP64_MFMA_PEAK_TFLOPS = 78.6
 
 
@@ -442,6 +442,14 @@ def main():
     setup_s = time.perf_counter() - t0
     assert ok, "setup failed"
     kkt = solver.kkt_stats()
+    # exact-structure work of one factorization (sum over columns of c_j (c_j - 1)
+    # with the true column counts of the final order): the algorithmic flops;
+    # kkt["flops"] counts the stored supernodal structure, explicit zeros of
+    # relaxed amalgamation included
+    from conicxx_amd import analyze_pattern
+    _, cc, _ = analyze_pattern(PR.kkt_lower_pattern(p))
+    cc = cc.astype(np.float64)
+    flops_exact = float((cc * (cc - 1.0)).sum())
     print(f"[bench] setup {setup_s:.2f} s, N={kkt['N']:.0f}, nnz(L)={kkt['nnz_L']:.3g}, "
           f"factor flops={kkt['flops']:.3g}, largest front={kkt['max_front']:.0f}", file=sys.stderr, flush=True)
     for w in range(args.warmup):
@@ -474,13 +482,22 @@ def main():
         avg_s = a["ms"] / max(1, a["launches"]) / 1e3
         per_launch = a["work"] / max(1, a["launches"])
         if dom == "trail":
-            ach = per_launch / avg_s / 1e12
+            # per-launch work counted on the stored structure, scaled to the
+            # exact structure by the whole factorization's ratio
+            ach_stored = per_launch / avg_s / 1e12
+            ach = ach_stored * flops_exact / kkt["flops"]
             roof = {"kernel": "k_trail (FP64 MFMA trailing update)", "bound": "mfma",
                     "achieved": ach, "peak": FP64_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
                     "frac": ach / FP64_MFMA_PEAK_TFLOPS,
-                    # HBM bytes per launch on this workload from the FETCH_SIZE / WRITE_SIZE
-                    # counter passes, scaled by a 1 GiB copy (profiles/k_trail_config1_traffic.md)
-                    "traffic": K_TRAIL_C1_TRAFFIC_BYTES, "traffic_unit": "bytes per launch (counters)"}
+                    "work": "exact-structure flops (true column counts), per launch: the stored-structure "
+                            f"count x {flops_exact / kkt['flops']:.4f} (exact / stored over the factorization)",
+                    "achieved_stored_structure": ach_stored,
+                    "frac_stored_structure": ach_stored / FP64_MFMA_PEAK_TFLOPS,
+                    # a constant: HBM bytes per launch on this workload from the FETCH_SIZE /
+                    # WRITE_SIZE counter passes, scaled by a 1 GiB copy, in the profile
+                    # summary k_trail_config1_traffic.md (not measured by this run)
+                    "traffic": K_TRAIL_C1_TRAFFIC_BYTES if p.n == 20000 else None,
+                    "traffic_unit": "bytes per launch (counter profile of this workload; not measured in this run)"}
         else:
             ach = per_launch / avg_s / 1e9 if per_launch else 0.0
             roof = {"kernel": f"k_{dom}", "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
@@ -510,7 +527,8 @@ def main():
                                "per step; the batched (config2) and fwd+bwd (config3) figures are in "
                                "'batched' and 'differentiation'",
                    "psd_repair": p.meta.get("psd_repair"), "kkt_N": kkt["N"], "nnz_L": kkt["nnz_L"],
-                   "factor_flops": kkt["flops"], "largest_front": kkt["max_front"],
+                   "factor_flops": kkt["flops"], "factor_flops_exact": flops_exact,
+                   "largest_front": kkt["max_front"],
                    "iterations": iters, "all_solved": valid},
         "roofline": roof,
         "kernel_ms_per_step": kernel_ms,
