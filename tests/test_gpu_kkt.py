@@ -231,6 +231,71 @@ def test_escalation_ladder_on_nonfinite_solve(oracle):
     assert not np.isfinite(rel) and not np.isfinite(relo)
 
 
+# The retry ladder, the escalation steps and the RegularizedLdlt cap on the
+# device, each against the closed form the oracle is pinned by
+# (tests/test_oracle.py): static regularization and refinement off, so the
+# solution is exactly a dense solve of K with the prescribed bump.  A nonnegative
+# row whose only A entry is an explicit zero (Hs = 1 at s = z = e) keeps m > 0.
+NO_REG = dict(static_reg_P=0.0, static_reg_A=0.0, refine_max_iter=0)
+
+
+def test_retry_ladder_first_bump_on_device():
+    """factorizeWithRetry (kkt_system.cpp:242-264): an exact zero pivot fails
+    the first factorization; the retry bumps by +-dynamic_reg_delta (1e-7)."""
+    P = sp.csc_matrix(np.diag([1.0, 0.0]))
+    P.eliminate_zeros()
+    A = sp.csc_matrix(np.array([[1.0, 0.0]]))
+    p = PR.Problem(P, np.zeros(2), A, np.zeros(1), 1, 0, [])
+    rhs = np.array([1.0, 2.0, 3.0])
+    x, rel, nf = setup(p, **NO_REG).kkt_factor_solve(rhs)
+    d = 1e-7
+    Kb = np.array([[1.0 + d, 0.0, 1.0], [0.0, d, 0.0], [1.0, 0.0, -d]])
+    print("ladder", x, nf)
+    assert nf == 2
+    # 1e-8: the elimination order decides rounding here (the -1e-7 row pivot
+    # eliminated first turns the other pivot into 1e7); a wrong bump moves
+    # x[1] = 2e7 by a factor of 10
+    np.testing.assert_allclose(x, np.linalg.solve(Kb, rhs), rtol=1e-8)
+
+
+def test_escalation_steps_by_1000_on_device():
+    """escalateAndResolve (kkt_system.cpp:266-294): the solve overflows on the
+    accepted factorization (pivot 1e-13); bumps max(1000 x last, 1e-7): 1e-7
+    still overflows, 1e-4 does not.  Three factorizations."""
+    P = sp.csc_matrix(np.array([[1e-13]]))
+    A = sp.csc_matrix((np.array([0.0]), (np.array([0]), np.array([0]))), shape=(1, 1))  # one explicit zero
+    p = PR.Problem(P, np.zeros(1), A, np.ones(1), 0, 1, [])
+    rhs = np.array([1e303, 1.0])
+    x, rel, nf = setup(p, **NO_REG).kkt_factor_solve(rhs)
+    print("escalation", x, nf)
+    assert nf == 3 and np.isfinite(x).all()
+    np.testing.assert_allclose(x, [1e303 / (1e-13 + 1e-4), 1.0 / (-1.0 - 1e-4)], rtol=1e-12)
+
+
+@pytest.mark.parametrize("nzero, capped", [(1, False), (2, True)])
+def test_regularized_backend_five_percent_cap_on_device(nzero, capped):
+    """RegularizedLdlt (regularized_ldlt.cpp:107-139, 251-253): 21 pivots; one
+    corrected to +dynamic_reg_delta (4.8%) is accepted, two (9.5%) exceed the
+    5% cap and the ladder bumps the diagonal instead."""
+    n = 20
+    dg = np.full(n, 1e-3)
+    dg[:nzero] = 0.0
+    P = sp.csc_matrix(np.diag(dg))
+    P.eliminate_zeros()
+    A = sp.csc_matrix((np.array([0.0]), (np.array([0]), np.array([n - 1]))), shape=(1, n))  # one explicit zero
+    p = PR.Problem(P, np.zeros(n), A, np.ones(1), 0, 1, [])
+    rhs = np.linspace(1.0, 2.0, n + 1)
+    x, rel, nf = setup(p, linear_solver=2, **NO_REG).kkt_factor_solve(rhs)
+    d = 1e-7
+    if capped:
+        expect = rhs / np.concatenate([dg + d, [-1.0 - d]])
+    else:
+        expect = rhs / np.concatenate([np.where(dg == 0.0, d, dg), [-1.0]])
+    print("cap", nzero, nf, x[:3], expect[:3])
+    assert nf == (2 if capped else 1)
+    np.testing.assert_allclose(x, expect, rtol=1e-12)
+
+
 def test_fp32_factorization_with_fp64_refinement(oracle):
     """Precision form on one KKT solve: the FP32 factorization preconditions
     FP64 GMRES on the exact K, which must reach refine_tol."""

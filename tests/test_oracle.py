@@ -255,6 +255,62 @@ def test_kkt_rank_deficient_equality_block(oracle):
     assert ok and np.all(np.isfinite(x)) and rel < 1e-6
 
 
+# Retry ladder, escalation and the RegularizedLdlt cap, each pinned by a
+# closed form: with static regularization and refinement off, the returned
+# solution is exactly a dense solve of K with the bump the rule prescribes,
+# and a different bump (or none) moves it by orders of magnitude.
+NO_REG = dict(static_reg_P=0.0, static_reg_A=0.0, refine_max_iter=0)
+
+
+def test_retry_ladder_first_bump(oracle):
+    """factorizeWithRetry (kkt_system.cpp:242-264): a zero pivot fails the
+    first factorization (QDLDL reports it), the retry bumps the P diagonal by
+    +dynamic_reg_delta and the cone diagonal by -dynamic_reg_delta (1e-7)."""
+    P = _sp([(0, 0, 1.0), (1, 1, 0.0)], (2, 2))  # variable 1 has an exact zero pivot
+    A = _sp([(0, 0, 1.0)], (1, 2))               # one zero-cone row on variable 0
+    rhs = np.array([1.0, 2.0, 3.0])
+    x, rel, ok = oracle.kkt_solve(P, A, 1, 0, [], rhs, **NO_REG)
+    assert ok
+    d = 1e-7
+    Kb = np.array([[1.0 + d, 0.0, 1.0], [0.0, d, 0.0], [1.0, 0.0, -d]])
+    np.testing.assert_allclose(x, np.linalg.solve(Kb, rhs), rtol=1e-12)
+    assert abs(x[1] - 2.0 / d) < 1e-3  # 2e7: a bump of 1e-6 would give 2e6
+
+
+def test_escalation_steps_by_1000(oracle):
+    """escalateAndResolve (kkt_system.cpp:266-294): a solve that overflows on
+    an accepted factorization (pivot 1e-13 > dynamic_reg_eps) is redone with
+    bumps max(1000 x last, 1e-7): 1e-7 still overflows for this right-hand
+    side, 1e-4 does not (a x10 ladder would land on 1e-5)."""
+    P = _sp([(0, 0, 1e-13)], (1, 1))
+    A = sp.csc_matrix((0, 1))
+    rhs = np.array([1e303])
+    x, rel, ok = oracle.kkt_solve(P, A, 0, 0, [], rhs, **NO_REG)
+    assert ok and np.isfinite(x[0])
+    np.testing.assert_allclose(x[0], 1e303 / (1e-13 + 1e-4), rtol=1e-12)
+
+
+@pytest.mark.parametrize("nzero, capped", [(1, False), (2, True)])
+def test_regularized_backend_five_percent_cap(oracle, nzero, capped):
+    """RegularizedLdlt (regularized_ldlt.cpp:107-139, 251-253): a pivot below
+    dynamic_reg_eps is replaced by sign * dynamic_reg_delta, and the
+    factorization fails when more than 5% of the pivots were corrected; the
+    ladder then bumps the whole diagonal by 1e-7.  20 pivots: one corrected
+    (5%) is accepted, two (10%) are not."""
+    n = 20
+    dg = np.full(n, 1e-3)
+    dg[:nzero] = 0.0
+    P = sp.diags(dg).tocsc()
+    P.sort_indices()
+    A = sp.csc_matrix((0, n))
+    rhs = np.linspace(1.0, 2.0, n)
+    x, rel, ok = oracle.kkt_solve(P, A, 0, 0, [], rhs, linear_solver=2, **NO_REG)
+    assert ok
+    d = 1e-7
+    expect = rhs / (dg + d) if capped else rhs / np.where(dg == 0.0, d, dg)
+    np.testing.assert_allclose(x, expect, rtol=1e-12)
+
+
 @pytest.mark.parametrize("seed", [0, 1])
 def test_kkt_random_soc_vs_dense(oracle, seed):
     rng = np.random.default_rng(seed)
