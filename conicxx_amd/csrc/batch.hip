@@ -389,6 +389,24 @@ __device__ inline double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
 }
+// Butterfly sum of 16 per-lane values over the 64 lanes of a wave: every lane
+// l returns the wave total of value (l >> 2) & 15.  Each exchange step halves
+// the values a lane holds (17 exchanges in all instead of 16 x 6).
+__device__ inline double wave_sum16(const double (&v)[16]) {
+  const int lane = threadIdx.x & 63;
+  const bool h5 = lane & 32, h4 = lane & 16, h3 = lane & 8, h2 = lane & 4;
+  double a[8], b[4], c[2];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) a[k] = (h5 ? v[k + 8] : v[k]) + __shfl_xor(h5 ? v[k] : v[k + 8], 32, 64);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) b[k] = (h4 ? a[k + 4] : a[k]) + __shfl_xor(h4 ? a[k] : a[k + 4], 16, 64);
+#pragma unroll
+  for (int k = 0; k < 2; ++k) c[k] = (h3 ? b[k + 2] : b[k]) + __shfl_xor(h3 ? b[k] : b[k + 2], 8, 64);
+  double d = (h2 ? c[1] : c[0]) + __shfl_xor(h2 ? c[0] : c[1], 4, 64);
+  d += __shfl_xor(d, 2, 64);
+  d += __shfl_xor(d, 1, 64);
+  return d;
+}
 // orders this wave's earlier memory writes before its later reads (M may be in
 // LDS or in global memory)
 __device__ inline void wave_sync() {
@@ -397,11 +415,40 @@ __device__ inline void wave_sync() {
 }
 
 typedef double v4d __attribute__((ext_vector_type(4)));
+// Wave-uniform copies of values the compiler cannot prove uniform: function
+// arguments arrive in vector registers and the per-problem view is read from
+// private memory.  Uniform copies live in scalar registers, and addresses
+// built from them need one vector register (the lane's offset) instead of a
+// 64-bit vector address per load.
+__device__ inline int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ inline double uni(double v) {
+  return __hiloint2double(__builtin_amdgcn_readfirstlane(__double2hiint(v)),
+                          __builtin_amdgcn_readfirstlane(__double2loint(v)));
+}
+template <typename T>
+__device__ inline T* uni(T* ptr) {
+  if constexpr (sizeof(T*) == 4) {
+    return (T*)(unsigned long)(unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(unsigned long)ptr);
+  } else {
+    const unsigned long v = (unsigned long)ptr;
+    const unsigned long lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)v);
+    const unsigned long hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(v >> 32));
+    return (T*)((hi << 32) | lo);
+  }
+}
 // address-space qualified views of M and the work vectors: LDS accesses then
 // compile to ds_read/ds_write and global ones to global_load/global_store with
 // a scalar base, instead of flat accesses that wait on both memory counters
 typedef __attribute__((address_space(3))) double ldsd;
 typedef __attribute__((address_space(1))) double glbd;
+
+// M[idx] as the (uniform) base plus a 32-bit byte offset: a global load then
+// takes the scalar-base + vector-offset form, one vector register per address
+__device__ inline double ldo(const glbd* M, unsigned idx) {
+  return *(const glbd*)((const __attribute__((address_space(1))) char*)M + 8u * idx);
+}
+__device__ inline double ldo(const ldsd* M, unsigned idx) { return M[idx]; }
+__device__ inline double ldo(const double* M, unsigned idx) { return *(const double*)((const char*)M + 8u * idx); }
 
 constexpr int PB = 32;  // panel width of the blocked factorization
 constexpr int JC = 64;  // trailing columns per chunk (16 per wave)
@@ -650,34 +697,38 @@ __device__ bool factor_retry(CArgs& A, PV& p, double* colk, double* red) {
 // x = K^-1 rhs through the factored M (no refinement).  Bound rows:
 // y_i = (a_i x_j - r2_i) / (hs_i + da) after the solve of M.
 template <typename MT, typename VT>
-__device__ void reduced_solve_t(CArgs& A, const PV& p, const double* rhs, double* x, const MT* M, VT* t);
-__device__ void reduced_solve(CArgs& A, PV& p, const double* rhs, double* x) {
+__device__ void reduced_solve_t(CArgs& A, const PV& p, const double* rhs, double* x, const MT* M, VT* t,
+                                ldsd* part);
+// colk: LDS scratch of the factorization (256 doubles of partial sums here)
+__device__ void reduced_solve(CArgs& A, PV& p, const double* rhs, double* x, double* colk) {
   ++p.nsolve;
   const long long t0 = wall_clock64();
   // the intermediate t lives in LDS: among the vectors, or else in the panel
   // stage, which only the factorization uses
   if (p.m_lds && p.v_lds)
-    reduced_solve_t(A, p, rhs, x, (const ldsd*)p.M, (ldsd*)p.tmp);
+    reduced_solve_t(A, p, rhs, x, (const ldsd*)p.M, (ldsd*)p.tmp, (ldsd*)colk);
   else if (p.v_lds)
-    reduced_solve_t(A, p, rhs, x, (const glbd*)p.M, (ldsd*)p.tmp);
+    reduced_solve_t(A, p, rhs, x, (const glbd*)p.M, (ldsd*)p.tmp, (ldsd*)colk);
   else if (!p.m_lds && p.stage != nullptr && p.stage_cap >= p.N)
-    reduced_solve_t(A, p, rhs, x, (const glbd*)p.M, (ldsd*)p.stage);
+    reduced_solve_t(A, p, rhs, x, (const glbd*)p.M, (ldsd*)p.stage, (ldsd*)colk);
   else
-    reduced_solve_t(A, p, rhs, x, (const double*)p.M, p.tmp);
+    reduced_solve_t(A, p, rhs, x, (const double*)p.M, p.tmp, (ldsd*)colk);
   p.t_solve += wall_clock64() - t0;
 }
 template <typename MT, typename VT>
-__device__ void reduced_solve_t(CArgs& A, const PV& p, const double* rhs, double* x, const MT* M, VT* t) {
-  // scalars and pointers in registers (A and p live in memory that the stores
-  // below might alias, which would force a reload per use)
-  const int n = p.n, m = p.m, nr = p.nr, nk = p.nk, tid = threadIdx.x;
-  const double da = p.cur_a;
-  const int* __restrict__ acp = A.ac_ptr + p.acp;
+__device__ void reduced_solve_t(CArgs& A, const PV& p, const double* rhs, double* x, const MT* M, VT* t,
+                                ldsd* part) {
+  // scalars and pointers in scalar registers (A and p live in memory that the
+  // stores below might alias, which would force a reload per use)
+  const int n = uni(p.n), m = uni(p.m), nr = uni(p.nr), nk = uni(p.nk), tid = threadIdx.x;
+  const double da = uni(p.cur_a);
+  M = uni(M), t = uni(t), part = uni(part), rhs = uni(rhs), x = uni(x);
+  const int* __restrict__ acp = A.ac_ptr + uni(p.acp);
   const int* __restrict__ acr = A.ac_row;
-  const int* __restrict__ rk = A.rk + p.co;
-  const int* __restrict__ kr = A.kr + p.ko;
+  const int* __restrict__ rk = A.rk + uni(p.co);
+  const int* __restrict__ kr = A.kr + uni(p.ko);
   const double* __restrict__ Av = A.Av;
-  const double* __restrict__ hsd = p.C.hsd;
+  const double* __restrict__ hsd = uni(p.C.hsd);
   for (int j = tid; j < n; j += BG) {
     double acc = rhs[j];
     for (int e = acp[j]; e < acp[j + 1]; ++e) {
@@ -688,91 +739,214 @@ __device__ void reduced_solve_t(CArgs& A, const PV& p, const double* rhs, double
   }
   for (int kk = tid; kk < nk; kk += BG) t[n + kk] = rhs[n + kr[kk]];
   __syncthreads();
-  const int lane = tid & 63, wv = tid >> 6;
-  // 32-bit offsets (the host bounds the order): L(k0 + c + r, k0 + c) is
-  // M[cb0 + c * w - c (c + 1) / 2 + c + r], cb0 = colbase(k0), w = nr - k0
-  auto coff = [&](int cb0, int w, int c) { return cb0 + c * w - (c * (c + 1)) / 2; };
-  // L y = t in 64-row blocks.  Wave 0 loads its diagonal block into registers
-  // (one row per lane, independent loads) and runs the dependent 64-step
-  // recurrence on registers only; then every thread updates one row below the
-  // block with a fully unrolled, clamped-address dot product.
-  for (int k0 = 0; k0 < nr; k0 += 64) {
-    const int kb = min(64, nr - k0), cb0 = (int)colbase(k0, nr), w = nr - k0;
-    if (wv == 0) {
-      double Lr[64];
+  const int lane = tid & 63, wv = uni(tid >> 6);
+  const int nb = (nr + 63) >> 6;
+  // L(i, c) = M[lx(i, c)] for i >= c (32-bit offsets: the host bounds the order)
+  auto lx = [&](int i, int c) { return (unsigned)((int)colbase(c, nr) - c + i); };
+  // Column offsets along consecutive columns: L(i, j) = M[o + i] with
+  // o = colbase(j) - j, and the next column's o is o + nr - j - 1.  The running
+  // offset is kept opaque in one scalar register; otherwise every column's
+  // offset of an unrolled loop is precomputed up front and spilled.
+  auto col0 = [&](int j) { return (unsigned)((int)colbase(j, nr) - j); };
+  auto next_col = [&](unsigned& o, int j) {
+    o += (unsigned)(nr - j - 1);
+    __asm__ volatile("" : "+s"(o));
+  };
+  // L y = t in 64-row blocks (block b holds rows 64b .. 64b + kb - 1; only the
+  // last block can be short).  Two phases per block, each closed by a barrier:
+  //   A(b), b >= 1: the rows of block b take the contribution of block b - 1,
+  //     each wave 16 of its columns (operands loaded during B(b - 1)), the four
+  //     partial sums to LDS;
+  //   B(b): wave 0 adds them and runs the dependent recurrence of the diagonal
+  //     block on registers (operands loaded during B(b - 1)); meanwhile waves
+  //     1..3 apply block b - 1 to every row below block b, a row per thread
+  //     with all 64 of its loads in flight.  Every wave then loads its
+  //     operands of A(b + 1), and wave 0 the next diagonal block.
+  // The critical path per block is the recurrence and one 16-term update.
+  // Wave 0 and waves 1..3 run separate loops with the same sequence of
+  // barriers (the branch is uniform per wave), so the registers of the
+  // diagonal block and of the row updates are never live together.
+  double La[16];
+  // Loads of a full 64-row block (every block but possibly the last) need no
+  // masks: an index above the diagonal of a column lands in the tail of the
+  // previous column, in bounds, and the value is discarded.  Each load is then
+  // a scalar base plus the lane's offset.
+  // La[u] = L(kr + lane, kc + 16 wv + u): rows of one block against this
+  // wave's 16 columns of the (full) block left of it
+  auto load_a = [&](int kr, int kc) {
+    const int i = kr + lane, j0 = kc + 16 * wv;
+    unsigned o = col0(j0);
+    if (kr + 64 <= nr) {
 #pragma unroll
-      for (int c = 0; c < 64; ++c) {
-        const bool ok = c < lane && lane < kb;
-        const double v = M[ok ? coff(cb0, w, c) + lane : 0];
-        Lr[c] = ok ? v : 0.0;
+      for (int u = 0; u < 16; ++u) {
+        La[u] = ldo(M, o + i);
+        next_col(o, j0 + u);
       }
+    } else {
+      const bool ok = i < nr;
+#pragma unroll
+      for (int u = 0; u < 16; ++u) {
+        const double v = ldo(M, ok ? o + i : 0u);
+        La[u] = ok ? v : 0.0;
+        next_col(o, j0 + u);
+      }
+    }
+  };
+  auto phase_a_fwd = [&](int k0) {
+    const int kp = k0 - 64;
+    double acc = 0.0;
+#pragma unroll
+    for (int u = 0; u < 16; ++u) acc += La[u] * t[kp + 16 * wv + u];
+    part[wv * 64 + lane] = acc;
+    __syncthreads();
+  };
+  if (wv == 0) {
+    double Lr[64];  // Lr[c] = L(k0 + lane, k0 + c), c < lane < kb
+    auto load_diag = [&](int k0, int kb) {
+      unsigned o = col0(k0);
+      if (kb == 64) {
+#pragma unroll
+        for (int c = 0; c < 64; ++c) {
+          const double v = ldo(M, o + k0 + lane);
+          Lr[c] = c < lane ? v : 0.0;
+          next_col(o, k0 + c);
+        }
+      } else {
+#pragma unroll
+        for (int c = 0; c < 64; ++c) {
+          const bool ok = c < lane && lane < kb;
+          const double v = ldo(M, ok ? o + k0 + lane : 0u);
+          Lr[c] = ok ? v : 0.0;
+          if (c + 1 < kb) next_col(o, k0 + c);
+        }
+      }
+    };
+    load_diag(0, min(64, nr));
+    for (int b = 0; b < nb; ++b) {
+      const int k0 = 64 * b, kb = min(64, nr - k0);
+      if (b > 0) phase_a_fwd(k0);
       double tl = lane < kb ? t[k0 + lane] : 0.0;
+      if (b > 0) tl -= (part[lane] + part[64 + lane]) + (part[128 + lane] + part[192 + lane]);
 #pragma unroll
       for (int c = 0; c < 64; ++c) tl -= Lr[c] * bcast(tl, c);
       if (lane < kb) t[k0 + lane] = tl;
-    }
-    __syncthreads();
-    for (int i = k0 + kb + tid; i < nr; i += BG) {
-      double acc = 0.0;
-#pragma unroll 16
-      for (int c = 0; c < 64; ++c) {
-        const bool ok = c < kb;
-        const double lv = M[ok ? coff(cb0, w, c) + (i - k0) : 0];
-        const double yv = t[ok ? k0 + c : 0];
-        acc += ok ? lv * yv : 0.0;
+      if (b + 1 < nb) {
+        load_a(k0 + 64, k0);
+        load_diag(k0 + 64, min(64, nr - k0 - 64));
       }
-      t[i] -= acc;
+      __syncthreads();
     }
-    __syncthreads();
-  }
-  for (int k = tid; k < nr; k += BG) t[k] /= M[colbase(k, nr)];
-  __syncthreads();
-  // L' x = y, blocks from the last: column dot products with the rows below
-  // (each wave owns 16 columns and keeps their 16 partial sums live, so every
-  // row step issues 16 independent coalesced loads), then wave 0 solves the
-  // diagonal block on registers
-  for (int k0 = ((nr - 1) / 64) * 64; k0 >= 0; k0 -= 64) {
-    const int kb = min(64, nr - k0), cb0 = (int)colbase(k0, nr), w = nr - k0;
-    if (k0 + kb < nr) {
-      const int c0 = 16 * wv;
-      double acc[16];
+  } else {
+    for (int b = 0; b < nb; ++b) {
+      const int k0 = 64 * b;
+      if (b > 0) {
+        phase_a_fwd(k0);
+        const int kp = k0 - 64;
+        for (int i = k0 + tid; i < nr; i += BG - 64) {  // rows k0 + 64 .. nr - 1
+          double lv[64];
+          unsigned o = col0(kp);
 #pragma unroll
-      for (int u = 0; u < 16; ++u) acc[u] = 0.0;
-      for (int i = k0 + kb + lane; i < nr; i += 64) {
-        const double xv = t[i];
+          for (int c = 0; c < 64; ++c) {
+            lv[c] = ldo(M, o + i);
+            next_col(o, kp + c);
+          }
+          double acc = 0.0;
 #pragma unroll
-        for (int u = 0; u < 16; ++u) {
-          const int c = c0 + u;
-          const bool ok = c < kb;
-          const double lv = M[ok ? coff(cb0, w, c) + (i - k0) : 0];
-          acc[u] += ok ? lv * xv : 0.0;
+          for (int c = 0; c < 64; ++c) acc += lv[c] * t[kp + c];
+          t[i] -= acc;
         }
       }
-#pragma unroll
-      for (int u = 0; u < 16; ++u) acc[u] = wave_sum(acc[u]);
-      if (lane < 16 && c0 + lane < kb) {
-        double a = acc[0];
-#pragma unroll
-        for (int u = 1; u < 16; ++u) a = lane == u ? acc[u] : a;
-        t[k0 + c0 + lane] -= a;
-      }
+      if (b + 1 < nb) load_a(k0 + 64, k0);
+      __syncthreads();
     }
-    __syncthreads();
-    if (wv == 0) {
-      double Lc[64];  // Lc[c] = L(k0 + c, k0 + lane) for c > lane
-      const int lb = coff(cb0, w, lane);  // M[lb + c] = L(k0 + c, k0 + lane)
+  }
+  for (int k = tid; k < nr; k += BG) t[k] /= ldo(M, (unsigned)colbase(k, nr));
+  __syncthreads();
+  // L' x = y, blocks from the last, right-looking: once x_b is final, every
+  // column j < 64b takes the sum over block b's rows i of L(i, j) x_i, lanes
+  // along the rows (coalesced) and a butterfly sum per 16 columns.  Per block:
+  //   A(b), b < nb - 1: the columns of block b take the contribution of block
+  //     b + 1, each wave 16 of them (operands loaded during B(b + 1));
+  //   B(b): wave 0 runs the recurrence of the diagonal block; meanwhile waves
+  //     1..3 apply block b + 1 to the columns left of block b, 16-column chunks
+  //     dealt round-robin.  Then the operands of A(b - 1) and wave 0's next
+  //     diagonal block are loaded.
+  auto phase_a_bwd = [&](int k0) {
+    const int i = k0 + 64 + lane;
+    const double xv = i < nr ? t[i] : 0.0;
+    double pr[16];
 #pragma unroll
-      for (int c = 0; c < 64; ++c) {
-        const bool ok = c > lane && c < kb;
-        const double v = M[ok ? lb + c : 0];
-        Lc[c] = ok ? v : 0.0;
+    for (int u = 0; u < 16; ++u) pr[u] = La[u] * xv;
+    const double s = wave_sum16(pr);
+    if ((lane & 3) == 0) part[16 * wv + (lane >> 2)] = s;
+    __syncthreads();
+  };
+  if (wv == 0) {
+    double Lc[64];  // Lc[c] = L(k0 + c, k0 + lane), lane < c < kb
+    auto load_diag = [&](int k0, int kb) {
+      if (kb == 64) {
+        const unsigned cl = lx(k0, k0 + lane);  // L(k0 + c, k0 + lane) = M[cl + c]
+#pragma unroll
+        for (int c = 0; c < 64; ++c) {
+          const double v = ldo(M, cl + c);
+          Lc[c] = c > lane ? v : 0.0;
+        }
+      } else {
+#pragma unroll
+        for (int c = 0; c < 64; ++c) {
+          const bool ok = c > lane && c < kb;
+          const double v = ldo(M, ok ? lx(k0 + c, k0 + lane) : 0u);
+          Lc[c] = ok ? v : 0.0;
+        }
       }
+    };
+    load_diag(64 * (nb - 1), nr - 64 * (nb - 1));
+    for (int b = nb - 1; b >= 0; --b) {
+      const int k0 = 64 * b, kb = min(64, nr - k0);
+      if (b < nb - 1) phase_a_bwd(k0);
       double tl = lane < kb ? t[k0 + lane] : 0.0;
+      if (b < nb - 1) tl -= part[lane];
 #pragma unroll
       for (int c = 63; c > 0; --c) tl -= Lc[c] * bcast(tl, c);
       if (lane < kb) t[k0 + lane] = tl;
+      if (b > 0) {
+        load_a(k0, k0 - 64);
+        load_diag(k0 - 64, 64);
+      }
+      __syncthreads();
     }
-    __syncthreads();
+  } else {
+    for (int b = nb - 1; b >= 0; --b) {
+      const int k0 = 64 * b;
+      if (b < nb - 1) {
+        phase_a_bwd(k0);
+        const int i = k0 + 64 + lane;
+        const bool okr = i < nr, full = k0 + 128 <= nr;
+        const double xv = okr ? t[i] : 0.0;
+        for (int j0 = 16 * (wv - 1); j0 < k0; j0 += 48) {
+          double pr[16];
+          unsigned o = col0(j0);
+          if (full) {
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+              pr[u] = ldo(M, o + i) * xv;
+              next_col(o, j0 + u);
+            }
+          } else {
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+              const double v = ldo(M, okr ? o + i : 0u);
+              pr[u] = okr ? v * xv : 0.0;
+              next_col(o, j0 + u);
+            }
+          }
+          const double s = wave_sum16(pr);
+          if ((lane & 3) == 0) t[j0 + (lane >> 2)] -= s;
+        }
+      }
+      if (b > 0) load_a(k0, k0 - 64);
+      __syncthreads();
+    }
   }
   const int* __restrict__ arp = A.ar_ptr + p.arp;
   const int* __restrict__ ars = A.ar_src;
@@ -840,7 +1014,7 @@ __device__ bool escalate(CArgs& A, PV& p, const double* rhs, double* x, double* 
     ep = dmax(ep * 1000.0, A.dyn_delta);
     ea = dmax(ea * 1000.0, A.dyn_delta);
     if (!try_factor(A, p, ep, ea, colk, red)) continue;
-    reduced_solve(A, p, rhs, x);
+    reduced_solve(A, p, rhs, x, colk);
     if (count_nonfinite(p.N, x, red) == 0) {
       p.last_p = ep, p.last_a = ea;
       return true;
@@ -856,7 +1030,7 @@ __device__ double kkt_solve(CArgs& A, PV& p, const double* rhs, double* x, doubl
                            double tol, int min_steps) {
   const long long t_kkt0 = wall_clock64();
   ++p.nkkt;
-  reduced_solve(A, p, rhs, x);
+  reduced_solve(A, p, rhs, x, colk);
   const double nonfin = count_nonfinite(p.N, x, red);
   kkt_residual(A, p, rhs, x, p.res);
   double v[2] = {0.0, 0.0};
@@ -879,7 +1053,7 @@ __device__ double kkt_solve(CArgs& A, PV& p, const double* rhs, double* x, doubl
   // solves, as the reference's solves do).
   const int rmax = A.refine_max_iter > 0 ? max(A.refine_max_iter, BATCH_REFINE_MAX) : 0;
   for (int it = 0; it < rmax && rel > tol; ++it) {
-    reduced_solve(A, p, p.res, p.dsol);
+    reduced_solve(A, p, p.res, p.dsol, colk);
     vaxpy(p.N, 1.0, p.dsol, x);
     kkt_residual(A, p, rhs, x, p.res);
     double r = 0.0;
