@@ -450,6 +450,21 @@ __device__ inline double ldo(const glbd* M, unsigned idx) {
 __device__ inline double ldo(const ldsd* M, unsigned idx) { return M[idx]; }
 __device__ inline double ldo(const double* M, unsigned idx) { return *(const double*)((const char*)M + 8u * idx); }
 
+__device__ inline void sto(glbd* M, unsigned idx, double v) {
+  *(glbd*)((__attribute__((address_space(1))) char*)M + 8u * idx) = v;
+}
+__device__ inline void sto(ldsd* M, unsigned idx, double v) { M[idx] = v; }
+__device__ inline void sto(double* M, unsigned idx, double v) { *(double*)((char*)M + 8u * idx) = v; }
+// Column offsets along consecutive columns of the packed M: L(i, j) =
+// M[o + i] with o = colbase(j) - j, and the next column's o is o + nr - j - 1.
+// The running offset is kept opaque in one scalar register; otherwise every
+// column's offset of an unrolled loop is precomputed up front and spilled.
+__device__ inline unsigned col_off(int j, int nr) { return (unsigned)((int)colbase(j, nr) - j); }
+__device__ inline void col_next(unsigned& o, int j, int nr) {
+  o += (unsigned)(nr - j - 1);
+  __asm__ volatile("" : "+s"(o));
+}
+
 constexpr int PB = 32;  // panel width of the blocked factorization
 constexpr int JC = 64;  // trailing columns per chunk (16 per wave)
 constexpr int COLK = PB * JC + 2 * PB + 8;  // factorization scratch in LDS (doubles)
@@ -466,11 +481,12 @@ constexpr int PS = PB + 1;  // row stride of a staged panel
 
 template <typename MT, bool STAGED>
 __device__ bool factor_M_t(CArgs& A, const PV& p, MT* M, ldsd* wbuf, ldsd* pk, ldsd* stage, double* red) {
-  // scalars in registers: A and p live in memory that every store in the
-  // loops below might alias, which would force a reload per use
-  const int nr = p.nr, pn = p.n, mode = A.mode;
+  // scalars in scalar registers: A and p live in memory that every store in
+  // the loops below might alias, which would force a reload per use
+  const int nr = uni(p.nr), pn = uni(p.n), mode = A.mode;
   const double dyn_eps = A.dyn_eps, dyn_delta = A.dyn_delta;
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, wv = uni(tid >> 6);
+  M = uni(M), wbuf = uni(wbuf), pk = uni(pk), stage = uni(stage);
   // LDS scratch: Wt of the diagonal block (PB rows of stride 2 PB) in wbuf;
   // the pivot reciprocals follow the pivots pk
   static_assert(JC == 2 * PB, "Wt rows of the diagonal block fill wbuf");
@@ -492,14 +508,30 @@ __device__ bool factor_M_t(CArgs& A, const PV& p, MT* M, ldsd* wbuf, ldsd* pk, l
         return (M[off(i, c)]);
     };
     if constexpr (staged) {
-#pragma unroll 8
-      for (int c = 0; c < kb; ++c)
-        for (int i = k0 + c + tid; i < nr; i += BG) stage[(i - k0) * PS + c] = M[off(i, c)];
+      if (kb == PB) {
+        // a row per thread with all its loads in flight; entries above the
+        // diagonal of a column read the previous column's tail (in bounds)
+        // and are never used
+        for (int i = k0 + tid; i < nr; i += BG) {
+          double v[PB];
+          unsigned o = col_off(k0, nr);
+#pragma unroll
+          for (int c = 0; c < PB; ++c) {
+            v[c] = ldo(M, o + i);
+            col_next(o, k0 + c, nr);
+          }
+#pragma unroll
+          for (int c = 0; c < PB; ++c) stage[(i - k0) * PS + c] = v[c];
+        }
+      } else {
+        for (int c = 0; c < kb; ++c)
+          for (int i = k0 + c + tid; i < nr; i += BG) stage[(i - k0) * PS + c] = M[off(i, c)];
+      }
       __syncthreads();
     }
     // panel: wave 0 factors the kb x kb diagonal block in registers (lane i
-    // holds row i; at each pivot every lane publishes its entry of the scaled
-    // column d l in LDS, read back as broadcasts) and leaves D, 1/D and
+    // holds row i; at each pivot the entries of the scaled column d l reach
+    // the other lanes by readlane broadcasts) and leaves D, 1/D and
     // Wt(q, c) = d_q L(c, q) in LDS (rows of stride 2 PB); then every thread
     // solves its own rows below the block against it
     // (L(i, c) = (a_ic - sum_q L(i, q) W(c, q)) / d_c, the row in registers).
@@ -529,20 +561,22 @@ __device__ bool factor_M_t(CArgs& A, const PV& p, MT* M, ldsd* wbuf, ldsd* pk, l
           }
           const double inv = 1.0 / d;
           const double l = lane > j ? r[j] * inv : 0.0;
+          const double w = d * l;  // lane k holds W(k, j) = d L(k, j)
           r[j] = lane > j ? l : (lane == j ? d : r[j]);
-          wbuf[j * 2 * PB + lane] = d * l;
+          wbuf[j * 2 * PB + lane] = w;
           if (lane == j) pk[j] = d, ipk[j] = inv;
-          wave_sync();
 #pragma unroll
-          for (int k = j + 1; k < PB; ++k) r[k] -= l * wbuf[j * 2 * PB + k];
+          for (int k = j + 1; k < PB; ++k) r[k] -= l * bcast(w, k);
         }
       }
+      unsigned o = col_off(k0, nr);
 #pragma unroll
       for (int c = 0; c < PB; ++c) {
         if (c <= lane && lane < kb) {
           L(k0 + lane, c) = r[c];
-          if constexpr (STAGED) M[off(k0 + lane, c)] = r[c];
+          if constexpr (STAGED) sto(M, o + k0 + lane, r[c]);
         }
+        if (c + 1 < kb) col_next(o, k0 + c, nr);
       }
     }
     __syncthreads();
@@ -561,11 +595,13 @@ __device__ bool factor_M_t(CArgs& A, const PV& p, MT* M, ldsd* wbuf, ldsd* pk, l
           a[c] = acc * ipk[c];
         }
       }
+      unsigned o = col_off(k0, nr);
 #pragma unroll
       for (int c = 0; c < PB; ++c)
         if (c < kb) {
           L(i, c) = a[c];
-          if constexpr (STAGED) M[off(i, c)] = a[c];
+          if constexpr (STAGED) sto(M, o + i, a[c]);
+          if (c + 1 < kb) col_next(o, k0 + c, nr);
         }
     }
     __syncthreads();
@@ -743,15 +779,8 @@ __device__ void reduced_solve_t(CArgs& A, const PV& p, const double* rhs, double
   const int nb = (nr + 63) >> 6;
   // L(i, c) = M[lx(i, c)] for i >= c (32-bit offsets: the host bounds the order)
   auto lx = [&](int i, int c) { return (unsigned)((int)colbase(c, nr) - c + i); };
-  // Column offsets along consecutive columns: L(i, j) = M[o + i] with
-  // o = colbase(j) - j, and the next column's o is o + nr - j - 1.  The running
-  // offset is kept opaque in one scalar register; otherwise every column's
-  // offset of an unrolled loop is precomputed up front and spilled.
-  auto col0 = [&](int j) { return (unsigned)((int)colbase(j, nr) - j); };
-  auto next_col = [&](unsigned& o, int j) {
-    o += (unsigned)(nr - j - 1);
-    __asm__ volatile("" : "+s"(o));
-  };
+  auto col0 = [&](int j) { return col_off(j, nr); };
+  auto next_col = [&](unsigned& o, int j) { col_next(o, j, nr); };
   // L y = t in 64-row blocks (block b holds rows 64b .. 64b + kb - 1; only the
   // last block can be short).  Two phases per block, each closed by a barrier:
   //   A(b), b >= 1: the rows of block b take the contribution of block b - 1,
