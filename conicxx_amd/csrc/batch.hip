@@ -626,20 +626,42 @@ __device__ bool factor_M_t(CArgs& A, const PV& p, MT* M, ldsd* wbuf, ldsd* pk, l
         }
         // M(i, jr[r]) = M[cb[r] + i]; outside the tile the access is clamped to
         // the diagonal entry of a valid column and its value discarded
-        int cb[4], jr[4], jd[4];
+        unsigned cb[4];
+        int jr[4], jd[4];
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           jr[r] = tj + kq + 4 * r;
           jd[r] = min(jr[r], nr - 1);
-          cb[r] = (int)colbase(jd[r], nr) - jd[r];
+          cb[r] = (unsigned)((int)colbase(jd[r], nr) - jd[r]);
         }
+        // a tile below the diagonal tile, inside the matrix, in a tile column
+        // of 16 valid columns needs no masks
+        const bool cols_ok = tj + 16 <= nr;
+        auto interior = [&](int ti) { return cols_ok && ti > tj && ti + 16 <= nr; };
         auto load_m = [&](int ti, double (&mv)[4]) {
           const int i = ti + l15;
+          if (interior(ti)) {
 #pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const bool ok = i >= jr[r] && i < nr;
-            const double v = M[cb[r] + (ok ? i : jd[r])];
-            mv[r] = ok ? v : 0.0;
+            for (int r = 0; r < 4; ++r) mv[r] = ldo(M, cb[r] + i);
+          } else {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              // masked-out values only reach masked-out stores: no select
+              // (which would wait for the load here)
+              const bool ok = i >= jr[r] && i < nr;
+              mv[r] = ldo(M, cb[r] + (unsigned)(ok ? i : jd[r]));
+            }
+          }
+        };
+        auto store_m = [&](int ti, const double (&mv)[4], const v4d& acc) {
+          const int i = ti + l15;
+          if (interior(ti)) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) sto(M, cb[r] + i, mv[r] - acc[r]);
+          } else {
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+              if (i >= jr[r] && i < nr) sto(M, cb[r] + i, mv[r] - acc[r]);
           }
         };
         auto load_a = [&](int ti, double (&av)[PB / 4]) {
@@ -652,28 +674,28 @@ __device__ bool factor_M_t(CArgs& A, const PV& p, MT* M, ldsd* wbuf, ldsd* pk, l
             av[k] = ok ? v : 0.0;
           }
         };
-        // the tiles of a column are disjoint: M is read two tiles ahead of
-        // the stores and the A operands one tile ahead
-        double m0[4], m1[4], an[PB / 4];
-        load_m(tj, m0);
-        load_m(tj + 16, m1);
-        load_a(tj, an);
-        for (int ti = tj; ti < nr; ti += 16) {
-          double av[PB / 4], m2[4];
-#pragma unroll
-          for (int k = 0; k < PB / 4; ++k) av[k] = an[k];
-          load_m(ti + 32, m2);
-          load_a(ti + 16, an);
+        auto tile = [&](const double (&av)[PB / 4]) {
           v4d acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
           for (int k = 0; k < PB / 4; ++k) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[k], av[k], acc, 0, 0, 0);
-          const int i = ti + l15;
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            if (i >= jr[r] && i < nr) M[cb[r] + i] = m0[r] - acc[r];
-            m0[r] = m1[r];
-            m1[r] = m2[r];
-          }
+          return acc;
+        };
+        // the tiles of a column are disjoint.  Two register sets, each
+        // refilled with the tile two ahead as soon as its own tile is stored:
+        // no rotation of registers, since moving a register whose load is in
+        // flight waits for the load
+        double mA[4], mB[4], aA[PB / 4], aB[PB / 4];
+        load_m(tj, mA);
+        load_a(tj, aA);
+        load_m(tj + 16, mB);
+        load_a(tj + 16, aB);
+        for (int ti = tj; ti < nr; ti += 32) {
+          store_m(ti, mA, tile(aA));
+          load_m(ti + 32, mA);
+          load_a(ti + 32, aA);
+          if (ti + 16 < nr) store_m(ti + 16, mB, tile(aB));
+          load_m(ti + 48, mB);
+          load_a(ti + 48, aB);
         }
       }
     }
