@@ -93,6 +93,7 @@ SIGNATURES = [
                                              ctypes.POINTER(CxInfo)]),
     ("cx_batch_set_settings", None, [ctypes.c_void_p, ctypes.POINTER(CxSettings)]),
     ("cx_batch_autotune", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("cx_batch_set_tuning", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
     ("cx_batch_get_tuning", None, [ctypes.c_void_p, _f64p]),
     ("cx_batch_get_stats", None, [ctypes.c_void_p, ctypes.c_int, _f64p]),
     ("cx_batch_set_differentiable", None, [ctypes.c_void_p, ctypes.c_int]),

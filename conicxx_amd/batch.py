@@ -171,6 +171,11 @@ class BatchSolver:
     def autotune(self, reps=1):
         return int(_lib.lib().cx_batch_autotune(self._h, reps))
 
+    def set_tuning(self, lds_doubles, grid):
+        """Pins a launch configuration: lds_doubles of LDS per workgroup (what
+        does not fit lives in global memory) and grid persistent workgroups."""
+        return bool(_lib.lib().cx_batch_set_tuning(self._h, int(lds_doubles), int(grid)))
+
     def tuning(self):
         out = np.zeros(4)
         _lib.lib().cx_batch_get_tuning(self._h, out.ctypes.data_as(_lib._f64p))

@@ -2177,13 +2177,25 @@ int BatchSolver::get_solution(int k, double* x, double* s, double* z, double* ob
 }
 
 // Launch-configuration autotuner.  A candidate is an LDS budget per
-// workgroup (all of it: one resident workgroup per CU with every factor that
-// fits kept on chip; or a half, third, quarter: more resident workgroups,
-// larger factors in global memory with their panels staged) and the matching
-// persistent grid (resident slots, then twice that).  Each candidate solves
+// workgroup (all of it: every factor that fits kept on chip; or a half, third,
+// quarter: larger factors in global memory, some unstaged) and the matching
+// persistent grid (resident slots, then twice that).  A smaller budget is a
+// candidate only when the runtime's occupancy calculator gives it more
+// resident workgroups per CU than every larger budget (registers can cap the
+// residency below what the LDS budget would allow).  Each candidate solves
 // this batch reps times from the same starting state (the warm-start flags
 // are restored before every trial and at the end); the fastest device time
 // wins.  Solutions left in place are those of the last trial.
+bool BatchSolver::set_tuning(int lds_doubles, int grid) {
+  if (!ready_ || lds_doubles < 0 || lds_doubles > LDS_MAX_BYTES / 8 || grid < 1) return false;
+  tune_.lds_doubles = lds_doubles;
+  tune_.grid = grid;
+  tuned_ = true;
+  if (gexec_) (void)hipGraphExecDestroy(gexec_), gexec_ = nullptr;
+  if (graph_) (void)hipGraphDestroy(graph_), graph_ = nullptr;
+  return true;
+}
+
 int BatchSolver::autotune(int reps) {
   if (!ready_) return -1;
   reps = std::max(1, reps);
@@ -2195,9 +2207,17 @@ int BatchSolver::autotune(int reps) {
   CX_HIP(hipStreamSynchronize(st_));
   const long full = std::min<long>(max_lds_need_, LDS_MAX_BYTES / 8);
   std::vector<BatchTuning> cand;
+  int prev_res = 0;
   for (int per_cu = 1; per_cu <= 4; ++per_cu) {
     const int lds = (int)std::min<long>(full, (LDS_MAX_BYTES / 8) / per_cu);
-    const int slots = ncu * std::max(1, (int)((LDS_MAX_BYTES / 8) / std::max(1, lds)));
+    int res = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&res, reinterpret_cast<const void*>(&k_batch_solve), BG,
+                                                     (size_t)lds * sizeof(double)) != hipSuccess ||
+        res < 1)
+      res = 1;
+    if (res <= prev_res) continue;
+    prev_res = res;
+    const int slots = ncu * res;
     for (int over = 1; over <= 2; ++over) {
       BatchTuning t;
       t.lds_doubles = lds;

@@ -58,6 +58,7 @@ class BatchSolver {
   int get_solution(int k, double* x, double* s, double* z, double* obj, cx_info* info);
   void set_settings(const cx_settings& st);
   int autotune(int reps);
+  bool set_tuning(int lds_doubles, int grid);  // pins a launch configuration
   // differentiation: when on, every Solved problem's forward pass also factors
   // its adjoint KKT matrix; backward() is then one KKT solve per problem
   void set_differentiable(bool on);

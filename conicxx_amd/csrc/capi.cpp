@@ -293,6 +293,16 @@ int cx_batch_autotune(cx_batch* h, int reps) {
   }
 }
 
+int cx_batch_set_tuning(cx_batch* h, int lds_doubles, int grid) {
+  if (!h) return 0;
+  try {
+    return h->impl->set_tuning(lds_doubles, grid) ? 1 : 0;
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "[conicxx_amd] batch set_tuning failed: %s\n", e.what());
+    return 0;
+  }
+}
+
 void cx_batch_get_tuning(const cx_batch* h, double* out4) {
   if (h && out4) h->impl->tuning(out4);
 }

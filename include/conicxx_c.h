@@ -230,6 +230,12 @@ void cx_batch_set_settings(cx_batch* h, const cx_settings* settings);
 /* Times the candidate launch configurations on this batch (reps solves each)
  * and keeps the fastest; returns the number of candidates tried. */
 int cx_batch_autotune(cx_batch* h, int reps);
+/* Pins a launch configuration (no counterpart in the reference: the launch
+ * configuration is this implementation's own): lds_doubles of dynamic LDS per
+ * workgroup (0 .. the device maximum; factors and vectors that do not fit live
+ * in global memory) and grid persistent workgroups (>= 1).  Returns 1, or 0 on
+ * an out-of-range value (nothing changed). */
+int cx_batch_set_tuning(cx_batch* h, int lds_doubles, int grid);
 /* out4 = (packed M doubles kept in LDS, persistent workgroups, largest packed M,
  * autotuned flag). */
 void cx_batch_get_tuning(const cx_batch* h, double* out4);

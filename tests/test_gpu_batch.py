@@ -57,23 +57,33 @@ def test_batch_mixed_sizes_and_global_M(oracle):
 
 def test_batch_autotune_keeps_results():
     """The autotuner times every candidate launch configuration (LDS budget per
-    workgroup, persistent grid) on the batch itself: all of them must give the
-    untuned results, and the warm-start state must be the one it found."""
+    workgroup, persistent grid) on the batch itself, and pinned configurations
+    with smaller LDS budgets move factors and vectors to global memory (staged
+    and unstaged panels): all of them must give the untuned results."""
     probs = [PR.config2_problem(i) for i in range(40)] + [PR.config0(seed=0), PR.config3_problem(0)]
     ref = batch_of(probs, Settings(warm_start=False))
     ref.solve()
+    full = ref.tuning()["lds_doubles"]
     b = batch_of(probs, Settings(warm_start=False))
     ntried = b.autotune(1)
     t = b.tuning()
     print("candidates", ntried, "chosen", t)
-    assert ntried >= 2 and t["autotuned"]
-    st = b.solve()
-    for k, p in enumerate(probs):
-        a, c = ref.solution(k), b.solution(k)
-        assert int(st[k]) == int(a.status), k
-        if a.status == Status.Solved:
-            err = np.abs(c.x - a.x).max() / max(1.0, np.abs(a.x).max())
-            assert err < 1e-6, (k, err)
+    assert ntried >= 1 and t["autotuned"]
+
+    def check(st):
+        for k, p in enumerate(probs):
+            a, c = ref.solution(k), b.solution(k)
+            assert int(st[k]) == int(a.status), k
+            if a.status == Status.Solved:
+                err = np.abs(c.x - a.x).max() / max(1.0, np.abs(a.x).max())
+                assert err < 1e-6, (k, err)
+
+    check(b.solve())
+    assert not b.set_tuning(-1, 8) and not b.set_tuning(full, 0)
+    for lds, grid in ((full // 2, 64), (full // 4, 256), (4000, 42), (0, 7)):
+        assert b.set_tuning(lds, grid)
+        print("pinned", b.tuning())
+        check(b.solve())
 
 
 def test_batch_timesteps_with_update_data(oracle):
