@@ -307,6 +307,27 @@ __device__ inline double hs_stored(const PV& p, int k, int off, bool id, double 
 // the product the refinement residual needs, so that refinement converges to
 // the solution of the matrix that was factored (kkt_system.cpp:296-311 refines
 // against the assembled K)
+// out = -(H x + 0) on the orthant and SOC rows, no barrier (the zero rows are
+// left to the caller)
+__device__ void neg_hs_plus_stored_nb(const PV& p, const double* x, double* out) {
+  for (int i = threadIdx.x; i < p.nn; i += BG) {
+    const int r = p.nz + i;
+    out[r] = -(p.C.hsd[r] * x[r] + 0.0);
+  }
+  for (int k = threadIdx.x; k < p.nsoc; k += BG) {
+    const int off = p.C.soc_off[k], d = p.C.soc_dim[k];
+    const double* v = p.C.v + off;
+    const double be = p.C.beta[k], vvk = p.C.vv[k];
+    const bool id = p.C.ident[k] != 0;
+    for (int a = 0; a < d; ++a) {
+      double acc = 0.0;
+      for (int c = 0; c < d; ++c)
+        acc += (a >= c ? hs_stored(p, k, off, id, be, vvk, v, a, c) : hs_stored(p, k, off, id, be, vvk, v, c, a)) *
+               x[off + c];
+      out[off + a] = -(acc + 0.0);
+    }
+  }
+}
 __device__ void neg_hs_plus_stored(const PV& p, const double* x, const double* y, double* out) {
   for (int i = threadIdx.x; i < p.nz; i += BG) out[i] = -y[i];
   for (int i = threadIdx.x; i < p.nn; i += BG) {
@@ -389,22 +410,55 @@ __device__ inline double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
 }
+// Cross-lane moves of a double on the VALU: DPP within rows of 16 lanes, and
+// the gfx950 permlane swaps between 16-lane rows (odd rows of x with even rows
+// of y) and between 32-lane halves (upper half of x with lower half of y).
+template <int CTRL>
+__device__ inline double dppd(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+__device__ inline void swap16(double& x, double& y) {
+  const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)__double2loint(x), (unsigned)__double2loint(y), false, false);
+  const auto hi = __builtin_amdgcn_permlane16_swap((unsigned)__double2hiint(x), (unsigned)__double2hiint(y), false, false);
+  x = __hiloint2double((int)hi[0], (int)lo[0]);
+  y = __hiloint2double((int)hi[1], (int)lo[1]);
+}
+__device__ inline void swap32(double& x, double& y) {
+  const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)__double2loint(x), (unsigned)__double2loint(y), false, false);
+  const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)__double2hiint(x), (unsigned)__double2hiint(y), false, false);
+  x = __hiloint2double((int)hi[0], (int)lo[0]);
+  y = __hiloint2double((int)hi[1], (int)lo[1]);
+}
 // Butterfly sum of 16 per-lane values over the 64 lanes of a wave: every lane
-// l returns the wave total of value (l >> 2) & 15.  Each exchange step halves
-// the values a lane holds (17 exchanges in all instead of 16 x 6).
+// l returns the wave total of value (l >> 2) & 15.  Each step pairs every lane
+// with one in the other half of a lane group and halves the values a lane
+// holds: swapping halves of two registers and adding them does a whole step
+// on two values (32- and 16-lane steps); within a row, DPP rotates by 8 and
+// mirrors 8-lane halves (pairing lanes that differ in bit 2), then quad
+// permutations add the last four lanes.  All on the VALU, no LDS traffic.
 __device__ inline double wave_sum16(const double (&v)[16]) {
   const int lane = threadIdx.x & 63;
-  const bool h5 = lane & 32, h4 = lane & 16, h3 = lane & 8, h2 = lane & 4;
+  const bool h3 = lane & 8, h2 = lane & 4;
   double a[8], b[4], c[2];
 #pragma unroll
-  for (int k = 0; k < 8; ++k) a[k] = (h5 ? v[k + 8] : v[k]) + __shfl_xor(h5 ? v[k] : v[k + 8], 32, 64);
+  for (int k = 0; k < 8; ++k) {  // lanes < 32 keep value k, lanes >= 32 value k + 8
+    double x = v[k], y = v[k + 8];
+    swap32(x, y);
+    a[k] = x + y;
+  }
 #pragma unroll
-  for (int k = 0; k < 4; ++k) b[k] = (h4 ? a[k + 4] : a[k]) + __shfl_xor(h4 ? a[k] : a[k + 4], 16, 64);
+  for (int k = 0; k < 4; ++k) {  // even rows keep a[k], odd rows a[k + 4]
+    double x = a[k], y = a[k + 4];
+    swap16(x, y);
+    b[k] = x + y;
+  }
 #pragma unroll
-  for (int k = 0; k < 2; ++k) c[k] = (h3 ? b[k + 2] : b[k]) + __shfl_xor(h3 ? b[k] : b[k + 2], 8, 64);
-  double d = (h2 ? c[1] : c[0]) + __shfl_xor(h2 ? c[0] : c[1], 4, 64);
-  d += __shfl_xor(d, 2, 64);
-  d += __shfl_xor(d, 1, 64);
+  for (int k = 0; k < 2; ++k) c[k] = (h3 ? b[k + 2] : b[k]) + dppd<0x128>(h3 ? b[k] : b[k + 2]);  // row_ror:8
+  double d = (h2 ? c[1] : c[0]) + dppd<0x141>(h2 ? c[0] : c[1]);                                  // row_half_mirror
+  d += dppd<0x4E>(d);                                                                              // quad_perm 2,3,0,1
+  d += dppd<0xB1>(d);                                                                              // quad_perm 1,0,3,2
   return d;
 }
 // orders this wave's earlier memory writes before its later reads (M may be in
@@ -972,27 +1026,34 @@ __device__ void reduced_solve_t(CArgs& A, const PV& p, const double* rhs, double
       if (b < nb - 1) {
         phase_a_bwd(k0);
         const int i = k0 + 64 + lane;
-        const bool okr = i < nr, full = k0 + 128 <= nr;
+        const bool okr = i < nr;
         const double xv = okr ? t[i] : 0.0;
-        for (int j0 = 16 * (wv - 1); j0 < k0; j0 += 48) {
-          double pr[16];
+        const unsigned ir = okr ? (unsigned)i : 0u;  // masked rows read column heads
+        // 16-column chunks dealt round-robin (stride 48), two register sets:
+        // each chunk's loads are issued a chunk ahead of its butterfly sum
+        auto load16 = [&](int j0, double (&v)[16]) {
           unsigned o = col0(j0);
-          if (full) {
 #pragma unroll
-            for (int u = 0; u < 16; ++u) {
-              pr[u] = ldo(M, o + i) * xv;
-              next_col(o, j0 + u);
-            }
-          } else {
-#pragma unroll
-            for (int u = 0; u < 16; ++u) {
-              const double v = ldo(M, okr ? o + i : 0u);
-              pr[u] = okr ? v * xv : 0.0;
-              next_col(o, j0 + u);
-            }
+          for (int u = 0; u < 16; ++u) {
+            v[u] = ldo(M, o + ir);
+            next_col(o, j0 + u);
           }
+        };
+        auto apply16 = [&](int j0, const double (&v)[16]) {
+          double pr[16];
+#pragma unroll
+          for (int u = 0; u < 16; ++u) pr[u] = okr ? v[u] * xv : 0.0;
           const double s = wave_sum16(pr);
           if ((lane & 3) == 0) t[j0 + (lane >> 2)] -= s;
+        };
+        double va[16], vb[16];
+        int j0 = 16 * (wv - 1);
+        if (j0 < k0) load16(j0, va);
+        for (; j0 < k0; j0 += 96) {
+          if (j0 + 48 < k0) load16(j0 + 48, vb);
+          apply16(j0, va);
+          if (j0 + 96 < k0) load16(j0 + 96, va);
+          if (j0 + 48 < k0) apply16(j0 + 48, vb);
         }
       }
       if (b > 0) load_a(k0, k0 - 64);
@@ -1044,16 +1105,33 @@ __device__ void soc_tangent_apply(const PV& p, const double* x, double* out) {
 }
 
 // res = rhs - K x with K the statically regularized KKT matrix (the refinement target)
+// Two passes: the first computes the variable rows outright (P x and A'y per
+// row) and, independently, A x and -H y per constraint row (the loads of all
+// three overlap); the second combines the constraint rows.
 __device__ void kkt_residual(CArgs& A, PV& p, const double* rhs, const double* x, double* res) {
   const long long t0 = wall_clock64();
-  const int n = p.n;
-  pmul(A, p, x, p.wn);
-  atmul(A, p, x + n, p.wn2);
-  amul(A, p, x, p.wm);
-  vfill(p.m, 0.0, p.wm2);
-  neg_hs_plus_stored(p, x + n, p.wm2, p.wm2);  // -H y (each block read and written by one thread)
-  for (int j = threadIdx.x; j < n; j += BG) res[j] = rhs[j] - p.wn[j] - A.sreg_p * x[j] - p.wn2[j];
-  for (int i = threadIdx.x; i < p.m; i += BG) res[n + i] = rhs[n + i] - p.wm[i] - p.wm2[i] + A.sreg_a * x[n + i];
+  const int n = uni(p.n), m = uni(p.m), tid = threadIdx.x;
+  const double sp = A.sreg_p, sa = A.sreg_a;
+  const int* __restrict__ pfp = A.pf_ptr + uni(p.pfp);
+  const int* __restrict__ acp = A.ac_ptr + uni(p.acp);
+  const int* __restrict__ arp = A.ar_ptr + uni(p.arp);
+  double* wm = p.wm;
+  double* wm2 = p.wm2;
+  for (int j = tid; j < n; j += BG) {
+    double px = 0.0, aty = 0.0;
+    for (int e = pfp[j]; e < pfp[j + 1]; ++e) px += A.Pv[A.pf_src[e]] * x[A.pf_col[e]];
+    for (int e = acp[j]; e < acp[j + 1]; ++e) aty += A.Av[e] * x[n + A.ac_row[e]];
+    res[j] = rhs[j] - px - sp * x[j] - aty;
+  }
+  for (int i = tid; i < m; i += BG) {
+    double ax = 0.0;
+    for (int e = arp[i]; e < arp[i + 1]; ++e) ax += A.Av[A.ar_src[e]] * x[A.ar_col[e]];
+    wm[i] = ax;
+  }
+  for (int i = tid; i < p.nz; i += BG) wm2[i] = -0.0;
+  neg_hs_plus_stored_nb(p, x + n, wm2);  // -H y (each block read and written by one thread)
+  __syncthreads();
+  for (int i = tid; i < m; i += BG) res[n + i] = rhs[n + i] - wm[i] - wm2[i] + sa * x[n + i];
   __syncthreads();
   p.t_res += wall_clock64() - t0;
 }
