@@ -35,6 +35,7 @@ HBM_PEAK_GBS = 8000.0
 # per the copy calibration, averaged over the launches of setup + one solve)
 K_TRAIL_C1_TRAFFIC_BYTES = 2.41e8
 FP64_MFMA_PEAK_TFLOPS = 78.6
+MI355X_CUS = 256  # compute units: one k_batch_solve workgroup is resident on each
 
 
 def cpu_model():
@@ -163,7 +164,7 @@ def batched_form(K=4096, timesteps=100, plain_sample=512, cpu=None):
     tune_s = time.perf_counter() - t0
     print(f"[bench] batch autotune: {ncand} candidates in {tune_s:.1f} s -> {b.tuning()}", file=sys.stderr, flush=True)
     b.solve()  # graph capture with the chosen configuration
-    solved, wall, dev_ms, flops, nonsolved = 0, 0.0, [], 0.0, []
+    solved, wall, dev_ms, flops, nonsolved, wg_us = 0, 0.0, [], 0.0, [], 0.0
     last = None
     for t in range(1, timesteps + 1):
         qs, bs = [], []
@@ -179,8 +180,10 @@ def batched_form(K=4096, timesteps=100, plain_sample=512, cpu=None):
         ns = int((stt == int(Status.Solved)).sum())
         solved += ns
         nonsolved.append(K - ns)
-        cnt = np.array([[s["factorizations"], s["solves"]] for s in (b.stats(k) for k in range(K))], dtype=np.float64)
+        stats = [b.stats(k) for k in range(K)]
+        cnt = np.array([[s["factorizations"], s["solves"]] for s in stats], dtype=np.float64)
         flops += float((cnt[:, 0] * f_fact + cnt[:, 1] * f_solve).sum())
+        wg_us += sum(s["us"] for s in stats)
         if t == 1:
             last = (stt.copy(), qs, bs, cnt)
         if t == 1 or t % 10 == 0:
@@ -224,7 +227,14 @@ def batched_form(K=4096, timesteps=100, plain_sample=512, cpu=None):
                              f"{float(r.mean()):.0f}, max {int(r.max())}); device time from HIP events around "
                              "each timestep's graph",
                      "flops_per_timestep": flops / timesteps,
-                     "compulsory_hbm_gbs": hbm / (dev_s / timesteps) / 1e9},
+                     "compulsory_hbm_gbs": hbm / (dev_s / timesteps) / 1e9,
+                     # the same work over the workgroups' own device clocks (the time each
+                     # workgroup spent on its problems): the rate of one workgroup, and of
+                     # all CUs while each has a problem (no idle tail)
+                     "per_workgroup_gflops": flops / (wg_us / 1e6) / 1e9,
+                     "all_cus_busy_tflops": flops / (wg_us / 1e6) * MI355X_CUS / 1e12,
+                     "all_cus_busy_frac": flops / (wg_us / 1e6) * MI355X_CUS / 1e12 / FP64_MFMA_PEAK_TFLOPS,
+                     "busy_fraction_of_device_time": wg_us / 1e6 / MI355X_CUS / dev_s},
         "cpu_baseline": cpu,
         "autotune_s": tune_s, "generate_s": gen_s,
     }
