@@ -259,6 +259,86 @@ def _sp(rows, cols, vals, shape):
     return _csc(sp.coo_matrix((vals, (rows, cols)), shape=shape))
 
 
+# --------------------------------------------------------------------------
+# Planted optimum: a strictly convex, nondegenerate conic QP whose optimum
+# (x*, s*, z*) is chosen first.  Each cone row or block is given one of the
+# strictly complementary patterns
+#   nonnegative row:  s* > 0, z* = 0   or   s* = 0, z* > 0
+#   SOC block:        s* interior, z* = 0;   s* = 0, z* interior;   or both on
+#                     the boundary, s* = a(1, u), z* = c(1, -u), |u| = 1
+#                     (s*'z* = 0 and s* + z* interior)
+# and q = -(P x* + A'z*), b = A x* + s*.  P is positive definite and the active
+# rows of A are random and fewer than n, so the KKT conditions have exactly one
+# solution and it is (x*, s*, z*): the optimum is known in closed form and the
+# solution map is Lipschitz near it (no degenerate directions that the
+# stopping tolerance leaves undetermined).
+# --------------------------------------------------------------------------
+def planted(seed=0, n=120, zero=10, nonneg=80, soc_dims=(3, 3, 4, 5, 6, 8), p_kind="sparse",
+            boundary_pairs=False):
+    """p_kind: "sparse" (M'M + I/2, M with n/2 rows of 10% density), "dense"
+    (B B'/n + I/2) or "tridiag" (diagonally dominant tridiagonal, for large n).
+    boundary_pairs adds SOC blocks with both s* and z* on the boundary; there
+    the interior-point iterates approach the optimum only as sqrt(mu)."""
+    rng = _rng(100, seed)
+    if p_kind == "dense":
+        B = rng.standard_normal((n, n)) / np.sqrt(n)
+        Pf = sp.csc_matrix(B @ B.T + 0.5 * np.eye(n))
+    elif p_kind == "tridiag":
+        e = rng.normal(0.0, 0.1, n - 1)
+        ae = np.abs(e)
+        dg = rng.uniform(0.5, 1.0, n) + np.concatenate([[0.0], ae]) + np.concatenate([ae, [0.0]])
+        Pf = sp.diags([e, dg, e], [-1, 0, 1], shape=(n, n), format="csc")
+    else:
+        rows_M = n // 2
+        pos = rng.choice(rows_M * n, size=int(0.1 * rows_M * n), replace=False)
+        M = sp.coo_matrix((rng.standard_normal(pos.size), (pos // n, pos % n)), shape=(rows_M, n))
+        Pf = (M.T @ M + 0.5 * sp.identity(n)).tocsc()
+    Pu = _upper(Pf)
+    soc_dims = list(soc_dims)
+    m = zero + nonneg + sum(soc_dims)
+    A = random_sparse_A(rng, m, n, 5)
+    x = rng.standard_normal(n)
+    s = np.zeros(m)
+    z = np.zeros(m)
+    z[:zero] = rng.standard_normal(zero)
+    act = rng.random(nonneg) < 0.5
+    v = rng.uniform(0.5, 2.0, nonneg)
+    s[zero:zero + nonneg] = np.where(act, 0.0, v)
+    z[zero:zero + nonneg] = np.where(act, v, 0.0)
+    o = zero + nonneg
+    for k, d in enumerate(soc_dims):
+        u = rng.standard_normal(d - 1)
+        u /= np.linalg.norm(u)
+        kind = k % 3 if boundary_pairs else k % 2
+        if kind == 0:    # s interior, z = 0
+            s[o:o + d] = np.concatenate([[rng.uniform(1.5, 2.5)], rng.uniform(0.2, 0.8) * u])
+        elif kind == 1:  # s = 0, z interior
+            z[o:o + d] = np.concatenate([[rng.uniform(1.5, 2.5)], rng.uniform(0.2, 0.8) * u])
+        else:            # both on the boundary, complementary
+            s[o:o + d] = rng.uniform(0.5, 2.0) * np.concatenate([[1.0], u])
+            z[o:o + d] = rng.uniform(0.5, 2.0) * np.concatenate([[1.0], -u])
+        o += d
+    b = A @ x + s
+    q = -(Pf @ x) - A.T @ z
+    return Problem(Pu, np.asarray(q), A, np.asarray(b), zero, nonneg, soc_dims,
+                   name=f"planted-n{n}-s{seed}", meta={"x": x, "s": s, "z": z})
+
+
+def planted_suite():
+    """The planted-optimum instances of the tight-tolerance parity tier: small
+    sparse and dense P, a 400-variable instance with 20 SOC blocks, and a
+    1000-variable tridiagonal-P instance whose KKT factorization has fronts of
+    several hundred rows."""
+    return [
+        planted(0),
+        planted(1, p_kind="dense"),
+        planted(2, n=400, zero=30, nonneg=300, soc_dims=[3] * 20 + [10, 20]),
+        planted(3, n=120, zero=0, nonneg=150, soc_dims=[]),
+        planted(4, n=60, zero=5, nonneg=0, soc_dims=[3, 4, 5, 6, 7, 8, 9, 10], p_kind="dense"),
+        planted(7, n=1000, zero=100, nonneg=800, soc_dims=[3] * 80 + [50] * 2, p_kind="tridiag"),
+    ]
+
+
 def kat_problems():
     """(name, Problem, expected_status, expected_x or None, expected_obj or None, tol)."""
     out = []

@@ -524,3 +524,54 @@ def test_extended_precision_oracle_tight_optimum(oracle):
     f = oracle.solve(p.P, p.q, p.A, p.b, p.zero_dim, p.nonneg_dim, p.soc_dims)
     assert f["status"] == 1
     assert np.abs(f["x"] - x).max() / max(1.0, np.abs(x).max()) < 1e-5
+
+
+# ----------------------------------------------------------------------------
+# planted optimum: strictly complementary instances whose optimum is fixed in
+# closed form before q and b are formed (problems.planted); at tolerance 1e-11
+# both oracle builds must land on it
+# ----------------------------------------------------------------------------
+PLANTED_TIGHT = dict(tol_feas=1e-11, tol_gap_abs=1e-11, tol_gap_rel=1e-11)
+
+
+def _planted_objective(p):
+    x = p.meta["x"]
+    return float(0.5 * x @ (p.P_full() @ x) + p.q @ x)
+
+
+@pytest.mark.parametrize("p", PR.planted_suite(), ids=lambda p: p.name)
+def test_oracle_recovers_planted_optimum(oracle, p):
+    """x*, s*, z* within 1e-9 and the objective within 1e-11 relative, in FP64
+    and in extended precision, with the default and the regularizing backend
+    (measured: 1e-13 to 3.5e-10)."""
+    obj = _planted_objective(p)
+    for r in (oracle.solve(p.P, p.q, p.A, p.b, p.zero_dim, p.nonneg_dim, p.soc_dims, **PLANTED_TIGHT),
+              oracle.solve(p.P, p.q, p.A, p.b, p.zero_dim, p.nonneg_dim, p.soc_dims, linear_solver=2,
+                           **PLANTED_TIGHT),
+              oracle.solve_xp(p.P, p.q, p.A, p.b, p.zero_dim, p.nonneg_dim, p.soc_dims, **PLANTED_TIGHT)):
+        assert r["status"] == 1
+        for k in ("x", "s", "z"):
+            v = np.asarray(r[k], dtype=np.float64)
+            assert np.abs(v - p.meta[k]).max() / max(1.0, np.abs(p.meta[k]).max()) < 1e-9, k
+        assert abs(float(r["objective"]) - obj) / max(1.0, abs(obj)) < 1e-11
+
+
+def test_planted_construction_is_optimal_and_strictly_complementary():
+    """The planted point satisfies the KKT conditions exactly (up to rounding)
+    and every cone pair is strictly complementary: s + z in the interior."""
+    for p in PR.planted_suite():
+        x, s, z = p.meta["x"], p.meta["s"], p.meta["z"]
+        c = PR.kkt_certificate(p, x, s, z)
+        assert c["stationarity"] < 1e-12 and c["feasibility"] < 1e-12 and c["complementarity"] < 1e-12, c
+        w = s + z
+        nn = slice(p.zero_dim, p.zero_dim + p.nonneg_dim)
+        assert (w[nn] > 0.4).all()
+        o = p.zero_dim + p.nonneg_dim
+        for d in p.soc_dims:
+            assert w[o] - np.linalg.norm(w[o + 1:o + d]) > 0.1
+            o += d
+    # with both s and z on the boundary of one SOC block the iterates approach
+    # the optimum only as sqrt(mu): the suite leaves such pairs out
+    p = PR.planted(0, boundary_pairs=True)
+    c = PR.kkt_certificate(p, p.meta["x"], p.meta["s"], p.meta["z"])
+    assert c["complementarity"] < 1e-12
