@@ -304,10 +304,24 @@ __device__ __forceinline__ void lstep_a(T (&a)[64], double sg, int mode, double 
   pin(s.l);  // the division stays in the region of the previous step's far FMAs
 }
 
+// B of step K also publishes column K of L: the multipliers go to the
+// block's LDS rows (A, stride LDSP), and every fourth step prog becomes K + 1,
+// so a wave running the forward substitution of the rows below the block
+// (which needs row i of L complete, i.e. steps 0..i-1) can follow the
+// factorization a few rows behind.  Both are LDS stores of this one wave,
+// which the LDS executes in issue order, so a reader that sees prog = K + 1
+// sees columns 0..K; the signal fence keeps the compiler from reordering the
+// two stores (a release store would also wait for this wave's LDS reads in
+// flight, which the software pipelining keeps busy).
 template <typename T, int K>
-__device__ __forceinline__ void lstep_b(T (&a)[64], int lane, const LStep<T>& s, T* colk) {
+__device__ __forceinline__ void lstep_b(T (&a)[64], int lane, const LStep<T>& s, T* colk, T* A, int* prog) {
   a[K] = (lane == K) ? s.d : (lane > K ? s.l : a[K]);
   pin(a[K]);
+  A[lane * LDSP + K] = a[K];  // (lanes above the diagonal write entries nobody reads)
+  if constexpr ((K & 3) == 3) {
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    if (lane == 0) __hip_atomic_store(prog, K + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
   if constexpr (K + 1 < 64) a[K + 1] -= s.l * s.c1, pin(a[K + 1]);
   if constexpr (K + 2 < 64) a[K + 2] -= s.l * s.c2, pin(a[K + 2]);
   if constexpr (K + 3 < 64) colk[(K & 1) * 64 + lane] = s.c;
@@ -320,7 +334,8 @@ __device__ __forceinline__ void lstep_b(T (&a)[64], int lane, const LStep<T>& s,
 // chunk of its own step's far columns for the next group.
 template <typename T, int K>
 __device__ __forceinline__ void lstep_group(T (&a)[64], double sg, int lane, int mode, double eps, double delta,
-                                            PivotAcc& pa, T* colk, LStep<T>& prev, T (&v0)[16]) {
+                                            PivotAcc& pa, T* colk, LStep<T>& prev, T (&v0)[16], T* A,
+                                            int* prog) {
   constexpr int J = K + 2;  // first far column of step K-1
   const T* cb = colk + ((K - 1) & 1) * 64;
   LStep<T> cur;
@@ -344,7 +359,7 @@ __device__ __forceinline__ void lstep_group(T (&a)[64], double sg, int lane, int
       }
     }
   }
-  lstep_b<T, K>(a, lane, cur, colk);
+  lstep_b<T, K>(a, lane, cur, colk, A, prog);
   prev = cur;
   if constexpr (K + 3 < 64) {
     const T* nb = colk + (K & 1) * 64;
@@ -358,31 +373,34 @@ __device__ __forceinline__ void lstep_group(T (&a)[64], double sg, int lane, int
 template <typename T, int... Ks>
 __device__ __forceinline__ void lstep_groups(std::integer_sequence<int, Ks...>, T (&a)[64], double sg, int lane,
                                              int mode, double eps, double delta, PivotAcc& pa, T* colk,
-                                             LStep<T>& prev, T (&v0)[16]) {
-  (lstep_group<T, Ks + 1>(a, sg, lane, mode, eps, delta, pa, colk, prev, v0), ...);
+                                             LStep<T>& prev, T (&v0)[16], T* A, int* prog) {
+  (lstep_group<T, Ks + 1>(a, sg, lane, mode, eps, delta, pa, colk, prev, v0, A, prog), ...);
 }
 
 template <typename T>
 __device__ __forceinline__ void ldl_full(T (&a)[64], double sg, int lane, int mode, double eps, double delta,
-                                         PivotAcc& pa, T* colk) {
+                                         PivotAcc& pa, T* colk, T* A, int* prog) {
   LStep<T> prev;
   T v0[16];
   lstep_a<T, 0>(a, sg, mode, eps, delta, pa, prev);
-  lstep_b<T, 0>(a, lane, prev, colk);
+  lstep_b<T, 0>(a, lane, prev, colk, A, prog);
 #pragma unroll
   for (int q = 0; q < 16; ++q) v0[q] = colk[3 + q];
   __builtin_amdgcn_sched_barrier(0);
-  lstep_groups<T>(std::make_integer_sequence<int, 63>{}, a, sg, lane, mode, eps, delta, pa, colk, prev, v0);
+  lstep_groups<T>(std::make_integer_sequence<int, 63>{}, a, sg, lane, mode, eps, delta, pa, colk, prev, v0, A,
+                  prog);
 }
 
 // The same partial LDL^T as ldl_lds for n <= 64 rows, by one wave with no
 // workgroup barrier: lane i keeps row i in registers, pivot k is read from
 // lane k, and the unscaled column k goes through LDS (colk) so that every lane
 // reads entry j of it as a broadcast.  Called by the first wave only (threads
-// 0..63); the caller synchronizes the workgroup before and after.
+// 0..63); the caller synchronizes the workgroup before and after.  A full
+// block (n = nc = 64) publishes each column of L as it is computed (prog =
+// number of finished steps, in LDS; see lstep_b).
 template <typename T>
 __device__ __forceinline__ void ldl_wave(T* A, int n, int nc, const double* sgn, int mode, double eps, double delta,
-                         int* stats, bool count, T* colk) {
+                         int* stats, bool count, T* colk, int* prog) {
   const int lane = threadIdx.x & 63;
   const double sg = lane < nc ? sgn[lane] : 1.0;  // pivot signs, read from lane k at step k
   // each path loads and stores the block itself: no register array lives
@@ -393,7 +411,7 @@ __device__ __forceinline__ void ldl_wave(T* A, int n, int nc, const double* sgn,
 #pragma unroll
     for (int j = 0; j < 64; ++j) a[j] = (j <= lane) ? A[lane * LDSP + j] : T(0);
     PivotAcc pa;
-    ldl_full<T>(a, sg, lane, mode, eps, delta, pa, colk);
+    ldl_full<T>(a, sg, lane, mode, eps, delta, pa, colk, A, prog);
     if (count && lane == 0) {
       if (pa.zero) atomicOr(&stats[0], 1);
       if (pa.nreg) atomicAdd(&stats[1], pa.nreg);
@@ -416,11 +434,13 @@ __device__ __forceinline__ void ldl_wave(T* A, int n, int nc, const double* sgn,
 // Forward substitution with the unit-lower part of a 64 x 64 block held
 // row-major in LDS (stride LDSP), one right-hand side per lane:
 // y_i = b_i - sum_{t<i} L(i, t) y_t, four accumulator chains per row (t mod 4).
-// The row entries L(i, t), which do not depend on y, are read in chunks of 16
-// (row i takes ceil(i/16) chunks), each chunk's reads issued before the FMAs
+// The row entries L(i, t), which do not depend on y, are read in chunks of IC
+// (row i takes ceil(i/IC) chunks), each chunk's reads issued before the FMAs
 // of the previous one, across row boundaries; chunk indices, rows and register
-// indices are compile-time constants.
-constexpr int inv_row_chunks(int i) { return (i + 15) / 16; }
+// indices are compile-time constants.  Chunks of 8 keep the substitution wave
+// within the register file while it waits for rows of L (16 spilled there).
+constexpr int IC = 8;  // entries of a row of L per chunk
+constexpr int inv_row_chunks(int i) { return (i + IC - 1) / IC; }
 constexpr int inv_chunk_row(int c) {
   int i = 1;
   while (c >= inv_row_chunks(i)) c -= inv_row_chunks(i++);
@@ -437,40 +457,51 @@ constexpr int inv_nchunks() {
   return n;
 }
 
-template <typename T, int C>
-__device__ __forceinline__ void inv_chunk(const T* L, const T* b, T (&y)[64], T (&acc)[4], T (&cb)[2][16]) {
-  constexpr int I = inv_chunk_row(C), G = inv_chunk_grp(C), T0 = 16 * G;
+template <typename T, int C, typename W>
+__device__ __forceinline__ void inv_chunk(const T* L, const T* b, T (&y)[64], T (&acc)[4], T (&cb)[2][IC],
+                                          W& wait_row) {
+  constexpr int I = inv_chunk_row(C), G = inv_chunk_grp(C), T0 = IC * G;
   if constexpr (G == 0) {
     acc[0] = b[I];
     acc[1] = acc[2] = acc[3] = T(0);
   }
   if constexpr (C + 1 < inv_nchunks()) {
-    constexpr int I1 = inv_chunk_row(C + 1), T1 = 16 * inv_chunk_grp(C + 1);
+    constexpr int I1 = inv_chunk_row(C + 1), T1 = IC * inv_chunk_grp(C + 1);
+    if constexpr (I1 != I) {
+      // the accumulators must exist before the wait: otherwise the compiler
+      // sinks the arithmetic below every wait and holds all of L's reads
 #pragma unroll
-    for (int q = 0; q < 16; ++q)
+      for (int j = 0; j < 4; ++j) pin(acc[j]);
+      wait_row(I1);
+    }
+#pragma unroll
+    for (int q = 0; q < IC; ++q)
       if (T1 + q < I1) cb[(C + 1) & 1][q] = L[I1 * LDSP + T1 + q];
   }
   __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-  for (int q = 0; q < 16; ++q)
+  for (int q = 0; q < IC; ++q)
     if (T0 + q < I) acc[(T0 + q) & 3] -= cb[C & 1][q] * y[T0 + q];
   __builtin_amdgcn_sched_barrier(0);
-  if constexpr (T0 + 16 >= I) y[I] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+  if constexpr (T0 + IC >= I) y[I] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
 }
 
-template <typename T, int... Cs>
+template <typename T, typename W, int... Cs>
 __device__ __forceinline__ void inv_chunks(std::integer_sequence<int, Cs...>, const T* L, const T* b, T (&y)[64],
-                                           T (&acc)[4], T (&cb)[2][16]) {
-  (inv_chunk<T, Cs>(L, b, y, acc, cb), ...);
+                                           T (&acc)[4], T (&cb)[2][IC], W& wait_row) {
+  (inv_chunk<T, Cs>(L, b, y, acc, cb, wait_row), ...);
 }
 
-// b: the lane's right-hand side (64 entries, LDS)
-template <typename T>
-__device__ __forceinline__ void unit_lower_solve(const T* L, const T* b, T (&y)[64]) {
-  T acc[4], cb[2][16];
+// b: the lane's right-hand side (64 entries, LDS).  wait_row(i) returns once
+// row i of L is complete in LDS (it is called before the first read of each
+// row, rows in increasing order).
+template <typename T, typename W>
+__device__ __forceinline__ void unit_lower_solve(const T* L, const T* b, T (&y)[64], W& wait_row) {
+  T acc[4], cb[2][IC];
   y[0] = b[0];
+  wait_row(1);
   cb[0][0] = L[1 * LDSP + 0];  // chunk 0: row 1, t = 0
-  inv_chunks<T>(std::make_integer_sequence<int, inv_nchunks()>{}, L, b, y, acc, cb);
+  inv_chunks<T>(std::make_integer_sequence<int, inv_nchunks()>{}, L, b, y, acc, cb, wait_row);
 }
 
 // ---------------------------------------------------------------------------
@@ -582,6 +613,7 @@ __global__ __launch_bounds__(256) void k_panel(
     T dinv[NB];
     T L11[NB * LDSP];
     T X[PANEL_ROWS * LDSP];
+    int prog;  // finished LDL^T steps of a full diagonal block
   };
   __shared__ Lds sm;
   T* colk = sm.colk;
@@ -594,6 +626,7 @@ __global__ __launch_bounds__(256) void k_panel(
   const int tid = threadIdx.x;
   const bool owner = Tk.r0 == a + w;
   const int rows = Tk.r1 - Tk.r0;
+  if (tid == 0) sm.prog = 0;
   {
     // lane = row, wave g = columns g, g+4, ...: all 32 loads of a thread are
     // issued from clamped (always valid, lower-triangle) addresses before any
@@ -617,7 +650,41 @@ __global__ __launch_bounds__(256) void k_panel(
     }
   }
   __syncthreads();
-  if (tid < 64) ldl_wave(L11, w, w, sgn + first + a, mode, eps, delta, stats, owner, colk);
+  // Wave 1 runs the forward substitution of the chunk's rows (below) while
+  // wave 0 factors the diagonal block: a full 64-column block publishes each
+  // row of L as soon as it is final and the substitution follows a row
+  // behind; a narrower block publishes all rows at the end.  The arithmetic
+  // is that of the sequential order.
+  // (the wave index in a scalar register: uniform branches, so the two
+  // waves' register sets are allocated as separate paths)
+  const int wvu = __builtin_amdgcn_readfirstlane(tid >> 6);
+  if (wvu == 0) {
+    ldl_wave(L11, w, w, sgn + first + a, mode, eps, delta, stats, owner, colk, &sm.prog);
+    if (tid == 0) __hip_atomic_store(&sm.prog, NB, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+  } else if (wvu == 1 && rows > 0) {
+    const int lane = tid & 63;
+    int seen = 0;
+    auto wait_row = [&](int i) __attribute__((always_inline)) {
+      if (i > seen) {
+        // wave 0 always runs to its final store of NB; the spin limit only
+        // bounds a broken invariant.  The count is read into a scalar
+        // register: a wave-uniform loop.
+        int v;
+        for (int spins = 0; spins < (int)SPIN_LIMIT; ++spins) {
+          v = __builtin_amdgcn_readfirstlane(
+              __hip_atomic_load(&sm.prog, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
+          if (v >= i) break;
+          __builtin_amdgcn_s_sleep(1);
+        }
+        seen = v;
+      }
+    };
+    T y[64];
+    unit_lower_solve(L11, X + lane * LDSP, y, wait_row);
+    __builtin_amdgcn_wave_barrier();  // the lane's right-hand side is read: its row takes the result
+#pragma unroll
+    for (int k = 0; k < 64; ++k) X[lane * LDSP + k] = y[k];
+  }
   __syncthreads();
   if (owner) {
     // other workgroups of this launch may still be reading the unfactored
@@ -638,14 +705,7 @@ __global__ __launch_bounds__(256) void k_panel(
   // badly scaled diagonal blocks of late interior-point iterations the
   // inverse loses accuracy that the refinement cannot recover.  Columns at or
   // past w (and lanes past the chunk's rows) compute unused values.
-  if (tid < 64) {
-    T y[64];
-    unit_lower_solve(L11, X + tid * LDSP, y);
-    __builtin_amdgcn_wave_barrier();  // the lane's right-hand side is read: its row takes the result
-#pragma unroll
-    for (int k = 0; k < 64; ++k) X[tid * LDSP + k] = y[k];
-  }
-  __syncthreads();
+
   // column by column, lanes along the rows: coalesced stores by all waves
   for (int idx = tid; idx < rows * w; idx += blockDim.x) {
     const int k = idx / rows, rr = idx - k * rows;
