@@ -40,6 +40,7 @@ constexpr int NB = 64;        // sub-panel width
 // largest fronts, where the pass count over the trailing triangle dominates
 inline int panel_width(int nr) { return nr > 32768 ? 1024 : 512; }
 constexpr int TM = 128;       // trailing-update tile
+constexpr long SMALL_TRAIL_TILES = 512;  // below this many 128-tiles a launch uses 64-tiles
 constexpr int KC = 32;        // K chunk staged in LDS per step
 constexpr int SB = 64;        // solve block
 constexpr int LDSP = NB + 1;  // padded LDS row stride for 64x64 blocks
@@ -757,20 +758,24 @@ struct Mfma<float> {
   static __device__ inline int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
-// WN: waves across the 128 tile columns (2: four waves of 64 x 64; 4: eight
-// waves of 64 x 32, half the accumulator registers per wave and twice the
-// waves to hide latency).  Two wave rows of 64.
+// TT: square tile (128, or 64 for launches with too few 128-tiles to fill the
+// GPU: four times the workgroups on the same entries, each entry's
+// accumulation in the same order).  WN: waves across the tile columns (TT =
+// 128: 2 gives four waves of 64 x 64, 4 eight waves of 64 x 32, half the
+// accumulator registers per wave and twice the waves to hide latency; TT =
+// 64: 4 gives four waves of 64 x 16).  TT / 64 wave rows of 64.
 // (the second launch bound is the minimum number of waves per SIMD)
-template <typename T, int WN>
-__global__ __launch_bounds__(128 * WN, (WN == 4 || sizeof(T) == 4) ? 4 : 2) void k_trail(
+template <typename T, int WN, int TT = TM>
+__global__ __launch_bounds__(TT * WN, (WN == 4 || sizeof(T) == 4) ? 4 : 2) void k_trail(
     const TrailTask* __restrict__ tasks, const long* __restrict__ fofs, const int* __restrict__ nrv,
     const int* __restrict__ firstv, const double* __restrict__ Dg, T* F, const int* stats) {
   if (factor_stopped(stats)) return;
   typedef typename Mfma<T>::acc acc_t;
-  constexpr int NT = 128 * WN;          // threads
-  constexpr int NI = 8 / WN;            // 16-column MFMA blocks per wave
-  constexpr int LDA = TM + 4;
-  constexpr int PER = KC * TM / NT;     // staged elements per thread and operand
+  constexpr int NT = TT * WN;           // threads
+  constexpr int NI = TT / 16 / WN;      // 16-column MFMA blocks per wave
+  static_assert(NI >= 1, "a wave covers at least one 16-column block");
+  constexpr int LDA = TT + 4;
+  constexpr int PER = KC * TT / NT;     // staged elements per thread and operand
   __shared__ T As[KC * LDA];
   __shared__ T Bs[KC * LDA];
   __shared__ T Ds[KC];
@@ -798,7 +803,7 @@ __global__ __launch_bounds__(128 * WN, (WN == 4 || sizeof(T) == 4) ? 4 : 2) void
   // flight together while the MFMAs run.  D of the chunk goes to LDS too and
   // scales the B fragments as they are read.
   constexpr int VR = 16 / (int)sizeof(T);  // rows per load
-  constexpr int RG = TM / VR;              // row groups of a column
+  constexpr int RG = TT / VR;              // row groups of a column
   constexpr int CS = NT / RG;              // column stride of a thread's loads
   constexpr int NQ = KC / CS;              // loads per operand and chunk
   static_assert(NQ * VR == PER, "staging covers the chunk");
@@ -860,7 +865,7 @@ __global__ __launch_bounds__(128 * WN, (WN == 4 || sizeof(T) == 4) ? 4 : 2) void
   // checks: one address per row block and column, the lane's four rows at
   // fixed offsets from it; in FP32 they are consecutive rows, read and
   // written as one 16-byte vector.
-  const bool interior = Tk.ti + TM <= nr && Tk.tj + TM <= Tk.chi && Tk.ti >= Tk.tj + TM && Tk.ti >= Tk.rlo;
+  const bool interior = Tk.ti + TT <= nr && Tk.tj + TT <= Tk.chi && Tk.ti >= Tk.tj + TT && Tk.ti >= Tk.rlo;
   if (interior) {
 #pragma unroll
     for (int mi = 0; mi < 4; ++mi) {
@@ -1541,21 +1546,29 @@ void DeviceKKT::build_schedule() {
     for (int i = 0; i < P.nr(s) - P.sn_ncol[s]; ++i)
       rmap[rmap_ofs[s] + i] = (int)(std::lower_bound(pb, pe, P.rows[P.row_ptr[s] + P.sn_ncol[s] + i]) - pb);
   }
-  auto push_trail = [&](int s, int ti, int tj, int k0, int k1, int rlo, int chi, int nr, double& fl) {
+  auto push_trail = [&](int s, int ti, int tj, int k0, int k1, int rlo, int chi, int nr, int tt, double& fl) {
     trail_h_.push_back({s, ti, tj, k0, k1, rlo, chi});
     // algorithmic flops: 2 * K * (#updated lower elements in the tile)
     double cnt = 0;
-    int jmax = std::min(tj + TM, chi);
+    int jmax = std::min(tj + tt, chi);
     if (ti >= jmax - 1 && ti >= rlo) {  // tile entirely on or below the diagonal
-      fl += 2.0 * (k1 - k0) * (double)(jmax - tj) * (double)std::max(0, std::min(ti + TM, nr) - ti);
+      fl += 2.0 * (k1 - k0) * (double)(jmax - tj) * (double)std::max(0, std::min(ti + tt, nr) - ti);
       return;
     }
     for (int j = tj; j < jmax; ++j) {
       int i0 = std::max({ti, j, rlo});
-      int i1 = std::min(ti + TM, nr);
+      int i1 = std::min(ti + tt, nr);
       if (i1 > i0) cnt += i1 - i0;
     }
     fl += 2.0 * (k1 - k0) * cnt;
+  };
+  // tiles of a rectangle [r0, nr) x [c0, c1) of the lower triangle
+  auto count_tiles = [](int r0, int nr, int c0, int c1, int tt) {
+    long n = 0;
+    for (int ti = r0; ti < nr; ti += tt)
+      for (int tj = c0; tj < c1; tj += tt)
+        if (ti + tt - 1 >= tj) ++n;
+    return n;
   };
   for (int l = 0; l < P.nlevels; ++l) {
     // assembly
@@ -1609,6 +1622,21 @@ void DeviceKKT::build_schedule() {
       int tbeg = (int)trail_h_.size();
       double fl = 0;
       const int rbeg = (int)panel_h_.size();
+      // the launch's tile: 128 unless its 128-tiles would leave most of the
+      // GPU idle (fewer than one per CU)
+      long n128 = 0;
+      for (int s : bigs) {
+        int nc = P.sn_ncol[s], nr = P.nr(s);
+        if (a0 >= nc) continue;
+        const int pw = panel_width(nr);
+        const int P0 = (a0 / pw) * pw;
+        const int pend = std::min(P0 + pw, nc), b = a0 + std::min(NB, pend - a0);
+        if (b < pend)
+          n128 += count_tiles(b, nr, b, pend, TM);
+        else if (pend < nr)
+          n128 += count_tiles(pend, nr, pend, nr, TM);
+      }
+      const int tt = n128 < SMALL_TRAIL_TILES ? TM / 2 : TM;
       for (int s : bigs) {
         int nc = P.sn_ncol[s], nr = P.nr(s);
         if (a0 >= nc) continue;
@@ -1624,18 +1652,18 @@ void DeviceKKT::build_schedule() {
           panel_h_.push_back({s, a0, w, r0, std::min(nr, r0 + PANEL_ROWS), blk});
         int b = a0 + w;
         if (b < pend) {
-          for (int ti = b; ti < nr; ti += TM)
-            for (int tj = b; tj < pend; tj += TM)
-              if (ti + TM - 1 >= tj) push_trail(s, ti, tj, a0, b, b, pend, nr, fl);
+          for (int ti = b; ti < nr; ti += tt)
+            for (int tj = b; tj < pend; tj += tt)
+              if (ti + tt - 1 >= tj) push_trail(s, ti, tj, a0, b, b, pend, nr, tt, fl);
         } else if (pend < nr) {  // panel [P0, pend) complete: update everything right of it
-          for (int ti = pend; ti < nr; ti += TM)
-            for (int tj = pend; tj < nr; tj += TM)
-              if (ti + TM - 1 >= tj) push_trail(s, ti, tj, P0, pend, pend, nr, nr, fl);
+          for (int ti = pend; ti < nr; ti += tt)
+            for (int tj = pend; tj < nr; tj += tt)
+              if (ti + tt - 1 >= tj) push_trail(s, ti, tj, P0, pend, pend, nr, nr, tt, fl);
         }
       }
       if ((int)panel_h_.size() > rbeg) launches_.push_back({L_PANEL, rbeg, (int)panel_h_.size()});
       if ((int)trail_h_.size() > tbeg) {
-        launches_.push_back({L_TRAIL, tbeg, (int)trail_h_.size()});
+        launches_.push_back({tt == TM ? L_TRAIL : L_TRAIL64, tbeg, (int)trail_h_.size()});
         trail_launch_flops_.push_back(fl);
       }
     }
@@ -1737,6 +1765,13 @@ void DeviceKKT::launch_factor(T* F, const double* Kx, double extra_p, double ext
         else
           hipLaunchKernelGGL((k_trail<T, 4>), dim3(n), dim3(512), 0, st_, trail_.p + L.beg, fofs_.p, nr_.p,
                              first_.p, D_.p, F, fstat_.p);
+        tend(T_TRAIL, trail_launch_flops_[tl]);
+        ++tl;
+        break;
+      case L_TRAIL64:
+        tbegin();
+        hipLaunchKernelGGL((k_trail<T, 4, TM / 2>), dim3(n), dim3(TM / 2 * 4), 0, st_, trail_.p + L.beg, fofs_.p,
+                           nr_.p, first_.p, D_.p, F, fstat_.p);
         tend(T_TRAIL, trail_launch_flops_[tl]);
         ++tl;
         break;

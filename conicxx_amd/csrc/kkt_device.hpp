@@ -33,7 +33,7 @@ struct SolveTask { int s, blk; };
 // buffer, update vectors, block flags and done counters at these strides.
 struct SweepCtx { int nctx; long vstride, hstride, ustride, fstride, dstride; };
 
-enum LaunchKind { L_ASSEMBLE = 0, L_SMALL = 1, L_PANEL = 3, L_TRAIL = 4 };
+enum LaunchKind { L_ASSEMBLE = 0, L_SMALL = 1, L_PANEL = 3, L_TRAIL = 4, L_TRAIL64 = 5 };
 struct Launch { int kind, beg, end; };
 
 // Pivot modes.
