@@ -1692,8 +1692,9 @@ void DeviceKKT::tbegin() {
 
 void DeviceKKT::tend(int kind, double work) {
   if (!timing_) return;
-  kraw_[kind].all += 1;
-  kraw_[kind].work += work;
+  KindRaw& r = in_factor_ ? fpend_[kind] : kraw_[kind];
+  r.all += 1;
+  r.work += work;
   if (!sample_) return;
   CX_HIP(hipEventRecord(evpool_[evused_ + 1], st_));
   evpending_.push_back(kind);
@@ -1734,6 +1735,9 @@ void DeviceKKT::launch_factor(T* F, const double* Kx, double extra_p, double ext
                               double delta) {
   size_t tl = 0;
   sample_ = timing_ && fact_seq_++ % timing_every_ == 0;
+  in_factor_ = true;
+  fact_ev0_ = evpending_.size();
+  for (auto& k : fpend_) k = KindRaw{};
   for (const Launch& L : launches_) {
     const int n = L.end - L.beg;
     if (n <= 0) continue;
@@ -1786,6 +1790,7 @@ void DeviceKKT::launch_factor(T* F, const double* Kx, double extra_p, double ext
     CX_LAUNCH_CHECK();
   }
   sample_ = false;
+  in_factor_ = false;
 }
 
 void DeviceKKT::factorize_async(const double* Kx, double extra_p, double extra_a, int mode,
@@ -1808,6 +1813,16 @@ FactorStats DeviceKKT::factor_stats() {
   fstat_.download(st, 3, st_);
   dstat_.download(&mn, 1, st_);
   CX_HIP(hipStreamSynchronize(st_));
+  if (timing_) {
+    if (st[2]) {
+      // stopped at a failing pivot: drop its timed launches
+      evpending_.resize(std::min(evpending_.size(), fact_ev0_));
+      evused_ = 2 * evpending_.size();
+    } else {
+      for (int k = 0; k < T_NKINDS; ++k) kraw_[k].all += fpend_[k].all, kraw_[k].work += fpend_[k].work;
+    }
+    for (auto& k : fpend_) k = KindRaw{};
+  }
   collect_timing();
   fs.zero_pivot = st[0];
   fs.nreg = st[1];

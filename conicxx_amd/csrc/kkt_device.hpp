@@ -154,6 +154,13 @@ class DeviceKKT {
     long timed = 0, all = 0;
   };
   KindRaw kraw_[T_NKINDS];
+  // launches and work of the factorization in flight, added to kraw_ by
+  // factor_stats() only if it ran to completion (a checked factorization that
+  // stops at a failing pivot leaves its remaining launches returning at once:
+  // neither its time nor its work is kept)
+  KindRaw fpend_[T_NKINDS];
+  bool in_factor_ = false;
+  size_t fact_ev0_ = 0;  // first timed launch of that factorization in evpending_
   double sweep_bytes_ = 0;
 };
 

@@ -281,7 +281,8 @@ class Solver:
     def kernel_stats(self):
         """Device time per kernel class of the last solve (CONICXX_KERNEL_TIMING=N:
         HIP events around the launches of every Nth factorization and KKT solve,
-        scaled to all launches)."""
+        scaled to all launches).  Factorizations stopped at a failing pivot are
+        left out with their launches, time and work."""
         out = np.zeros(3 * len(self.KERNEL_CLASSES))
         n = _lib.lib().cx_get_kernel_stats(self._h, out.ctypes.data_as(_lib._f64p), len(self.KERNEL_CLASSES))
         return {self.KERNEL_CLASSES[k]: dict(ms=out[3 * k], launches=int(out[3 * k + 1]), work=out[3 * k + 2])

@@ -157,8 +157,9 @@ void cx_get_profile(const cx_solver* h, double* out13);
  * update (class 0) and bytes for the forward/backward sweeps (classes 4, 5).
  * Classes: 0 trailing update, 1 sub-panel (diagonal block LDL^T + panel solve,
  * and the store of the diagonal blocks), 2 assembly, 3 small fronts, 4 forward
- * sweep, 5 backward sweep.  Returns the number of classes written.  No
- * counterpart in the reference. */
+ * sweep, 5 backward sweep.  A checked factorization that stopped at a failing
+ * pivot is not counted (its later launches return at once).  Returns the
+ * number of classes written.  No counterpart in the reference. */
 int cx_get_kernel_stats(const cx_solver* h, double* out, int max_kinds);
 
 /* Backward (adjoint) pass of the last solve, which must have returned
