@@ -34,8 +34,10 @@ os.environ.setdefault("CONICXX_KERNEL_TIMING", "8")
 
 HBM_PEAK_GBS = 8000.0
 # k_trail HBM bytes per launch on config 1 (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0
-# per the copy calibration, averaged over the launches of setup + one solve)
-K_TRAIL_C1_TRAFFIC_BYTES = 2.41e8
+# per the copy calibration, averaged over the launches of setup + one solve
+# that did work: launches of a factorization stopped at a failing pivot
+# return at once and are left out, as in the timing)
+K_TRAIL_C1_TRAFFIC_BYTES = 3.17e8
 FP64_MFMA_PEAK_TFLOPS = 78.6
 MI355X_CUS = 256  # compute units: one k_batch_solve workgroup is resident on each
 
