@@ -21,7 +21,11 @@ perturbed copies) and requires:
     x*, s*, z* and the objective of the extended-precision oracle solved to
     1e-11 (1e-10 or 1e-9 where the tighter runs stall; x* is known to ~1e-7
     there), with a floor of 2e-7;
-  * for Solved results, the KKT optimality certificate of the GPU solution.
+  * for Solved results, the KKT optimality certificate of the GPU solution;
+  * in every case the objective within 1e-8 and x, s, z within 5e-5 of the
+    nearest same-status oracle run (fixed caps above the envelope).
+The 1e-8 bar on x, s, z itself is tested on instances whose optimum is known
+in closed form and strictly complementary (tests/test_gpu_tight.py).
 """
 import numpy as np
 import pytest
@@ -34,6 +38,10 @@ pytestmark = pytest.mark.gpu
 
 TOL = 1e-8
 NPERT = 3
+# hard cap on x, s, z whatever the envelope says (measured: at most 6.8e-6 over
+# every comparison of the GPU suite at the default tolerance, the oracle's own
+# perturbed runs spreading by up to 1e-5)
+XSZ_CAP = 5e-5
 
 
 def rel(a, b):
@@ -98,6 +106,10 @@ def assert_parity(sol, runs, prob=None):
     print("parity", {k: f"{v:.2e}" for k, v in best.items()},
           "oracle spread", {k: f"{v:.1e}" for k, v in spread.items()},
           "iters gpu", sol.info.iterations, "oracle", [r["iterations"] for r in runs])
+    # the objective at the survey's 1e-8 on every instance (measured at most
+    # 6.2e-10), and x, s, z under a fixed cap
+    assert best["objective"] < TOL, best
+    assert max(best[k] for k in ("x", "s", "z")) < XSZ_CAP, best
     miss = [k for k, v in best.items() if v > max(TOL, 2.0 * spread[k])]
     if miss:
         # accuracy against the optimum instead of closeness to the oracle's cluster
